@@ -1,0 +1,228 @@
+"""Benchmark: the fused BSSN RHS + RK4 step at 256^3 FP64 on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--no-cpu] [--no-twopass]
+
+A step is one full RK4 step (four fused stage kernels: RHS + Kreiss-Oliger + stage update,
+enforcement folded into the last; then the periodic ghost refill) of the 256^3-interior,
+h = 1/16 equal-mass Brill-Lindquist binary (m = 0.5 each, 6M apart, 1e-4 seeded sinusoid
+perturbations on every field), FD order 4 / KO order 6, sigma_KO = 0.1. Each step feeds the next,
+so no input is reused from cache. ``value`` is interior points per second of the whole step
+(Mpts/s). Also reported: the bare fused RHS and the materialised two-pass RHS (138 derivative
+arrays through HBM, then the algebra) on the same state, the roofline position of the fused
+stage kernel (timed live with HIP events on the launch stream), and the CPU oracle timed on
+this host's cores on a bounded 48^3 sample.
+
+Rank 0 prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "integer-schemes_amd"))
+
+METRIC = "Mpts/s BSSN RHS+RK4 at 256^3 FP64 vs INT8-split/batched; % HBM & FP64 peak"
+PEAK_HBM = 8.0e12          # B/s, MI355X HBM3E
+PEAK_FP64 = 78.6e12        # flop/s, MI355X FP64 vector (= FP64 matrix) peak
+
+# Algorithmic work per interior point (order 4 FD, KO order 6), counted from the operators:
+#   stencils: 72 first derivatives x 8 flops (4 taps, multiply + add), 33 diagonal second
+#   derivatives x 10 (5 taps), 33 mixed x 36 (4 x 4 nested taps)              = 2094
+#   algebra: the reference CSE's operation count (2486 mul + 1259 add + 482 sub + 52 div
+#   + 30 for the integer powers, sqrt, exp; vendor/bssneqs_SSL_HD_dxsq.cpp)     = 4309
+#   KO: 24 fields x 3 axes x 7 taps x 2 + 2 adds per field                         = 1056
+#   stage update: 24 fields x ~4 flops                                            =   96
+FLOP_RHS = 2094 + 4309
+FLOP_STAGE = FLOP_RHS + 1056 + 96
+# HBM bytes per interior point per stage launch (24 fields x 8 B per array touched):
+#   stage 1: read y, write Y and S (576); stages 2, 3: read Y, y, S, write Y', S (960);
+#   stage 4: read Y, y, S, write y (768). RHS alone: read 24 + write 24 fields = 384.
+BYTES_STAGES = (576, 960, 960, 768)
+BYTES_RHS = 384
+
+
+def cpu_baseline(seconds: float = 12.0):
+    """The oracle's RK4 step (C, OpenMP on all host cores) on the 48^3 single-puncture
+    workload, repeated for about ``seconds``."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+    from bssn3d_hip import initial_data as idata
+    from bssn3d_hip.state import PhysicsParams, scalars
+
+    grid, st, _ = idata.config_single_puncture(seed=0, N=48)
+    F = st.numpy()
+    dt = 0.25 * grid.dx
+    S16 = scalars(PhysicsParams(), grid.dx, grid.dy, grid.dz, dt, 0.0)
+    O.rk4_step(F, S16, 0.0, dt, order=4, ko_sigma=0.1, ng=3)      # warm (page-in, thread pool)
+    n, t0 = 0, time.perf_counter()
+    U = F
+    while True:
+        U = O.rk4_step(U, S16, n * dt, dt, order=4, ko_sigma=0.1, ng=3)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or O.num_threads()
+    return {"value": round(48 ** 3 * n / el / 1e6, 4), "unit": "Mpts/s (RK4 step)",
+            "cores": threads, "kind": "port",
+            "sample": f"{n} RK4 steps of the 48^3 single-puncture workload (54^3 padded), "
+                      f"FD4/KO6, oracle/bssn_oracle.c, {el:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-twopass", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(the halo-exchange sharding is described in DESIGN.md, not built).",
+              file=sys.stderr)
+        return 2
+
+    import torch
+    from bssn3d_hip import initial_data as idata
+    from bssn3d_hip import _lib
+    from bssn3d_hip.evolve import BSSNEvolution
+    from bssn3d_hip.rhs import rhs_into, rhs_twopass_into, stream_handle
+    from bssn3d_hip.state import PhysicsParams, scalars
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    N = args.n
+    grid, U, _ = idata.config_binary(seed=0, N=N, device=dev)
+    p = PhysicsParams()
+    dt = 0.25 * grid.dx
+    evo = BSSNEvolution(grid, p, order=4, ko_sigma=0.1, bc="periodic", integrator="rk4")
+    npts = N ** 3
+    Sx, Sy, Sz = grid.shape
+    lib = _lib.load()
+
+    def ev():
+        return torch.cuda.Event(enable_timing=True)
+
+    # ---------------------------------------------------------------- RK4 step (the metric)
+    t = 0.0
+    for _ in range(args.warmup):
+        evo.step_(U, t, dt)
+        t += dt
+    torch.cuda.synchronize()
+    e0, e1 = ev(), ev()
+    w0 = time.perf_counter()
+    e0.record()
+    for _ in range(args.steps):
+        evo.step_(U, t, dt)
+        t += dt
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    ms_step = e0.elapsed_time(e1) / args.steps
+    assert bool(torch.isfinite(U).all()), "non-finite state after the timed steps"
+
+    # periodic refill alone, to separate the stage kernels inside the step
+    reps = max(args.steps, 5)
+    s0, s1 = ev(), ev()
+    s0.record()
+    for _ in range(reps):
+        _lib.check(lib.bssn_periodic_sync_f64(U.data_ptr(), Sx, Sy, Sz, grid.ng,
+                                              stream_handle(dev)))
+    s1.record()
+    torch.cuda.synchronize()
+    ms_sync = s0.elapsed_time(s1) / reps
+    ms_stage = (ms_step - ms_sync) / 4.0
+
+    # ---------------------------------------------------------------- bare fused RHS
+    S16 = scalars(p, grid.dx, grid.dy, grid.dz, dt, t)
+    out = torch.empty_like(U)
+    for _ in range(2):
+        rhs_into(U, out, S16, 4)
+    r0, r1 = ev(), ev()
+    r0.record()
+    for _ in range(reps):
+        rhs_into(U, out, S16, 4)
+    r1.record()
+    torch.cuda.synchronize()
+    ms_rhs = r0.elapsed_time(r1) / reps
+
+    # ---------------------------------------------------------------- two-pass RHS
+    ms_twopass = None
+    if not args.no_twopass:
+        D = torch.empty((138, Sx, Sy, Sz), dtype=torch.float64, device=dev)
+        rhs_twopass_into(U, out, D, S16, 4)
+        q0, q1 = ev(), ev()
+        q0.record()
+        for _ in range(reps):
+            rhs_twopass_into(U, out, D, S16, 4)
+        q1.record()
+        torch.cuda.synchronize()
+        ms_twopass = q0.elapsed_time(q1) / reps
+        del D
+
+    # ---------------------------------------------------------------- roofline of the stage kernel
+    flops_launch = FLOP_STAGE * npts
+    bytes_launch = sum(BYTES_STAGES) / 4.0 * npts
+    t_launch = ms_stage * 1e-3
+    hbm_frac = bytes_launch / t_launch / PEAK_HBM
+    fp64_frac = flops_launch / t_launch / PEAK_FP64
+    if bytes_launch / PEAK_HBM >= flops_launch / PEAK_FP64:
+        roof = {"bound": "hbm", "achieved": round(bytes_launch / t_launch / 1e9, 2),
+                "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": round(hbm_frac, 4)}
+    else:
+        roof = {"bound": "mfma", "achieved": round(flops_launch / t_launch / 1e12, 3),
+                "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": round(fp64_frac, 4)}
+    roof["traffic"] = None
+    roof["kernel"] = "k_fused<4,KO,WRAP> (RK4 stage)"
+    roof["fp64_frac"] = round(fp64_frac, 4)
+    roof["hbm_frac"] = round(hbm_frac, 4)
+    roof["ms_per_launch"] = round(ms_stage, 4)
+
+    value = npts / (ms_step * 1e-3) / 1e6
+    res = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mpts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"bssn_rk4_step_{N}^3_fp64_bl_binary", "interior": N,
+                   "padded": Sx, "h": grid.dx, "fd_order": 4, "ko_order": 6, "ko_sigma": 0.1,
+                   "integrator": "rk4", "masses": [0.5, 0.5], "separation": 6.0,
+                   "bc": "periodic"},
+        "wall_s_timed": round(wall, 4),
+        "rhs_ms": round(ms_rhs, 4),
+        "rhs_mpts_s": round(npts / (ms_rhs * 1e-3) / 1e6, 2),
+        "sync_ms": round(ms_sync, 4),
+        "roofline": roof,
+    }
+    if ms_twopass is not None:
+        res["twopass_rhs_ms"] = round(ms_twopass, 4)
+        res["twopass_rhs_mpts_s"] = round(npts / (ms_twopass * 1e-3) / 1e6, 2)
+        res["fused_vs_twopass_rhs"] = round(ms_twopass / ms_rhs, 3)
+    del out, U
+    torch.cuda.empty_cache()
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    else:
+        res["cpu_baseline"] = None
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,118 @@
+/*
+ * bssn_hip -- C ABI of the MI355X (gfx950) BSSN right-hand side and time-step library.
+ *
+ * Every entry point takes device pointers, plain sizes and a HIP stream (hipStream_t passed as
+ * void*, 0 for the null stream), enqueues its kernels on that stream and returns 0 on success.
+ * A nonzero return is an error code (BSSN_E_*); bssn_last_error() then holds a message for the
+ * calling thread. The library never allocates device memory: workspaces are passed in, sized
+ * by the *_workspace_bytes queries. Entry points are re-entrant; calls that share buffers on one
+ * stream are ordered by that stream.
+ *
+ * Layouts (the reference's):
+ *   state  F[f*P + (x*Sy + y)*Sz + z], P = Sx*Sy*Sz, float64, 24 fields in the order
+ *          alpha, chi, K, gt0..5, beta0..2, At0..5, Gt0..2, B0..2
+ *          (3D/src/bssn3d/state.py:26-44; index formula gen_fused.py:210-211).
+ *   S16    host array of 16 doubles, the reference's scalar pack
+ *          (3D/src/bssn3d/fused_rhs_cuda.py:52-55, gen_fused.py:58-60):
+ *          eta, C_CAHD, dt, dx_i, h_ssl, sig_ssl, t, lambda0..3, lambda_f0..1, dx, dy, dz.
+ *   D      derivative bundle, 138 arrays: d = 3*f + axis (first derivatives of all 24 fields),
+ *          d = 72 + 6*s + pair (second derivatives; s over alpha, chi, gt0..5, beta0..2;
+ *          pair over 00,01,02,11,12,22; mixed = d1_j(d1_i f)).
+ * fd_order is 4 or 6; the Kreiss-Oliger order is fd_order + 2 (the reference's pairing,
+ * common/src/mcs_common/derivatives.py:55-72).
+ */
+#ifndef BSSN_HIP_H
+#define BSSN_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BSSN_HIP_ABI_VERSION 1
+
+#define BSSN_OK 0
+#define BSSN_E_ARG 1   /* bad shape, order, pointer or parameter */
+#define BSSN_E_HIP 2   /* a HIP runtime error (launch or memory) */
+
+int bssn_abi_version(void);
+const char* bssn_last_error(void);
+
+/* Fused right-hand side at every padded point, stencils edge-padded at the array ends.
+ * Replaces the reference's XLA FFI handler Fused(stream, F, S) -> OUT
+ * (3D/src/bssn3d/cuda/rhs_fused.cu:22-61, bound in fused_rhs_cuda.py:41-72), which is
+ * what BSSNSolver(scheme="cuda_fused").rhs calls (3D/src/bssn3d/rhs.py:112-122).
+ * ko_sigma = 0 gives the bare RHS (the reference handler); ko_sigma > 0 also adds the
+ * Kreiss-Oliger term of BSSNEvolution.rhs (3D/src/bssn3d/evolve.py:86-90). */
+int bssn_rhs_f64(const double* F, const double* S16, double* OUT, int Sx, int Sy, int Sz,
+                 int fd_order, double ko_sigma, void* stream);
+
+/* Two-pass RHS, pass 1: the 138-derivative bundle at every padded point.
+ * Replaces derivative_bundle (3D/src/bssn3d/derivative_bundle.py:31-48) and the device
+ * derivative stage deriv_kernel FFI (3D/src/bssn3d/cuda/deriv_2p5d.cu:30-113).
+ * h3 = host array {dx, dy, dz}. */
+int bssn_deriv_bundle_f64(const double* F, double* D, int Sx, int Sy, int Sz, int fd_order,
+                          const double* h3, void* stream);
+
+/* Two-pass RHS, pass 2: the pointwise algebra over npts points of F (24 x npts) and
+ * D (138 x npts). Replaces bssn_rhs_algebra (3D/src/bssn3d/_bssn_rhs_generated.py) and the
+ * algebra-only kernel bssn_algebra (3D/src/bssn3d/cuda/gen_algebra_cuda.py:55-96). */
+int bssn_algebra_f64(const double* F, const double* D, const double* S16, double* OUT,
+                     long npts, void* stream);
+
+/* OUT += Kreiss-Oliger dissipation of all 24 fields, edge-padded (periodic = 0) or wrapped
+ * over the interior (periodic = 1, ng ghost cells). Replaces BSSNEvolution._ko
+ * (3D/src/bssn3d/evolve.py:76-84). */
+int bssn_ko_add_f64(const double* F, double* OUT, int Sx, int Sy, int Sz, int fd_order,
+                    double sigma, const double* h3, int periodic, int ng, void* stream);
+
+/* Periodic ghost fill. Replaces BSSNEvolution._periodic_sync (3D/src/bssn3d/evolve.py:65-73). */
+int bssn_periodic_sync_f64(double* F, int Sx, int Sy, int Sz, int ng, void* stream);
+
+/* det(gt) = 1, tr(At) = 0, chi/alpha floors over npts points.
+ * Replaces BSSNEvolution.enforce (3D/src/bssn3d/evolve.py:93-119). */
+int bssn_enforce_f64(double* F, long npts, double chi_floor, double alpha_floor, void* stream);
+
+/* Workspace for one step: integrator 0 = RK4 (3 states), 1 = MSRK rk4_2_2 (2 states). */
+size_t bssn_step_workspace_bytes(int integrator, int Sx, int Sy, int Sz);
+
+/* One classic RK4 step of U in place, four fused stage launches, enforcement folded into the
+ * last. periodic = 1: every RHS sees periodically wrapped data (the reference's sync before each
+ * RHS), interior points are advanced and the ghosts of U are refilled by a periodic sync at the
+ * end. periodic = 0: all padded points are advanced with edge-padded stencils and ghosts are not
+ * synced (the reference's bc != "periodic"). S16's dt and t entries are replaced by dt and the
+ * stage times. Replaces BSSNEvolution.step (3D/src/bssn3d/evolve.py:125-132). */
+int bssn_rk4_step_f64(double* U, double* work, int Sx, int Sy, int Sz, const double* S16,
+                      int fd_order, double ko_sigma, int ng, int periodic, double t, double dt,
+                      double chi_floor, double alpha_floor, void* stream);
+
+/* One MSRK rk4_2_2 step of U in place. fprev holds f(y_{n-1}) on entry and is clobbered;
+ * fnext receives f(y_n) (swap the two between steps). Replaces BSSNEvolution._msrk2_step
+ * (3D/src/bssn3d/evolve.py:139-153). */
+int bssn_msrk_step_f64(double* U, double* fprev, double* fnext, double* work, int Sx, int Sy,
+                       int Sz, const double* S16, int fd_order, double ko_sigma, int ng,
+                       int periodic, double t, double dt, double chi_floor, double alpha_floor,
+                       void* stream);
+
+/* The stage function f(y, t) of the steppers: RHS + KO, periodic wrap or edge padding.
+ * Replaces BSSNEvolution.rhs (3D/src/bssn3d/evolve.py:86-90); used for the MSRK start value. */
+int bssn_stage_rhs_f64(const double* Y, double* OUT, int Sx, int Sy, int Sz, const double* S16,
+                       int fd_order, double ko_sigma, int ng, int periodic, void* stream);
+
+/* Hamiltonian and momentum constraints at every padded point: HM = 4 arrays (H, M0, M1, M2).
+ * Replaces ConstraintSolver.constraints (3D/src/bssn3d/constraints.py:38-51). */
+int bssn_constraints_f64(const double* F, double* HM, int Sx, int Sy, int Sz, int fd_order,
+                         const double* h3, void* stream);
+
+/* Interior RMS norms {H_l2, M_l2} into the host array out2; synchronises the stream.
+ * HM is a 4-array workspace, partial holds bssn_l2_partial_doubles() doubles.
+ * Replaces ConstraintSolver.l2 (3D/src/bssn3d/constraints.py:53-62). */
+long bssn_l2_partial_doubles(int Sx, int Sy, int Sz, int ng);
+int bssn_constraint_l2_f64(const double* F, double* HM, double* partial, int Sx, int Sy, int Sz,
+                           int fd_order, const double* h3, int ng, double* out2, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,86 @@
+"""ctypes binding of libbssn_hip.so (the C ABI declared in include/bssn_hip.h).
+
+The library is built in-tree by integer-schemes_amd/build.py. A missing library raises
+FileNotFoundError naming the build command, as the reference does for its FFI library
+(3D/src/bssn3d/fused_rhs_cuda.py:41-49). There is no fallback path: every operation of this
+package runs the HIP kernels or fails.
+"""
+from __future__ import annotations
+
+import ctypes
+import threading
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libbssn_hip.so"
+BUILD_HINT = "python integer-schemes_amd/build.py"
+ABI_VERSION = 1
+
+_c_int, _c_long, _c_double, _c_void_p, _c_size_t = (
+    ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_void_p, ctypes.c_size_t)
+_PD = ctypes.POINTER(ctypes.c_double)
+
+# name -> (restype, argtypes); mirrors include/bssn_hip.h
+SIGNATURES = {
+    "bssn_abi_version": (_c_int, []),
+    "bssn_last_error": (ctypes.c_char_p, []),
+    "bssn_rhs_f64": (_c_int, [_c_void_p, _PD, _c_void_p, _c_int, _c_int, _c_int, _c_int,
+                              _c_double, _c_void_p]),
+    "bssn_deriv_bundle_f64": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, _c_int, _c_int,
+                                       _PD, _c_void_p]),
+    "bssn_algebra_f64": (_c_int, [_c_void_p, _c_void_p, _PD, _c_void_p, _c_long, _c_void_p]),
+    "bssn_ko_add_f64": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, _c_int, _c_int,
+                                 _c_double, _PD, _c_int, _c_int, _c_void_p]),
+    "bssn_periodic_sync_f64": (_c_int, [_c_void_p, _c_int, _c_int, _c_int, _c_int, _c_void_p]),
+    "bssn_enforce_f64": (_c_int, [_c_void_p, _c_long, _c_double, _c_double, _c_void_p]),
+    "bssn_step_workspace_bytes": (_c_size_t, [_c_int, _c_int, _c_int, _c_int]),
+    "bssn_rk4_step_f64": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, _c_int, _PD, _c_int,
+                                   _c_double, _c_int, _c_int, _c_double, _c_double, _c_double,
+                                   _c_double, _c_void_p]),
+    "bssn_msrk_step_f64": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_void_p, _c_int, _c_int,
+                                    _c_int, _PD, _c_int, _c_double, _c_int, _c_int, _c_double,
+                                    _c_double, _c_double, _c_double, _c_void_p]),
+    "bssn_stage_rhs_f64": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, _c_int, _PD, _c_int,
+                                    _c_double, _c_int, _c_int, _c_void_p]),
+    "bssn_constraints_f64": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, _c_int, _c_int, _PD,
+                                      _c_void_p]),
+    "bssn_l2_partial_doubles": (_c_long, [_c_int, _c_int, _c_int, _c_int]),
+    "bssn_constraint_l2_f64": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_int, _c_int, _c_int,
+                                        _c_int, _PD, _c_int, _PD, _c_void_p]),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+class BSSNError(RuntimeError):
+    """A nonzero return code from the library, with its bssn_last_error() message."""
+
+
+def load():
+    """Load (once) and return the library with typed entry points."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not LIB_PATH.exists():
+                raise FileNotFoundError(f"{LIB_PATH} not built. Run: {BUILD_HINT}")
+            lib = ctypes.CDLL(str(LIB_PATH))
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(lib, name)
+                fn.restype = res
+                fn.argtypes = args
+            if lib.bssn_abi_version() != ABI_VERSION:
+                raise RuntimeError(f"libbssn_hip ABI {lib.bssn_abi_version()} != {ABI_VERSION}")
+            _lib = lib
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        msg = load().bssn_last_error().decode(errors="replace")
+        raise BSSNError(f"libbssn_hip error {rc}: {msg}")
+
+
+def dbl(values) -> ctypes.Array:
+    """A host double array for the scalar pack / spacing arguments."""
+    vals = [float(v) for v in values]
+    return (ctypes.c_double * len(vals))(*vals)

@@ -1,0 +1,132 @@
+"""BSSN time evolution on the device: fused RK4 / MSRK steps (3D/src/bssn3d/evolve.py:24-194).
+
+BSSNEvolution(grid, params, order, ko_sigma, bc, precision, scheme, ko_order, integrator) keeps
+the reference's constructor, rhs/enforce/step/evolve methods and semantics:
+  * rhs(state, t, dt)  = periodic sync (bc="periodic") + RHS + Kreiss-Oliger
+  * step(state, t, dt) = classic RK4 with stage times t, t+dt/2, t+dt/2, t+dt, the CAHD
+                          factor on the full dt at every stage, then enforce
+  * integrator "rk4_2_2": one RK4 start step, then the 2-step / 3-stage MSRK recurrence
+Each RK stage is ONE fused kernel (RHS + KO + stage update; enforcement folded into the last
+stage). With bc="periodic" stencils read periodically wrapped data, the interior is advanced
+and the ghosts are refilled by a sync at the end of the step, so the returned state equals the
+reference's after the sync its next step would perform. Other bc values advance every padded
+point with edge-padded stencils (the reference leaves such ghosts unsynced too).
+"""
+from __future__ import annotations
+
+import torch
+
+from . import _lib
+from .grid import Grid
+from .rhs import BSSNSolver, as_tensor, stream_handle
+from .state import BSSNState, PhysicsParams
+
+INTEGRATORS = ("rk4", "rk4_2_2")
+
+
+class BSSNEvolution:
+    def __init__(self, grid: Grid, params: PhysicsParams | None = None, order: int = 6,
+                 ko_sigma: float = 0.0, bc: str = "periodic", precision: str = "fp64",
+                 scheme: str = "fused", ko_order: int | None = None, integrator: str = "rk4"):
+        self.grid = grid
+        self.params = params or PhysicsParams()
+        self.solver = BSSNSolver(grid, self.params, order, precision=precision, scheme=scheme,
+                                 ko_order=ko_order)
+        self.order = order
+        self.ng = grid.ng
+        self.ko_sigma = float(ko_sigma)
+        self.bc = bc
+        if integrator not in INTEGRATORS:
+            raise ValueError(f"integrator {integrator!r}: only 'rk4' or 'rk4_2_2' are supported")
+        self.integrator = integrator
+        self._work = None
+
+    @property
+    def periodic(self) -> bool:
+        return self.bc == "periodic"
+
+    def _S16(self, t: float, dt: float):
+        return self.solver.S16(t, dt)
+
+    def _workspace(self, F: torch.Tensor, integrator: int) -> torch.Tensor:
+        _, Sx, Sy, Sz = F.shape
+        nbytes = _lib.load().bssn_step_workspace_bytes(integrator, Sx, Sy, Sz)
+        n = nbytes // 8
+        if self._work is None or self._work.numel() < n or self._work.device != F.device:
+            self._work = torch.empty(n, dtype=torch.float64, device=F.device)
+        return self._work
+
+    # --- pieces -----------------------------------------------------------------------------
+    def rhs(self, state, t: float = 0.0, dt: float | None = None) -> BSSNState:
+        """d/dt state as the step sees it: sync (periodic) + RHS + KO (evolve.py:86-90).
+        Periodic: interior values only (ghost entries of the result are zero)."""
+        F = as_tensor(state)
+        out = torch.zeros_like(F)
+        _, Sx, Sy, Sz = F.shape
+        dt = self.solver.dt if dt is None else dt
+        _lib.check(_lib.load().bssn_stage_rhs_f64(
+            F.data_ptr(), out.data_ptr(), Sx, Sy, Sz, _lib.dbl(self._S16(t, dt)), self.order,
+            self.ko_sigma, self.ng, int(self.periodic), stream_handle(F.device)))
+        return BSSNState(out)
+
+    def enforce(self, state) -> BSSNState:
+        F = as_tensor(state).clone()
+        _lib.check(_lib.load().bssn_enforce_f64(
+            F.data_ptr(), F[0].numel(), self.params.chi_floor, self.params.alpha_floor,
+            stream_handle(F.device)))
+        return BSSNState(F)
+
+    def periodic_sync_(self, F: torch.Tensor) -> torch.Tensor:
+        _, Sx, Sy, Sz = F.shape
+        _lib.check(_lib.load().bssn_periodic_sync_f64(F.data_ptr(), Sx, Sy, Sz, self.ng,
+                                                      stream_handle(F.device)))
+        return F
+
+    # --- steps (in place on a tensor) ---------------------------------------------------------
+    def step_(self, U: torch.Tensor, t: float, dt: float) -> torch.Tensor:
+        """One RK4 step of U in place."""
+        _, Sx, Sy, Sz = U.shape
+        work = self._workspace(U, 0)
+        _lib.check(_lib.load().bssn_rk4_step_f64(
+            U.data_ptr(), work.data_ptr(), Sx, Sy, Sz, _lib.dbl(self._S16(t, dt)), self.order,
+            self.ko_sigma, self.ng, int(self.periodic), float(t), float(dt),
+            self.params.chi_floor, self.params.alpha_floor, stream_handle(U.device)))
+        return U
+
+    def msrk_step_(self, U: torch.Tensor, fprev: torch.Tensor, fnext: torch.Tensor, t: float,
+                   dt: float) -> torch.Tensor:
+        """One rk4_2_2 step of U in place; fprev (f(y_{n-1})) is clobbered, fnext <- f(y_n)."""
+        _, Sx, Sy, Sz = U.shape
+        work = self._workspace(U, 1)
+        _lib.check(_lib.load().bssn_msrk_step_f64(
+            U.data_ptr(), fprev.data_ptr(), fnext.data_ptr(), work.data_ptr(), Sx, Sy, Sz,
+            _lib.dbl(self._S16(t, dt)), self.order, self.ko_sigma, self.ng, int(self.periodic),
+            float(t), float(dt), self.params.chi_floor, self.params.alpha_floor,
+            stream_handle(U.device)))
+        return U
+
+    def step(self, state, t: float, dt: float) -> BSSNState:
+        U = as_tensor(state).clone()
+        return BSSNState(self.step_(U, t, dt))
+
+    def evolve(self, state, dt: float, nsteps: int, t0: float = 0.0) -> BSSNState:
+        """nsteps steps from t0 (evolve.py:155-194); the input is not modified."""
+        U = as_tensor(state).clone()
+        if nsteps <= 0:
+            return BSSNState(U)
+        if self.integrator == "rk4":
+            t = t0
+            for _ in range(nsteps):
+                self.step_(U, t, dt)
+                t = t + dt
+            return BSSNState(U)
+        # MSRK: f(y_0) fills the history, one RK4 start step, then the recurrence
+        fprev = self.rhs(U, t0, dt).data
+        fnext = torch.empty_like(U)
+        self.step_(U, t0, dt)
+        t = t0 + dt
+        for _ in range(1, nsteps):
+            self.msrk_step_(U, fprev, fnext, t, dt)
+            fprev, fnext = fnext, fprev
+            t = t + dt
+        return BSSNState(U)

@@ -1,0 +1,163 @@
+"""Initial data for the BSSN workloads (torch tensors, any device).
+
+* minkowski / gauge_wave: the reference's test data (3D/src/bssn3d/initial_data.py:28-137).
+* punctures: Brill-Lindquist punctures with seeded sinusoid perturbations, the workload data
+  of the benchmark configurations. The reference has no puncture data
+  (initial_data.py:1: "NOT BBH ID"), so this construction is defined here:
+      psi   = 1 + sum_i m_i / (2 max(r_i, h/2))
+      chi   = psi^-4,  alpha = psi^-2,  gt_ij = delta_ij,  every other field 0,
+  then every field f gets amp * p_f(x) with
+      p_f(x) = (1/8) sum_{n=1..8} sin(2 pi k_n . (x - x_lo) / L + phi_n),
+  integer k_n with 0 < |k_n|_inf <= 4 and phi_n ~ U[0, 2 pi), drawn per field from
+  numpy default_rng(seed) (L = interior extent; the modes are periodic on the domain).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import Sequence
+
+import numpy as np
+import torch
+
+from .grid import Grid
+from .state import (ALPHA, AT0, AT3, AT5, CHI, GT0, GT3, GT5, GTILDE0, K, NUM_VARS)
+
+
+def _coords(grid: Grid, device, dtype=torch.float64):
+    x = torch.as_tensor(grid.x, dtype=dtype, device=device).view(-1, 1, 1)
+    y = torch.as_tensor(grid.y, dtype=dtype, device=device).view(1, -1, 1)
+    z = torch.as_tensor(grid.z, dtype=dtype, device=device).view(1, 1, -1)
+    return x, y, z
+
+
+def minkowski(grid: Grid, device="cpu") -> torch.Tensor:
+    d = torch.zeros((NUM_VARS,) + grid.shape, dtype=torch.float64, device=device)
+    for f in (ALPHA, CHI, GT0, GT3, GT5):
+        d[f] = 1.0
+    return d
+
+
+def gauge_wave(grid: Grid, amplitude: float = 0.01, wavelength: float | None = None,
+               axis: int = 0, t: float = 0.0, device="cpu") -> torch.Tensor:
+    """Harmonic gauge wave along ``axis`` (initial_data.py:71-137, :140-196 for t != 0)."""
+    coords1d = (grid.x, grid.y, grid.z)[axis]
+    if wavelength is None:
+        wavelength = float(coords1d[-grid.ng - 1] - coords1d[grid.ng]) + \
+            (grid.dx, grid.dy, grid.dz)[axis]
+    X, Y, Z = _coords(grid, device)
+    coord = (X, Y, Z)[axis].expand(grid.shape)
+    k = 2.0 * math.pi / wavelength
+    phase = k * (coord - t)
+    A = amplitude
+    H = 1.0 - A * torch.sin(phase)
+    dH_dx = -A * k * torch.cos(phase)
+    dH_dt = -dH_dx
+    sqrtH = torch.sqrt(H)
+    Kxx = -dH_dt / (2.0 * sqrtH)
+    Ktrace = Kxx / H
+    chi = H ** (-1.0 / 3.0)
+    gt_wave = H ** (2.0 / 3.0)
+    gt_trans = chi
+    At_wave = chi * (Kxx - (1.0 / 3.0) * H * Ktrace)
+    At_trans = chi * (0.0 - (1.0 / 3.0) * 1.0 * Ktrace)
+    Gt_wave = (2.0 / 3.0) * H ** (-5.0 / 3.0) * dH_dx
+    d = torch.zeros((NUM_VARS,) + grid.shape, dtype=torch.float64, device=device)
+    d[ALPHA] = sqrtH
+    d[CHI] = chi
+    d[K] = Ktrace
+    gt_diag, at_diag = (GT0, GT3, GT5), (AT0, AT3, AT5)
+    for a in range(3):
+        d[gt_diag[a]] = gt_wave if a == axis else gt_trans
+        d[at_diag[a]] = At_wave if a == axis else At_trans
+    d[GTILDE0 + axis] = Gt_wave
+    return d
+
+
+@dataclass
+class Perturbation:
+    """Per-field sinusoid draws: k (24, 8, 3) integers, phi (24, 8)."""
+    k: np.ndarray
+    phi: np.ndarray
+
+    @staticmethod
+    def draw(rng: np.random.Generator, nmodes: int = 8, kmax: int = 4) -> "Perturbation":
+        k = np.zeros((NUM_VARS, nmodes, 3), dtype=np.int64)
+        for f in range(NUM_VARS):
+            for n in range(nmodes):
+                while True:
+                    v = rng.integers(-kmax, kmax + 1, size=3)
+                    if np.max(np.abs(v)) > 0:
+                        break
+                k[f, n] = v
+        phi = rng.uniform(0.0, 2.0 * math.pi, size=(NUM_VARS, nmodes))
+        return Perturbation(k, phi)
+
+
+def _mode_sum(grid: Grid, pert: Perturbation, f: int, device):
+    X, Y, Z = _coords(grid, device)
+    L = (grid.Nx * grid.dx, grid.Ny * grid.dy, grid.Nz * grid.dz)
+    lo = (grid.xmin, grid.ymin, grid.zmin)
+    acc = torch.zeros(grid.shape, dtype=torch.float64, device=device)
+    for n in range(pert.k.shape[1]):
+        kx, ky, kz = (int(v) for v in pert.k[f, n])
+        arg = (2.0 * math.pi * kx / L[0]) * (X - lo[0]) + (2.0 * math.pi * ky / L[1]) * (Y - lo[1]) \
+            + (2.0 * math.pi * kz / L[2]) * (Z - lo[2]) + float(pert.phi[f, n])
+        acc += torch.sin(arg)
+    return acc / pert.k.shape[1]
+
+
+def punctures(grid: Grid, masses: Sequence[float], positions: Sequence[Sequence[float]],
+              amp: float = 1e-4, seed: int = 0, device="cpu") -> torch.Tensor:
+    """Brill-Lindquist punctures + seeded perturbations (see the module docstring)."""
+    X, Y, Z = _coords(grid, device)
+    h = min(grid.dx, grid.dy, grid.dz)
+    psi = torch.ones(grid.shape, dtype=torch.float64, device=device)
+    for m, (px, py, pz) in zip(masses, positions):
+        r = torch.sqrt((X - px) ** 2 + (Y - py) ** 2 + (Z - pz) ** 2)
+        psi += m / (2.0 * torch.clamp(r, min=0.5 * h))
+    d = torch.zeros((NUM_VARS,) + grid.shape, dtype=torch.float64, device=device)
+    d[ALPHA] = psi ** -2
+    d[CHI] = psi ** -4
+    for f in (GT0, GT3, GT5):
+        d[f] = 1.0
+    del psi
+    if amp != 0.0:
+        pert = Perturbation.draw(np.random.default_rng(seed))
+        for f in range(NUM_VARS):
+            d[f] += amp * _mode_sum(grid, pert, f, device)
+    return d
+
+
+def offgrid_shift(rng: np.random.Generator, h: float) -> np.ndarray:
+    """A per-axis shift in [0.25 h, 0.75 h): keeps punctures at least h/4 off every grid plane."""
+    return (0.5 + rng.uniform(-0.25, 0.25, size=3)) * h
+
+
+# --- the workload configurations -------------------------------------------------------------
+
+def config_single_puncture(seed: int = 0, N: int = 48, device="cpu"):
+    """48^3 interior, 3 ghosts, h = 0.25 on [-6, 6): one puncture m = 1 at a seeded off-grid
+    position, 1e-4 perturbations. Returns (grid, state, positions)."""
+    h = 12.0 / N
+    grid = Grid(N, N, N, h, h, h, ng=3, xmin=-6.0, ymin=-6.0, zmin=-6.0)
+    rng = np.random.default_rng(seed)
+    pos = offgrid_shift(rng, h)
+    state = punctures(grid, [1.0], [pos], amp=1e-4, seed=seed + 1000, device=device)
+    return grid, state, [pos.tolist()]
+
+
+def config_binary(seed: int = 0, N: int = 256, L: float = 16.0, masses=(0.5, 0.5),
+                  separation: float = 6.0, device="cpu"):
+    """N^3 interior, 3 ghosts, h = L/N on [-L/2, L/2): two punctures separated by ``separation``
+    along x, shifted together off the grid planes by a seeded sub-h offset, 1e-4 perturbations.
+    The default is the 256^3, h = 1/16 equal-mass binary."""
+    h = L / N
+    lo = -0.5 * L
+    grid = Grid(N, N, N, h, h, h, ng=3, xmin=lo, ymin=lo, zmin=lo)
+    rng = np.random.default_rng(seed)
+    s = offgrid_shift(rng, h)
+    p1 = np.array([-0.5 * separation, 0.0, 0.0]) + s
+    p2 = np.array([0.5 * separation, 0.0, 0.0]) + s
+    state = punctures(grid, list(masses), [p1, p2], amp=1e-4, seed=seed + 1000, device=device)
+    return grid, state, [p1.tolist(), p2.tolist()]

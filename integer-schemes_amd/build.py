@@ -1,0 +1,66 @@
+"""Build libbssn_hip.so (gfx950) in-tree with hipcc.
+
+    python integer-schemes_amd/build.py          # compile every source, link the library
+
+Each HIP source is compiled separately (in parallel) for --offload-arch=gfx950 and linked into
+integer-schemes_amd/bssn3d_hip/libbssn_hip.so, which the Python host layer loads.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+OUT = PKG / "bssn3d_hip" / "libbssn_hip.so"
+OBJ = PKG / "_build"
+ARCH = os.environ.get("BSSN_OFFLOAD_ARCH", "gfx950")
+
+SOURCES = ["bssn_fused.hip", "bssn_aux.hip", "bssn_abi.cpp"]
+
+# Packed-FP32 instructions are disabled for every HIP compile (they are not used by this
+# FP64 library; the flag keeps hipcc from forming them from any float code).
+HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             f"-I{ROOT / 'include'}", f"-I{CSRC}"]
+
+
+def hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and Path(cand).exists():
+            return cand
+    raise RuntimeError("hipcc not found (set HIPCC or install ROCm)")
+
+
+def _compile(src: str) -> Path:
+    obj = OBJ / (Path(src).stem + ".o")
+    cmd = [hipcc(), *HIP_FLAGS, "-c", str(CSRC / src), "-o", str(obj)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    return obj
+
+
+def build(verbose: bool = False) -> Path:
+    OBJ.mkdir(exist_ok=True)
+    with cf.ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 1)) as ex:
+        objs = list(ex.map(_compile, SOURCES))
+    tmp = OUT.with_suffix(".so.tmp")
+    cmd = [hipcc(), "-shared", f"--offload-arch={ARCH}", "-fPIC", *map(str, objs), "-o", str(tmp)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, OUT)
+    if verbose:
+        print(f"built {OUT}")
+    return OUT
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    sys.exit(0)

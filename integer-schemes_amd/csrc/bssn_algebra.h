@@ -1,0 +1,521 @@
+// The BSSN right-hand side at one grid point, ordered field by field so that each derivative
+// is consumed as soon as it is formed (gfx950 register budget).
+//
+// Equations: the CAHD + SSL moving-puncture BSSN system the reference evaluates
+// (3D/src/bssn3d/vendor/bssneqs_SSL_HD_dxsq.cpp, output statements a_rhs .. B_rhs2, through
+// _bssn_rhs_generated.py:bssn_rhs_algebra), written here from the tensor form:
+//   Gamma_kij = (d_j gt_ki + d_i gt_kj - d_k gt_ij)/2,  Gamma^m_ij = gt^mk Gamma_kij,
+//   CalGt^m = gt^ij Gamma^m_ij,
+//   R_ij = Rt_ij + Rchi_ij (conformal Ricci: metric part with the evolved Gt^k in the
+//          d_(i Gt_j) term, chi part with CalGt), H = chi gt^ij R_ij - At.At + 2K^2/3,
+//   DDalpha_ij = d_i d_j alpha - Gamma3^l_ij d_l alpha (physical Christoffels),
+// and the evolution equations for alpha (1+log-free harmonic-K form with slice locking),
+// beta (Gamma-driver), gt, chi (with Hamiltonian damping), At, K, Gt, B. Advection is
+// centred. Parity: every output agrees with the compiled reference CSE to round-off
+// (tests/test_oracle_parity.py, tests/test_hip_parity.py).
+//
+// Src must provide (all calls uniform across the workgroup):
+//   void   stage(int f)        make field f current (may synchronise the workgroup)
+//   double val()               current field at the point
+//   double d1(int a)           first derivative of the current field along axis a
+//   double d2(int i, int j)    second derivative, i <= j (mixed = d1_j(d1_i))
+//   double ko()                Kreiss-Oliger sum over the three axes (KO builds only)
+//   double center(int f)       field f at the point, any time
+// Out must provide: void put(int f, double rhs)   -- called exactly once per field.
+#pragma once
+
+#include "bssn_params.h"
+
+namespace bssn {
+
+#ifndef BSSN_PIN
+#define BSSN_PIN(x) (x) = pin(x)
+#endif
+
+__host__ __device__ constexpr int sym(int i, int j) {
+  return (i <= j) ? (i == 0 ? j : (i == 1 ? 2 + j : 5)) : (j == 0 ? i : (j == 1 ? 2 + i : 5));
+}
+// packed symmetric index -> (i, j)
+__host__ __device__ constexpr int pi_(int q) { return q < 3 ? 0 : (q < 5 ? 1 : 2); }
+__host__ __device__ constexpr int pj_(int q) { return q < 3 ? q : (q < 5 ? q - 2 : 2); }
+
+// symmetric 3x3 (packed) contraction sum_ij a^ij b_ij
+__device__ __forceinline__ double contract_sym(const double* a, const double* b) {
+  return a[0] * b[0] + a[3] * b[3] + a[5] * b[5] + 2.0 * (a[1] * b[1] + a[2] * b[2] + a[4] * b[4]);
+}
+
+template <bool KO, class Src, class Out>
+__device__ __forceinline__ void bssn_rhs_point(Src& s, const Scal& P, Out& o) {
+  // ---------------------------------------------------------------- metric and shift centres
+  double g[6], ig[6], b[3];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) g[q] = s.center(GT0 + q);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) b[i] = s.center(BETA0 + i);
+  {
+    const double a00 = g[3] * g[5] - g[4] * g[4];
+    const double a01 = g[2] * g[4] - g[1] * g[5];
+    const double a02 = g[1] * g[4] - g[2] * g[3];
+    const double a11 = g[0] * g[5] - g[2] * g[2];
+    const double a12 = g[1] * g[2] - g[0] * g[4];
+    const double a22 = g[0] * g[3] - g[1] * g[1];
+    const double det = g[0] * a00 + g[1] * a01 + g[2] * a02;
+    const double idet = 1.0 / det;
+    ig[0] = a00 * idet; ig[1] = a01 * idet; ig[2] = a02 * idet;
+    ig[3] = a11 * idet; ig[4] = a12 * idet; ig[5] = a22 * idet;
+  }
+#pragma unroll
+  for (int q = 0; q < 6; ++q) BSSN_PIN(ig[q]);
+
+  // ---------------------------------------------------------------- gt_ij: Christoffels, Rt
+  double dg[3][6];     // d_k gt_q
+  double Rt[6];        // accumulates the Ricci tensor
+  double rg[6];        // gt right-hand sides (partial)
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    s.stage(GT0 + q);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) dg[a][q] = s.d1(a);
+    const double lapg = ig[0] * s.d2(0, 0) + ig[3] * s.d2(1, 1) + ig[5] * s.d2(2, 2) +
+                        2.0 * (ig[1] * s.d2(0, 1) + ig[2] * s.d2(0, 2) + ig[4] * s.d2(1, 2));
+    Rt[q] = -0.5 * lapg;
+    double r = b[0] * dg[0][q] + b[1] * dg[1][q] + b[2] * dg[2][q];
+    if (KO) r += s.ko();
+    rg[q] = r;
+  }
+  double C2[3][6], CalGt[3];
+  {
+    double C1[3][6];   // Gamma_k(ij), first index lowered
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        const int i = pi_(q), j = pj_(q);
+        C1[k][q] = 0.5 * (dg[j][sym(k, i)] + dg[i][sym(k, j)] - dg[k][q]);
+      }
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+      for (int q = 0; q < 6; ++q)
+        C2[m][q] = ig[sym(m, 0)] * C1[0][q] + ig[sym(m, 1)] * C1[1][q] + ig[sym(m, 2)] * C1[2][q];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) CalGt[m] = contract_sym(ig, C2[m]);
+    // raised Christoffels: Gu[i][k][m] = gt^ml Gamma^k_li,  Gd[k][j][m] = gt^ml Gamma_klj
+    //   Rt_ij += CalGt^k (Gamma_ijk + Gamma_jik)/2
+    //          + sum_km (Gu[i][k][m] Gamma_jkm + Gu[j][k][m] Gamma_ikm + Gamma^k_im Gd[k][j][m])
+    double Gu[3][3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+          Gu[i][k][m] = ig[sym(m, 0)] * C2[k][sym(0, i)] + ig[sym(m, 1)] * C2[k][sym(1, i)] +
+                        ig[sym(m, 2)] * C2[k][sym(2, i)];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const int i = pi_(q), j = pj_(q);
+      double t = 0.0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) t += CalGt[k] * (C1[i][sym(j, k)] + C1[j][sym(i, k)]);
+      double u = 0.0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          const double gd = ig[sym(m, 0)] * C1[k][sym(0, j)] + ig[sym(m, 1)] * C1[k][sym(1, j)] +
+                            ig[sym(m, 2)] * C1[k][sym(2, j)];
+          u += Gu[i][k][m] * C1[j][sym(k, m)] + Gu[j][k][m] * C1[i][sym(k, m)] +
+               C2[k][sym(i, m)] * gd;
+        }
+      Rt[q] += 0.5 * t + u;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    BSSN_PIN(Rt[q]);
+    BSSN_PIN(C2[0][q]); BSSN_PIN(C2[1][q]); BSSN_PIN(C2[2][q]);
+  }
+#pragma unroll
+  for (int m = 0; m < 3; ++m) BSSN_PIN(CalGt[m]);
+
+  // ---------------------------------------------------------------- chi: Ricci chi part
+  double chi, dchi[3], igdchi[3];
+  double rchi;
+  {
+    s.stage(CHI);
+    chi = s.val();
+#pragma unroll
+    for (int a = 0; a < 3; ++a) dchi[a] = s.d1(a);
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+      igdchi[m] = ig[sym(m, 0)] * dchi[0] + ig[sym(m, 1)] * dchi[1] + ig[sym(m, 2)] * dchi[2];
+    double d2c[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) d2c[q] = s.d2(pi_(q), pj_(q));
+    const double ichi = 1.0 / chi;
+    const double dcdc = dchi[0] * igdchi[0] + dchi[1] * igdchi[1] + dchi[2] * igdchi[2];
+    const double lapchi = contract_sym(ig, d2c) - 1.5 * ichi * dcdc -
+                          (CalGt[0] * dchi[0] + CalGt[1] * dchi[1] + CalGt[2] * dchi[2]);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const int i = pi_(q), j = pj_(q);
+      const double c = C2[0][q] * dchi[0] + C2[1][q] * dchi[1] + C2[2][q] * dchi[2];
+      Rt[q] += 0.5 * ichi * (d2c[q] - c) - 0.25 * ichi * ichi * dchi[i] * dchi[j] +
+               0.5 * ichi * g[q] * lapchi;
+    }
+    rchi = b[0] * dchi[0] + b[1] * dchi[1] + b[2] * dchi[2];
+    if (KO) rchi += s.ko();
+#pragma unroll
+    for (int q = 0; q < 6; ++q) BSSN_PIN(Rt[q]);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) BSSN_PIN(igdchi[m]);
+  }
+
+  // ---------------------------------------------------------------- Gt^k: d_(i Gt_j) term
+  double advGt[3], koGt[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    s.stage(GTI0 + k);
+    double dG[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) dG[a] = s.d1(a);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const int i = pi_(q), j = pj_(q);
+      Rt[q] += 0.5 * (g[sym(k, i)] * dG[j] + g[sym(k, j)] * dG[i]);
+    }
+    advGt[k] = b[0] * dG[0] + b[1] * dG[1] + b[2] * dG[2];
+    koGt[k] = KO ? s.ko() : 0.0;
+  }
+  // R_ij is complete from here on.
+
+  // ---------------------------------------------------------------- At_ij
+  double A[6], rA[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    s.stage(AT0 + q);
+    A[q] = s.val();
+    double r = b[0] * s.d1(0) + b[1] * s.d1(1) + b[2] * s.d1(2);
+    if (KO) r += s.ko();
+    rA[q] = r;
+  }
+  double Aup[6], AA, AkA[6];   // At^ij, At_ij At^ij, At_ik gt^kl At_lj
+  {
+    double IA[3][3];   // IA[k][j] = gt^kl At_lj
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        IA[k][j] = ig[sym(k, 0)] * A[sym(0, j)] + ig[sym(k, 1)] * A[sym(1, j)] +
+                   ig[sym(k, 2)] * A[sym(2, j)];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const int i = pi_(q), j = pj_(q);
+      Aup[q] = IA[i][0] * ig[sym(0, j)] + IA[i][1] * ig[sym(1, j)] + IA[i][2] * ig[sym(2, j)];
+      AkA[q] = A[sym(i, 0)] * IA[0][j] + A[sym(i, 1)] * IA[1][j] + A[sym(i, 2)] * IA[2][j];
+    }
+    AA = contract_sym(Aup, A);
+  }
+#pragma unroll
+  for (int q = 0; q < 6; ++q) { BSSN_PIN(Aup[q]); BSSN_PIN(AkA[q]); }
+  BSSN_PIN(AA);
+
+  // ---------------------------------------------------------------- alpha: D_iD_j alpha
+  const double K = s.center(KTR);
+  double al, dal[3], rGt[3], rK;
+  {
+    s.stage(ALPHA);
+    al = s.val();
+#pragma unroll
+    for (int a = 0; a < 3; ++a) dal[a] = s.d1(a);
+    const double ichi = 1.0 / chi;
+    const double igdd = igdchi[0] * dal[0] + igdchi[1] * dal[1] + igdchi[2] * dal[2];
+    double DD[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const int i = pi_(q), j = pj_(q);
+      const double c3 = C2[0][q] * dal[0] + C2[1][q] * dal[1] + C2[2][q] * dal[2] -
+                        0.5 * ichi * (dal[i] * dchi[j] + dal[j] * dchi[i] - g[q] * igdd);
+      DD[q] = s.d2(i, j) - c3;
+    }
+    const double Rs = contract_sym(ig, Rt);
+    double X[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) X[q] = al * Rt[q] - DD[q];
+    const double trX3 = contract_sym(ig, X) * (1.0 / 3.0);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) rA[q] += chi * (X[q] - g[q] * trX3);
+    const double lap = contract_sym(ig, DD);
+    rK = -chi * lap;
+    // Hamiltonian constraint and its damping of chi (CAHD)
+    const double ham = chi * Rs - AA + (2.0 / 3.0) * K * K;
+    rchi += P.cahd * chi * ham;
+    // lapse
+    const double sq = sqrt(chi);
+    double ra = P.lam[0] * (b[0] * dal[0] + b[1] * dal[1] + b[2] * dal[2]) - 2.0 * al * K -
+                sq * P.ssl_h * (al - sq) * P.ssl_e;
+    if (KO) ra += s.ko();
+    o.put(ALPHA, ra);
+    // connection-function terms that need alpha
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double t = 0.0, u = 0.0, w = 0.0;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        t += Aup[sym(i, j)] * dal[j];
+        w += Aup[sym(i, j)] * dchi[j];
+      }
+      u = contract_sym(C2[i], Aup);
+      rGt[i] = advGt[i] - 2.0 * t + 2.0 * al * u - 3.0 * al * ichi * w;
+    }
+  }
+
+  // ---------------------------------------------------------------- beta^i
+  double db[3][3];   // db[j][i] = d_j beta^i
+  {
+    const double Bf = 0.75 * (P.lamf[0] + P.lamf[1] * al);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      s.stage(BETA0 + i);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) db[a][i] = s.d1(a);
+      double d2b[6];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) d2b[q] = s.d2(pi_(q), pj_(q));
+      rGt[i] += contract_sym(ig, d2b);
+      // (1/3) gt^mj d_j d_i beta^i
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+        rGt[m] += (1.0 / 3.0) * (ig[sym(m, 0)] * d2b[sym(0, i)] + ig[sym(m, 1)] * d2b[sym(1, i)] +
+                                 ig[sym(m, 2)] * d2b[sym(2, i)]);
+      double rb = Bf * s.center(BB0 + i) +
+                  P.lam[1] * (b[0] * db[0][i] + b[1] * db[1][i] + b[2] * db[2][i]);
+      if (KO) rb += s.ko();
+      o.put(BETA0 + i, rb);
+    }
+  }
+  const double divb = db[0][0] + db[1][1] + db[2][2];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    rGt[i] += -(CalGt[0] * db[0][i] + CalGt[1] * db[1][i] + CalGt[2] * db[2][i]) +
+              (2.0 / 3.0) * CalGt[i] * divb;
+  // conformal metric
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const int i = pi_(q), j = pj_(q);
+    double lie = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) lie += g[sym(i, k)] * db[j][k] + g[sym(k, j)] * db[i][k];
+    o.put(GT0 + q, rg[q] + lie - (2.0 / 3.0) * g[q] * divb - 2.0 * al * A[q]);
+  }
+  rchi += (2.0 / 3.0) * chi * (al * K - divb);
+  o.put(CHI, rchi);
+
+  // ---------------------------------------------------------------- K
+  {
+    s.stage(KTR);
+    double dK[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) dK[a] = s.d1(a);
+    double r = rK + b[0] * dK[0] + b[1] * dK[1] + b[2] * dK[2] + al * (K * K * (1.0 / 3.0) + AA);
+    if (KO) r += s.ko();
+    o.put(KTR, r);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      rGt[i] -= (4.0 / 3.0) * al *
+                (ig[sym(i, 0)] * dK[0] + ig[sym(i, 1)] * dK[1] + ig[sym(i, 2)] * dK[2]);
+  }
+  // At
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const int i = pi_(q), j = pj_(q);
+    double lie = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) lie += A[sym(i, k)] * db[j][k] + A[sym(k, j)] * db[i][k];
+    o.put(AT0 + q, rA[q] + lie - (2.0 / 3.0) * A[q] * divb + al * (K * A[q] - 2.0 * AkA[q]));
+  }
+
+  // ---------------------------------------------------------------- B^i and Gt^i
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    s.stage(BB0 + i);
+    double r = rGt[i] - P.eta * s.val() +
+               P.lam[2] * (b[0] * s.d1(0) + b[1] * s.d1(1) + b[2] * s.d1(2)) - P.lam[3] * advGt[i];
+    if (KO) r += s.ko();
+    o.put(BB0 + i, r);
+    o.put(GTI0 + i, rGt[i] + koGt[i]);
+  }
+}
+
+
+// Hamiltonian and momentum constraints at one point (3D/src/bssn3d/constraints.py:38-51, the
+// reference's physcon CSE 3D/src/bssn3d/vendor/physcon.cpp:360-1287):
+//   H   = chi gt^ij R_ij - At_ij At^ij + 2K^2/3
+//   M^i = gt^jk (d_j At_ki - Gamma^m_ji At_km) - Gt^j At_ij - 3/(2chi) gt^jk At_ki d_j chi
+//         - 2/3 d_i K
+// Same Src interface as bssn_rhs_point (no KO); Out receives put(0..3, value).
+template <class Src, class Out>
+__device__ __forceinline__ void bssn_constraints_point(Src& s, Out& o) {
+  double g[6], ig[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) g[q] = s.center(GT0 + q);
+  {
+    const double a00 = g[3] * g[5] - g[4] * g[4];
+    const double a01 = g[2] * g[4] - g[1] * g[5];
+    const double a02 = g[1] * g[4] - g[2] * g[3];
+    const double a11 = g[0] * g[5] - g[2] * g[2];
+    const double a12 = g[1] * g[2] - g[0] * g[4];
+    const double a22 = g[0] * g[3] - g[1] * g[1];
+    const double idet = 1.0 / (g[0] * a00 + g[1] * a01 + g[2] * a02);
+    ig[0] = a00 * idet; ig[1] = a01 * idet; ig[2] = a02 * idet;
+    ig[3] = a11 * idet; ig[4] = a12 * idet; ig[5] = a22 * idet;
+  }
+  double dg[3][6], Rt[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    s.stage(GT0 + q);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) dg[a][q] = s.d1(a);
+    Rt[q] = -0.5 * (ig[0] * s.d2(0, 0) + ig[3] * s.d2(1, 1) + ig[5] * s.d2(2, 2) +
+                    2.0 * (ig[1] * s.d2(0, 1) + ig[2] * s.d2(0, 2) + ig[4] * s.d2(1, 2)));
+  }
+  double C2[3][6], CalGt[3];
+  {
+    double C1[3][6];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        const int i = pi_(q), j = pj_(q);
+        C1[k][q] = 0.5 * (dg[j][sym(k, i)] + dg[i][sym(k, j)] - dg[k][q]);
+      }
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+      for (int q = 0; q < 6; ++q)
+        C2[m][q] = ig[sym(m, 0)] * C1[0][q] + ig[sym(m, 1)] * C1[1][q] + ig[sym(m, 2)] * C1[2][q];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) CalGt[m] = contract_sym(ig, C2[m]);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const int i = pi_(q), j = pj_(q);
+      double t = 0.0, u = 0.0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) t += CalGt[k] * (C1[i][sym(j, k)] + C1[j][sym(i, k)]);
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          const double gui = ig[sym(m, 0)] * C2[k][sym(0, i)] + ig[sym(m, 1)] * C2[k][sym(1, i)] +
+                             ig[sym(m, 2)] * C2[k][sym(2, i)];
+          const double guj = ig[sym(m, 0)] * C2[k][sym(0, j)] + ig[sym(m, 1)] * C2[k][sym(1, j)] +
+                             ig[sym(m, 2)] * C2[k][sym(2, j)];
+          const double gd = ig[sym(m, 0)] * C1[k][sym(0, j)] + ig[sym(m, 1)] * C1[k][sym(1, j)] +
+                            ig[sym(m, 2)] * C1[k][sym(2, j)];
+          u += gui * C1[j][sym(k, m)] + guj * C1[i][sym(k, m)] + C2[k][sym(i, m)] * gd;
+        }
+      Rt[q] += 0.5 * t + u;
+    }
+  }
+  double chi, dchi[3];
+  {
+    s.stage(CHI);
+    chi = s.val();
+#pragma unroll
+    for (int a = 0; a < 3; ++a) dchi[a] = s.d1(a);
+    double d2c[6], igdchi[3];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) d2c[q] = s.d2(pi_(q), pj_(q));
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+      igdchi[m] = ig[sym(m, 0)] * dchi[0] + ig[sym(m, 1)] * dchi[1] + ig[sym(m, 2)] * dchi[2];
+    const double ichi = 1.0 / chi;
+    const double lapchi = contract_sym(ig, d2c) -
+                          1.5 * ichi * (dchi[0] * igdchi[0] + dchi[1] * igdchi[1] + dchi[2] * igdchi[2]) -
+                          (CalGt[0] * dchi[0] + CalGt[1] * dchi[1] + CalGt[2] * dchi[2]);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const int i = pi_(q), j = pj_(q);
+      const double c = C2[0][q] * dchi[0] + C2[1][q] * dchi[1] + C2[2][q] * dchi[2];
+      Rt[q] += 0.5 * ichi * (d2c[q] - c) - 0.25 * ichi * ichi * dchi[i] * dchi[j] +
+               0.5 * ichi * g[q] * lapchi;
+    }
+  }
+  double Gt[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    s.stage(GTI0 + k);
+    Gt[k] = s.val();
+    double dG[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) dG[a] = s.d1(a);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const int i = pi_(q), j = pj_(q);
+      Rt[q] += 0.5 * (g[sym(k, i)] * dG[j] + g[sym(k, j)] * dG[i]);
+    }
+  }
+  double A[6], mom[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    s.stage(AT0 + q);
+    A[q] = s.val();
+    double dA[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) dA[a] = s.d1(a);
+    // gt^jk d_j At_ki for every ordered (k, i) with At_ki = At_q
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+        if (sym(k, i) == q)
+          mom[i] += ig[sym(0, k)] * dA[0] + ig[sym(1, k)] * dA[1] + ig[sym(2, k)] * dA[2];
+  }
+  double AA;
+  {
+    double IA[3][3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        IA[k][j] = ig[sym(k, 0)] * A[sym(0, j)] + ig[sym(k, 1)] * A[sym(1, j)] +
+                   ig[sym(k, 2)] * A[sym(2, j)];
+    double Aup[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const int i = pi_(q), j = pj_(q);
+      Aup[q] = IA[i][0] * ig[sym(0, j)] + IA[i][1] * ig[sym(1, j)] + IA[i][2] * ig[sym(2, j)];
+    }
+    AA = contract_sym(Aup, A);
+    const double ichi = 1.0 / chi;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double c = 0.0, gA = 0.0, w = 0.0;
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const double cm = C2[0][sym(j, i)] * A[sym(k, 0)] + C2[1][sym(j, i)] * A[sym(k, 1)] +
+                            C2[2][sym(j, i)] * A[sym(k, 2)];
+          c += ig[sym(j, k)] * cm;
+          w += ig[sym(j, k)] * A[sym(k, i)] * dchi[j];
+        }
+#pragma unroll
+      for (int j = 0; j < 3; ++j) gA += Gt[j] * A[sym(i, j)];
+      mom[i] += -c - gA - 1.5 * ichi * w;
+    }
+  }
+  {
+    s.stage(KTR);
+    const double K = s.val();
+#pragma unroll
+    for (int i = 0; i < 3; ++i) mom[i] -= (2.0 / 3.0) * s.d1(i);
+    const double ham = chi * contract_sym(ig, Rt) - AA + (2.0 / 3.0) * K * K;
+    o.put(0, ham);
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) o.put(1 + i, mom[i]);
+}
+
+}  // namespace bssn

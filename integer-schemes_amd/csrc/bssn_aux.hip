@@ -1,0 +1,333 @@
+// The materialised two-pass RHS path and the auxiliary kernels of the step (gfx950).
+//
+// Two-pass path (the data flow of the reference's verbatim scheme, rhs.py:102-134):
+//   pass 1  k_deriv_bundle : 24 fields -> 138 derivative arrays in HBM
+//           (derivative_bundle.py:31-48; device analogue of cuda/deriv_2p5d.cu / deriv_smem.cu)
+//   pass 2  k_algebra      : 24 fields + 138 derivatives -> 24 right-hand sides, pointwise
+//           (_bssn_rhs_generated.py:bssn_rhs_algebra)
+//   KO      k_ko_add       : OUT += Kreiss-Oliger over all 24 fields (evolve.py:76-84)
+// Auxiliary:
+//   k_periodic_sync (evolve.py:65-73), k_enforce (evolve.py:93-119),
+//   k_constraints (constraints.py:38-51) and k_sumsq for the interior norms (:53-62).
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+
+#include "bssn_algebra.h"
+#include "bssn_brick.h"
+#include "bssn_kernels.h"
+#include "bssn_params.h"
+
+namespace bssn {
+
+// second-derivative slot of a field (alpha, chi, gt0..5, beta0..2), -1 otherwise
+__host__ __device__ constexpr int s2slot(int f) {
+  return f == ALPHA ? 0 : (f == CHI ? 1 : (f >= GT0 && f <= GT5 ? 2 + f - GT0
+                                          : (f >= BETA0 && f <= BETA2 ? 8 + f - BETA0 : -1)));
+}
+
+template <class G>
+__device__ __forceinline__ bool brick_point(const Dims& d, int* ox, int* oy, int* oz, int* tx,
+                                            int* ty, int* tz) {
+  const int nbx = (d.n[0] + G::BX - 1) / G::BX;
+  const int nby = (d.n[1] + G::BY - 1) / G::BY;
+  const int nbz = (d.n[2] + G::BZ - 1) / G::BZ;
+  int bx, by, bz;
+  if (!brick_of_block(nbx, nby, nbz, &bx, &by, &bz)) return false;
+  const int t = (int)threadIdx.x;
+  *tz = t % G::BZ;
+  *ty = (t / G::BZ) % G::BY;
+  *tx = t / (G::BZ * G::BY);
+  *ox = d.lo[0] + bx * G::BX;
+  *oy = d.lo[1] + by * G::BY;
+  *oz = d.lo[2] + bz * G::BZ;
+  return true;
+}
+
+template <class G>
+static unsigned brick_blocks(const Dims& d) {
+  const long nb = (long)((d.n[0] + G::BX - 1) / G::BX) * ((d.n[1] + G::BY - 1) / G::BY) *
+                  ((d.n[2] + G::BZ - 1) / G::BZ);
+  return (unsigned)((nb + 7) & ~7L);
+}
+
+// ------------------------------------------------------------------------- pass 1: bundle
+template <int ORD>
+__global__ __launch_bounds__(256) void k_deriv_bundle(const double* __restrict__ F,
+                                                      double* __restrict__ D, Dims d, FDW w) {
+  using G = Geo<ORD, false>;
+  __shared__ double lds[2 * G::LVOL];
+  int ox, oy, oz, tx, ty, tz;
+  if (!brick_point<G>(d, &ox, &oy, &oz, &tx, &ty, &tz)) return;
+  const bool valid = (ox + tx < d.lo[0] + d.n[0]) && (oy + ty < d.lo[1] + d.n[1]) &&
+                     (oz + tz < d.lo[2] + d.n[2]);
+  BrickSrc<ORD, false, CLAMP> s(lds, F, w, d, ox, oy, oz, tx, ty, tz);
+#pragma unroll 1
+  for (int f = 0; f < BSSN_NF; ++f) {
+    s.stage(f);
+    double v[9];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) v[a] = s.d1(a);
+    const int sl = s2slot(f);
+    if (sl >= 0) {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) v[3 + q] = s.d2(pi_(q), pj_(q));
+    }
+    if (valid) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) D[(long)(3 * f + a) * d.P + s.gc] = v[a];
+      if (sl >= 0) {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) D[(long)(72 + 6 * sl + q) * d.P + s.gc] = v[3 + q];
+      }
+    }
+  }
+}
+
+hipError_t launch_deriv_bundle(const double* F, double* D, const Dims& d, const FDW& w, int ord,
+                               hipStream_t st) {
+  if (ord == 4)
+    hipLaunchKernelGGL(k_deriv_bundle<4>, dim3(brick_blocks<Geo<4, false>>(d)), dim3(256), 0, st,
+                       F, D, d, w);
+  else
+    hipLaunchKernelGGL(k_deriv_bundle<6>, dim3(brick_blocks<Geo<6, false>>(d)), dim3(256), 0, st,
+                       F, D, d, w);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------- pass 2: algebra
+struct BundleSrc {
+  const double* F;
+  const double* D;
+  long n, p;
+  int cur;
+  __device__ __forceinline__ void stage(int f) { cur = f; }
+  __device__ __forceinline__ double center(int f) const { return F[(long)f * n + p]; }
+  __device__ __forceinline__ double val() const { return F[(long)cur * n + p]; }
+  __device__ __forceinline__ double d1(int a) const { return D[(long)(3 * cur + a) * n + p]; }
+  __device__ __forceinline__ double d2(int i, int j) const {
+    return D[(long)(72 + 6 * s2slot(cur) + sym(i, j)) * n + p];
+  }
+  __device__ __forceinline__ double ko() const { return 0.0; }
+};
+
+struct PlainOut {
+  double* OUT;
+  long n, p;
+  __device__ __forceinline__ void put(int f, double v) { OUT[(long)f * n + p] = v; }
+};
+
+__global__ __launch_bounds__(256) void k_algebra(const double* __restrict__ F,
+                                                 const double* __restrict__ D,
+                                                 double* __restrict__ OUT, long n, Scal sc) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  BundleSrc s{F, D, n, p, 0};
+  PlainOut o{OUT, n, p};
+  bssn_rhs_point<false>(s, sc, o);
+}
+
+hipError_t launch_algebra(const double* F, const double* D, double* OUT, long npts,
+                          const Scal& sc, hipStream_t st) {
+  const unsigned nb = (unsigned)((npts + 255) / 256);
+  hipLaunchKernelGGL(k_algebra, dim3(nb), dim3(256), 0, st, F, D, OUT, npts, sc);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------- KO pass
+template <int ORD, int MODE>
+__global__ __launch_bounds__(256) void k_ko_add(const double* __restrict__ F,
+                                                double* __restrict__ OUT, Dims d, FDW w) {
+  using G = Geo<ORD, true>;
+  __shared__ double lds[2 * G::LVOL];
+  int ox, oy, oz, tx, ty, tz;
+  if (!brick_point<G>(d, &ox, &oy, &oz, &tx, &ty, &tz)) return;
+  const bool valid = (ox + tx < d.lo[0] + d.n[0]) && (oy + ty < d.lo[1] + d.n[1]) &&
+                     (oz + tz < d.lo[2] + d.n[2]);
+  BrickSrc<ORD, true, MODE> s(lds, F, w, d, ox, oy, oz, tx, ty, tz);
+#pragma unroll 1
+  for (int f = 0; f < BSSN_NF; ++f) {
+    s.stage(f);
+    const double k = s.ko();
+    if (valid) {
+      const long i = (long)f * d.P + s.gc;
+      OUT[i] = OUT[i] + k;
+    }
+  }
+}
+
+hipError_t launch_ko_add(const double* F, double* OUT, const Dims& d, const FDW& w, int ord,
+                         int mode, hipStream_t st) {
+  if (ord == 4) {
+    if (mode == WRAP)
+      hipLaunchKernelGGL((k_ko_add<4, WRAP>), dim3(brick_blocks<Geo<4, true>>(d)), dim3(256), 0,
+                         st, F, OUT, d, w);
+    else
+      hipLaunchKernelGGL((k_ko_add<4, CLAMP>), dim3(brick_blocks<Geo<4, true>>(d)), dim3(256), 0,
+                         st, F, OUT, d, w);
+  } else {
+    if (mode == WRAP)
+      hipLaunchKernelGGL((k_ko_add<6, WRAP>), dim3(brick_blocks<Geo<6, true>>(d)), dim3(256), 0,
+                         st, F, OUT, d, w);
+    else
+      hipLaunchKernelGGL((k_ko_add<6, CLAMP>), dim3(brick_blocks<Geo<6, true>>(d)), dim3(256), 0,
+                         st, F, OUT, d, w);
+  }
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------- periodic sync
+// Every ghost point takes the value of its periodic image in the interior. One pass suffices:
+// the sources are interior points, which is also what the reference's x-then-y-then-z slab
+// copies produce at edges and corners.
+__global__ __launch_bounds__(256) void k_periodic_sync(double* __restrict__ F, Dims d) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= d.P) return;
+  int c[3];
+  long r = p;
+  c[2] = (int)(r % d.S[2]);
+  r /= d.S[2];
+  c[1] = (int)(r % d.S[1]);
+  c[0] = (int)(r / d.S[1]);
+  bool ghost = false;
+  int s[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int N = d.S[a] - 2 * d.ng;
+    int v = c[a] - d.ng;
+    if (v < 0 || v >= N) {
+      ghost = true;
+      v %= N;
+      v = v < 0 ? v + N : v;
+    }
+    s[a] = d.ng + v;
+  }
+  if (!ghost) return;
+  const long q = ((long)s[0] * d.S[1] + s[1]) * d.S[2] + s[2];
+#pragma unroll 4
+  for (int f = 0; f < BSSN_NF; ++f) F[(long)f * d.P + p] = F[(long)f * d.P + q];
+}
+
+hipError_t launch_periodic_sync(double* F, const Dims& d, hipStream_t st) {
+  const unsigned nb = (unsigned)((d.P + 255) / 256);
+  hipLaunchKernelGGL(k_periodic_sync, dim3(nb), dim3(256), 0, st, F, d);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------- enforce
+__global__ __launch_bounds__(256) void k_enforce(double* __restrict__ F, long P, double chi_floor,
+                                                 double alpha_floor) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  double g[6], a[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    g[i] = F[(long)(GT0 + i) * P + p];
+    a[i] = F[(long)(AT0 + i) * P + p];
+  }
+  const double det = g[0] * (g[3] * g[5] - g[4] * g[4]) - g[1] * (g[1] * g[5] - g[4] * g[2]) +
+                     g[2] * (g[1] * g[4] - g[3] * g[2]);
+  const double scale = pow(det, -1.0 / 3.0);
+#pragma unroll
+  for (int i = 0; i < 6; ++i) g[i] = g[i] * scale;
+  const double iuxx = g[3] * g[5] - g[4] * g[4];
+  const double iuxy = g[2] * g[4] - g[1] * g[5];
+  const double iuxz = g[1] * g[4] - g[2] * g[3];
+  const double iuyy = g[0] * g[5] - g[2] * g[2];
+  const double iuyz = g[1] * g[2] - g[0] * g[4];
+  const double iuzz = g[0] * g[3] - g[1] * g[1];
+  const double trA = iuxx * a[0] + iuyy * a[3] + iuzz * a[5] +
+                     2.0 * (iuxy * a[1] + iuxz * a[2] + iuyz * a[4]);
+  const double third = trA / 3.0;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    F[(long)(GT0 + i) * P + p] = g[i];
+    F[(long)(AT0 + i) * P + p] = a[i] - g[i] * third;
+  }
+  const double c = F[(long)CHI * P + p];
+  F[(long)CHI * P + p] = c < chi_floor ? chi_floor : c;
+  const double al = F[(long)ALPHA * P + p];
+  F[(long)ALPHA * P + p] = al < alpha_floor ? alpha_floor : al;
+}
+
+hipError_t launch_enforce(double* F, long P, double chi_floor, double alpha_floor,
+                          hipStream_t st) {
+  const unsigned nb = (unsigned)((P + 255) / 256);
+  hipLaunchKernelGGL(k_enforce, dim3(nb), dim3(256), 0, st, F, P, chi_floor, alpha_floor);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------- constraints
+struct ConsOut {
+  double* HM;
+  long P, gp;
+  bool valid;
+  __device__ __forceinline__ void put(int c, double v) {
+    if (valid) HM[(long)c * P + gp] = v;
+  }
+};
+
+template <int ORD>
+__global__ __launch_bounds__(256) void k_constraints(const double* __restrict__ F,
+                                                     double* __restrict__ HM, Dims d, FDW w) {
+  using G = Geo<ORD, false>;
+  __shared__ double lds[2 * G::LVOL];
+  int ox, oy, oz, tx, ty, tz;
+  if (!brick_point<G>(d, &ox, &oy, &oz, &tx, &ty, &tz)) return;
+  const bool valid = (ox + tx < d.lo[0] + d.n[0]) && (oy + ty < d.lo[1] + d.n[1]) &&
+                     (oz + tz < d.lo[2] + d.n[2]);
+  BrickSrc<ORD, false, CLAMP> s(lds, F, w, d, ox, oy, oz, tx, ty, tz);
+  ConsOut o{HM, d.P, s.gc, valid};
+  bssn_constraints_point(s, o);
+}
+
+hipError_t launch_constraints(const double* F, double* HM, const Dims& d, const FDW& w, int ord,
+                              hipStream_t st) {
+  if (ord == 4)
+    hipLaunchKernelGGL(k_constraints<4>, dim3(brick_blocks<Geo<4, false>>(d)), dim3(256), 0, st,
+                       F, HM, d, w);
+  else
+    hipLaunchKernelGGL(k_constraints<6>, dim3(brick_blocks<Geo<6, false>>(d)), dim3(256), 0, st,
+                       F, HM, d, w);
+  return hipGetLastError();
+}
+
+// Interior sums of H^2 and |M|^2: one (sum H^2, sum |M|^2) pair per block, in a fixed order,
+// so the reduction is deterministic. Blocks cover (x, y) rows of the interior.
+int sumsq_blocks(const Dims& d) { return d.n[0] * d.n[1]; }
+
+__global__ __launch_bounds__(256) void k_sumsq(const double* __restrict__ HM, Dims d,
+                                               double* __restrict__ partial) {
+  __shared__ double sh[256], sm[256];
+  const int row = (int)blockIdx.x;
+  const int x = d.lo[0] + row / d.n[1], y = d.lo[1] + row % d.n[1];
+  double h = 0.0, m = 0.0;
+  for (int k = (int)threadIdx.x; k < d.n[2]; k += 256) {
+    const long p = ((long)x * d.S[1] + y) * d.S[2] + d.lo[2] + k;
+    const double H = HM[p], M0 = HM[d.P + p], M1 = HM[2 * d.P + p], M2 = HM[3 * d.P + p];
+    h += H * H;
+    m += M0 * M0 + M1 * M1 + M2 * M2;
+  }
+  sh[threadIdx.x] = h;
+  sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sh[threadIdx.x] += sh[threadIdx.x + s];
+      sm[threadIdx.x] += sm[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partial[2 * row] = sh[0];
+    partial[2 * row + 1] = sm[0];
+  }
+}
+
+hipError_t launch_sumsq(const double* HM, const Dims& d, double* partial, int nblk,
+                        hipStream_t st) {
+  hipLaunchKernelGGL(k_sumsq, dim3((unsigned)nblk), dim3(256), 0, st, HM, d, partial);
+  return hipGetLastError();
+}
+
+}  // namespace bssn

@@ -1,0 +1,172 @@
+// Fused BSSN right-hand side and fused RK4 / MSRK stage kernels for gfx950.
+//
+// One kernel evaluates, per output point, the 138 finite-difference derivatives (from LDS
+// bricks, bssn_brick.h), the Kreiss-Oliger terms, the pointwise BSSN algebra
+// (bssn_algebra.h) and the write-back: either the bare RHS, or an RK stage update
+// (next stage input + running combination), with the algebraic enforcement of the last stage
+// folded into the final store.
+//
+// Replaces, on the reference side:
+//   bare RHS    : the M4 kernel bssn_rhs_fused behind the FFI handler Fused
+//                 (3D/src/bssn3d/cuda/rhs_fused.cu:22-61, gen_fused.py:182-262)
+//   RK4 stages  : BSSNEvolution.step (3D/src/bssn3d/evolve.py:125-132) with rhs() =
+//                 periodic sync + RHS + KO (evolve.py:86-90) and enforce (evolve.py:93-119)
+//   MSRK stages : BSSNEvolution._msrk2_step (evolve.py:139-153)
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+
+#include "bssn_algebra.h"
+#include "bssn_brick.h"
+#include "bssn_kernels.h"
+#include "bssn_params.h"
+
+namespace bssn {
+
+__device__ __forceinline__ void enforce_point(double* g, double* a) {
+  // det(gt) = 1 and tr(At) = 0 (3D/src/bssn3d/evolve.py:93-119)
+  const double det = g[0] * (g[3] * g[5] - g[4] * g[4]) - g[1] * (g[1] * g[5] - g[4] * g[2]) +
+                     g[2] * (g[1] * g[4] - g[3] * g[2]);
+  const double scale = pow(det, -1.0 / 3.0);
+#pragma unroll
+  for (int i = 0; i < 6; ++i) g[i] = g[i] * scale;
+  const double iuxx = g[3] * g[5] - g[4] * g[4];
+  const double iuxy = g[2] * g[4] - g[1] * g[5];
+  const double iuxz = g[1] * g[4] - g[2] * g[3];
+  const double iuyy = g[0] * g[5] - g[2] * g[2];
+  const double iuyz = g[1] * g[2] - g[0] * g[4];
+  const double iuzz = g[0] * g[3] - g[1] * g[1];
+  const double trA = iuxx * a[0] + iuyy * a[3] + iuzz * a[5] +
+                     2.0 * (iuxy * a[1] + iuxz * a[2] + iuyz * a[4]);
+  const double third = trA / 3.0;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) a[i] = a[i] - g[i] * third;
+}
+
+struct StageOut {
+  const StageArgs& A;
+  long P, gp;
+  bool valid;
+  double gnew[6], anew[6];
+
+  __device__ __forceinline__ StageOut(const StageArgs& a, long P_, long gp_, bool v)
+      : A(a), P(P_), gp(gp_), valid(v) {}
+
+  __device__ __forceinline__ void final_value(int f, double u) {
+    if (f >= GT0 && f <= GT5) {
+      gnew[f - GT0] = u;
+    } else if (f >= AT0 && f <= AT5) {
+      anew[f - AT0] = u;
+    } else {
+      if (f == CHI) u = u < A.chi_floor ? A.chi_floor : u;
+      if (f == ALPHA) u = u < A.alpha_floor ? A.alpha_floor : u;
+      A.Uout[(long)f * P + gp] = u;
+    }
+  }
+
+  __device__ __forceinline__ void put(int f, double k) {
+    if (!valid) return;
+    const long i = (long)f * P + gp;
+    switch (A.kind) {
+      case KIND_RHS:
+        A.Y[i] = k;
+        break;
+      case KIND_RK1:                                  // y + (dt/2) k1 ; S = k1
+        A.Y[i] = A.U[i] + A.c0 * k;
+        A.K1[i] = k;
+        break;
+      case KIND_RK2:                                  // y + (dt/2) k2 ; S = S + 2 k2
+        A.Y[i] = A.U[i] + A.c0 * k;
+        A.K1[i] = A.K1[i] + 2.0 * k;
+        break;
+      case KIND_RK3:                                  // y + dt k3 ; S = S + 2 k3
+        A.Y[i] = A.U[i] + A.c0 * k;
+        A.K1[i] = A.K1[i] + 2.0 * k;
+        break;
+      case KIND_RK4:                                  // y + (dt/6) (S + k4)
+        final_value(f, A.U[i] + A.c0 * (A.K1[i] + k));
+        break;
+      case KIND_MS1: {                                // y + dt (a20 k0 + a21 k1) ; store k1
+        A.Y[i] = A.U[i] + A.dt * (A.c0 * A.FP[i] + A.c1 * k);
+        A.K1[i] = k;
+        break;
+      }
+      case KIND_MS2: {                                // y + dt (a30 k0 + a31 k1 + a32 k2)
+        const double k0 = A.FP[i], k1 = A.K1[i];
+        A.Y[i] = A.U[i] + A.dt * (A.c0 * k0 + A.c1 * k1 + A.c2 * k);
+        A.FP[i] = MS_B0 * k0 + MS_B1 * k1 + MS_B2 * k;
+        break;
+      }
+      case KIND_MS3:                                  // y + dt (b0 k0 + b1 k1 + b2 k2 + b3 k3)
+        final_value(f, A.U[i] + A.dt * (A.FP[i] + MS_B3 * k));
+        break;
+    }
+  }
+
+  __device__ __forceinline__ void finish() {
+    if (!valid || (A.kind != KIND_RK4 && A.kind != KIND_MS3)) return;
+    enforce_point(gnew, anew);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      A.Uout[(long)(GT0 + q) * P + gp] = gnew[q];
+      A.Uout[(long)(AT0 + q) * P + gp] = anew[q];
+    }
+  }
+};
+
+template <int ORD, bool KO, int MODE>
+__global__ __launch_bounds__(256) void k_fused(StageArgs args, Dims d, FDW w, Scal sc, int nbx,
+                                               int nby, int nbz) {
+  using G = Geo<ORD, KO>;
+  __shared__ double lds[2 * G::LVOL];
+  int bx, by, bz;
+  if (!brick_of_block(nbx, nby, nbz, &bx, &by, &bz)) return;
+  const int t = (int)threadIdx.x;
+  const int tz = t % G::BZ, ty = (t / G::BZ) % G::BY, tx = t / (G::BZ * G::BY);
+  const int ox = d.lo[0] + bx * G::BX, oy = d.lo[1] + by * G::BY, oz = d.lo[2] + bz * G::BZ;
+  const bool valid = (ox + tx < d.lo[0] + d.n[0]) && (oy + ty < d.lo[1] + d.n[1]) &&
+                     (oz + tz < d.lo[2] + d.n[2]);
+  BrickSrc<ORD, KO, MODE> src(lds, args.Yin, w, d, ox, oy, oz, tx, ty, tz);
+  StageOut out(args, d.P, src.gc, valid);
+  bssn_rhs_point<KO>(src, sc, out);
+  out.finish();
+}
+
+template <int ORD, bool KO, int MODE>
+static hipError_t launch_fused_t(const StageArgs& a, const Dims& d, const FDW& w, const Scal& sc,
+                                 hipStream_t st) {
+  using G = Geo<ORD, KO>;
+  const int nbx = (d.n[0] + G::BX - 1) / G::BX;
+  const int nby = (d.n[1] + G::BY - 1) / G::BY;
+  const int nbz = (d.n[2] + G::BZ - 1) / G::BZ;
+  const long nb = (long)nbx * nby * nbz;
+  const long nb8 = (nb + 7) & ~7L;
+  hipLaunchKernelGGL((k_fused<ORD, KO, MODE>), dim3((unsigned)nb8), dim3(G::NT), 0, st, a, d, w,
+                     sc, nbx, nby, nbz);
+  return hipGetLastError();
+}
+
+hipError_t launch_fused(const StageArgs& a, const Dims& d, const FDW& w, const Scal& sc, int ord,
+                        bool ko, int mode, hipStream_t st) {
+  if (ord == 4) {
+    if (ko) return mode == WRAP ? launch_fused_t<4, true, WRAP>(a, d, w, sc, st)
+                                : launch_fused_t<4, true, CLAMP>(a, d, w, sc, st);
+    return mode == WRAP ? launch_fused_t<4, false, WRAP>(a, d, w, sc, st)
+                        : launch_fused_t<4, false, CLAMP>(a, d, w, sc, st);
+  }
+  if (ko) return mode == WRAP ? launch_fused_t<6, true, WRAP>(a, d, w, sc, st)
+                              : launch_fused_t<6, true, CLAMP>(a, d, w, sc, st);
+  return mode == WRAP ? launch_fused_t<6, false, WRAP>(a, d, w, sc, st)
+                      : launch_fused_t<6, false, CLAMP>(a, d, w, sc, st);
+}
+
+int fused_halo(int ord, bool ko) {
+  if (ord == 4) return ko ? Geo<4, true>::H : Geo<4, false>::H;
+  return ko ? Geo<6, true>::H : Geo<6, false>::H;
+}
+
+int fused_brick(int axis) {
+  return axis == 0 ? Geo<4, true>::BX : (axis == 1 ? Geo<4, true>::BY : Geo<4, true>::BZ);
+}
+
+}  // namespace bssn

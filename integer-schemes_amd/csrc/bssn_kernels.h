@@ -1,0 +1,60 @@
+// Internal launch interface between the C ABI (bssn_abi.cpp) and the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "bssn_params.h"
+
+namespace bssn {
+
+enum StageKind : int {
+  KIND_RHS = 0,
+  KIND_RK1 = 1, KIND_RK2 = 2, KIND_RK3 = 3, KIND_RK4 = 4,
+  KIND_MS1 = 5, KIND_MS2 = 6, KIND_MS3 = 7
+};
+
+// MSRK rk4_2_2 (2 steps, 3 stages): Sanches, Brandt, Kalinani, Ji & Schnetter,
+// arXiv:2603.05763, Table 1, solution (2) -- the exact fractions the reference uses
+// (2D/src/mcs2d/msrk.py:47-52).
+constexpr double MS_A20 = 1309.0 / 15500.0, MS_A21 = -31999.0 / 15500.0;
+constexpr double MS_A30 = -241289.0 / 5880000.0, MS_A31 = 22846301.0 / 16170000.0,
+                 MS_A32 = -936169.0 / 2587200.0;
+constexpr double MS_B0 = -191.0 / 882.0, MS_B1 = 48241.0 / 59994.0,
+                 MS_B2 = 193750.0 / 4351347.0, MS_B3 = 100000.0 / 271791.0;
+
+struct StageArgs {
+  const double* Yin;   // stencil source (state the RHS is evaluated on)
+  const double* U;     // step base state y_n (pointwise)
+  double* Y;           // next stage input, or the RHS itself (KIND_RHS)
+  double* K1;          // RK4: running k1 + 2k2 + 2k3;  MSRK: k1 = f(y_n)
+  double* FP;          // MSRK: f(y_{n-1}) on entry, partial combination after stage 2
+  double* Uout;        // y_{n+1} (may alias U)
+  int kind;
+  double c0, c1, c2;   // per-kind coefficients
+  double dt;
+  double chi_floor, alpha_floor;
+};
+
+hipError_t launch_fused(const StageArgs& a, const Dims& d, const FDW& w, const Scal& sc, int ord,
+                        bool ko, int mode, hipStream_t st);
+int fused_halo(int ord, bool ko);
+
+// two-pass (materialised derivative bundle) path
+hipError_t launch_deriv_bundle(const double* F, double* D, const Dims& d, const FDW& w, int ord,
+                               hipStream_t st);
+hipError_t launch_algebra(const double* F, const double* D, double* OUT, long npts,
+                          const Scal& sc, hipStream_t st);
+hipError_t launch_ko_add(const double* F, double* OUT, const Dims& d, const FDW& w, int ord,
+                         int mode, hipStream_t st);
+
+// auxiliary
+hipError_t launch_periodic_sync(double* F, const Dims& d, hipStream_t st);
+hipError_t launch_enforce(double* F, long P, double chi_floor, double alpha_floor,
+                          hipStream_t st);
+hipError_t launch_constraints(const double* F, double* HM, const Dims& d, const FDW& w, int ord,
+                              hipStream_t st);
+hipError_t launch_sumsq(const double* HM, const Dims& d, double* partial, int nblk,
+                        hipStream_t st);
+int sumsq_blocks(const Dims& d);
+
+}  // namespace bssn

@@ -1,0 +1,244 @@
+"""GPU: the HIP library, through its C ABI, against the oracle and the reference's golden vectors.
+
+Bar: per-field normwise relative error max|hip - ref| / max|ref| <= 1e-12 (summation-order and
+FMA differences only); ghost-sync copies bit-exact. Cases cover the reference's own sizes and
+edge cases: the golden point vectors (reference CSE), a padded golden grid with edge padding,
+the 48^3 single-puncture workload, ragged non-cubic extents that do not divide the 4x8x8
+brick, FD orders 4 and 6, KO on and off, periodic and edge-padded steps, RK4 and MSRK, the
+constraints, and the 256^3 binary workload through sampled points and size-independent
+properties.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from conftest import gpu_available, normwise
+
+from bssn3d_hip import initial_data as idata
+from bssn3d_hip.constraints import ConstraintSolver
+from bssn3d_hip.evolve import BSSNEvolution
+from bssn3d_hip.grid import Grid
+from bssn3d_hip.rhs import BSSNSolver, rhs_into, rhs_twopass_into, stream_handle
+from bssn3d_hip import _lib
+from bssn3d_hip.state import PhysicsParams, pack_S, scalars
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not gpu_available(), reason="needs a HIP device")]
+
+TOL = 1e-12
+DEV = "cuda:0"
+
+
+def say(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def to_dev(a: np.ndarray) -> torch.Tensor:
+    return torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+
+
+def to_host(t: torch.Tensor) -> np.ndarray:
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def smooth_state(shape, seed=0, amp=0.05):
+    rng = np.random.default_rng(seed)
+    S = shape
+    axes = [np.arange(n) / n for n in S]
+    X, Y, Z = np.meshgrid(*axes, indexing="ij")
+    F = np.zeros((24,) + tuple(S))
+    for f in range(24):
+        k = rng.integers(1, 3, size=3)
+        F[f] = amp * np.sin(2 * np.pi * (k[0] * X + k[1] * Y + k[2] * Z) + rng.uniform(0, 6))
+    for f in (0, 1, 3, 6, 8):
+        F[f] += 1.0
+    return F
+
+
+def test_algebra_kernel_matches_reference_cse(golden_points):
+    say("algebra kernel vs reference CSE, 32 seeded points")
+    g = golden_points
+    sc = g["sc13"]
+    # sc13 = eta, lambda0..3, lambda_f0..1, C, dt, dx_i, h_ssl, sig, t -> the 16-pack
+    S16 = pack_S(sc[0], sc[7], sc[8], sc[9], sc[10], sc[11], sc[12], sc[1:5], sc[5:7],
+                 sc[9], sc[9])
+    F = to_dev(g["F"])
+    D = to_dev(g["D"])
+    out = torch.empty_like(F)
+    n = F.shape[1]
+    _lib.check(_lib.load().bssn_algebra_f64(F.data_ptr(), D.data_ptr(), _lib.dbl(S16),
+                                            out.data_ptr(), n, stream_handle(F.device)))
+    got, ref = to_host(out), g["rhs"]
+    rel = np.abs(got - ref) / np.maximum(np.maximum(np.abs(got), np.abs(ref)), 1e-12)
+    assert rel.max() < TOL, rel.max()
+
+
+@pytest.mark.parametrize("order", [4, 6])
+def test_fused_rhs_golden_grid(golden_grid, order):
+    say(f"fused RHS vs reference CSE on the golden 14^3 padded grid, order {order}")
+    g = golden_grid
+    F = to_dev(g["F"])
+    out = torch.empty_like(F)
+    rhs_into(F, out, list(g["S16"]), order)
+    err = normwise(to_host(out), g[f"rhs{order}"])
+    assert err.max() < TOL, err
+
+
+@pytest.mark.parametrize("ko_sigma", [0.0, 0.1])
+def test_fused_rhs_single_puncture(ko_sigma):
+    say(f"fused RHS, 48^3 single puncture, ko_sigma={ko_sigma}")
+    grid, st, _ = idata.config_single_puncture(seed=0, N=48)
+    F = st.numpy()
+    S16 = scalars(PhysicsParams(), grid.dx, grid.dy, grid.dz, 0.25 * grid.dx, 0.0)
+    ref = O.rhs(F, S16, 4, ko_sigma=ko_sigma)
+    Fd = to_dev(F)
+    out = torch.empty_like(Fd)
+    rhs_into(Fd, out, S16, 4, ko_sigma)
+    err = normwise(to_host(out), ref)
+    assert err.max() < TOL, err
+
+
+def test_twopass_matches_oracle():
+    say("two-pass bundle + algebra + KO, 24^3 single puncture")
+    grid, st, _ = idata.config_single_puncture(seed=2, N=24)
+    F = st.numpy()
+    S16 = scalars(PhysicsParams(), grid.dx, grid.dy, grid.dz, 0.25 * grid.dx, 0.7)
+    h3 = [grid.dx, grid.dy, grid.dz]
+    Fd = to_dev(F)
+    D = torch.empty((138,) + F.shape[1:], dtype=torch.float64, device=DEV)
+    out = torch.empty_like(Fd)
+    rhs_twopass_into(Fd, out, D, S16, 4, 0.1)
+    Dref = O.deriv_bundle(F, h3, 4)
+    assert normwise(to_host(D), Dref).max() < TOL
+    err = normwise(to_host(out), O.rhs(F, S16, 4, ko_sigma=0.1))
+    assert err.max() < TOL, err
+
+
+@pytest.mark.parametrize("shape,order", [((13, 17, 22), 4), ((9, 11, 21), 6), ((5, 6, 7), 4)])
+def test_ragged_shapes(shape, order):
+    say(f"fused RHS on ragged extents {shape}, order {order}")
+    F = smooth_state(shape, seed=sum(shape))
+    S16 = [2.0, 0.06, 0.02, 0.1, 0.6, 20.0, 0.4, 1.1, 0.9, 1.2, 0.8, 1.0, 0.5, 0.1, 0.11, 0.09]
+    Fd = to_dev(F)
+    out = torch.empty_like(Fd)
+    rhs_into(Fd, out, S16, order, 0.05)
+    err = normwise(to_host(out), O.rhs(F, S16, order, ko_sigma=0.05))
+    assert err.max() < TOL, err
+
+
+@pytest.mark.parametrize("integrator,nsteps", [("rk4", 1), ("rk4", 3), ("rk4_2_2", 4)])
+def test_evolution_matches_oracle(integrator, nsteps):
+    say(f"{integrator} evolve {nsteps} steps, periodic, puncture 20^3")
+    grid, st, _ = idata.config_single_puncture(seed=4, N=20)
+    F = st.numpy()
+    dt = 0.25 * grid.dx
+    p = PhysicsParams()
+    evo = BSSNEvolution(grid, p, order=4, ko_sigma=0.1, bc="periodic", integrator=integrator)
+    got = to_host(evo.evolve(to_dev(F), dt, nsteps, t0=0.2).data)
+    S16 = scalars(p, grid.dx, grid.dy, grid.dz, dt, 0.2)
+    ref = O.evolve(F, S16, dt, nsteps, t0=0.2, integrator=integrator, order=4, ko_sigma=0.1, ng=3)
+    ref = O.periodic_sync(ref, 3)   # the reference state after the sync its next step performs
+    err = normwise(got, ref)
+    assert err.max() < TOL, err
+
+
+def test_step_edge_padded():
+    say("rk4 step with bc='none' (edge-padded stencils, all padded points advanced)")
+    grid, st, _ = idata.config_single_puncture(seed=5, N=12)
+    F = st.numpy()
+    dt = 0.25 * grid.dx
+    p = PhysicsParams()
+    evo = BSSNEvolution(grid, p, order=4, ko_sigma=0.1, bc="none")
+    got = to_host(evo.step(to_dev(F), 0.0, dt).data)
+    S16 = scalars(p, grid.dx, grid.dy, grid.dz, dt, 0.0)
+    ref = O.rk4_step(F, S16, 0.0, dt, order=4, ko_sigma=0.1, ng=3, periodic=False)
+    assert normwise(got, ref).max() < TOL
+
+
+def test_order6_step():
+    say("rk4 step, order 6 with KO 8, ng 4")
+    N = 12
+    grid = Grid.from_domain(N, order=6, ko_order=8)
+    F = idata.gauge_wave(grid, amplitude=0.02).numpy()
+    dt = 0.25 * grid.dx
+    p = PhysicsParams()
+    evo = BSSNEvolution(grid, p, order=6, ko_sigma=0.1, bc="periodic")
+    got = to_host(evo.step(to_dev(F), 0.0, dt).data)
+    ref = O.periodic_sync(O.rk4_step(F, scalars(p, grid.dx, grid.dy, grid.dz, dt, 0.0), 0.0, dt,
+                                     order=6, ko_sigma=0.1, ng=4), 4)
+    assert normwise(got, ref).max() < TOL
+
+
+def test_sync_and_enforce_kernels():
+    say("periodic sync (bit-exact) and enforce kernels")
+    rng = np.random.default_rng(9)
+    F = smooth_state((11, 12, 13), seed=9) + 0.01 * rng.normal(size=(24, 11, 12, 13))
+    d = to_dev(F)
+    lib = _lib.load()
+    _lib.check(lib.bssn_periodic_sync_f64(d.data_ptr(), 11, 12, 13, 3, stream_handle(d.device)))
+    np.testing.assert_array_equal(to_host(d), O.periodic_sync(F, 3))
+    d = to_dev(F)
+    _lib.check(lib.bssn_enforce_f64(d.data_ptr(), 11 * 12 * 13, 1e-4, 1e-4, stream_handle(d.device)))
+    assert normwise(to_host(d), O.enforce(F)).max() < TOL
+
+
+def test_constraints_match_oracle():
+    say("constraints and interior norms, 24^3 single puncture, order 4")
+    grid, st, _ = idata.config_single_puncture(seed=6, N=24)
+    F = st.numpy()
+    cs = ConstraintSolver(grid, order=4)
+    got = cs.constraints(to_dev(F))
+    HM = np.stack([to_host(got[k]) for k in ("H", "M0", "M1", "M2")])
+    h3 = [grid.dx, grid.dy, grid.dz]
+    assert normwise(HM, O.constraints(F, h3, 4)).max() < TOL
+    H, M = cs.l2(to_dev(F))
+    Hr, Mr = O.constraint_l2(F, h3, 4, grid.ng)
+    assert abs(H - Hr) <= 1e-12 * Hr and abs(M - Mr) <= 1e-12 * Mr
+
+
+def test_solver_interface():
+    say("BSSNSolver.rhs_dict fused == twopass, device_rhs_fused")
+    grid, st, _ = idata.config_single_puncture(seed=8, N=16)
+    F = to_dev(st.numpy())
+    a = BSSNSolver(grid, order=4, scheme="cuda_fused").rhs(F, t=0.3).data
+    b = BSSNSolver(grid, order=4, scheme="verbatim").rhs(F, t=0.3).data
+    assert normwise(to_host(a), to_host(b)).max() < TOL
+    from bssn3d_hip.rhs import device_rhs_fused
+    d = device_rhs_fused(F, PhysicsParams(), grid.dx, grid.dy, grid.dz, 0.25 * grid.dx, 0.3,
+                         order=4)
+    np.testing.assert_array_equal(to_host(d["alpha"]), to_host(a[0]))
+
+
+def test_full_size_binary_sampled():
+    """The 256^3 binary at full size: sampled points against the oracle (edges, faces and
+    corners included), Minkowski staticity and bitwise determinism."""
+    say("256^3 binary workload: sampled parity + properties")
+    grid, st, _ = idata.config_binary(seed=0, N=256, device=DEV)
+    S16 = scalars(PhysicsParams(), grid.dx, grid.dy, grid.dz, 0.25 * grid.dx, 0.0)
+    out = torch.empty_like(st)
+    rhs_into(st, out, S16, 4)
+    out2 = torch.empty_like(st)
+    rhs_into(st, out2, S16, 4)
+    torch.cuda.synchronize()
+    assert torch.equal(out, out2)
+    Sx, Sy, Sz = grid.shape
+    rng = np.random.default_rng(1)
+    pts = [rng.integers(0, n, size=3000) for n in (Sx, Sy, Sz)]
+    corners = np.array([[x, y, z] for x in (0, 1, Sx - 2, Sx - 1) for y in (0, 2, Sy - 1)
+                        for z in (0, 3, Sz - 1)]).T
+    P = np.concatenate([np.stack(pts), corners], axis=1)
+    idx = (P[0] * Sy + P[1]) * Sz + P[2]
+    F = to_host(st)
+    ref = O.rhs_points(F, S16, idx, 4)
+    got = to_host(out).reshape(24, -1)[:, idx]
+    err = normwise(got, ref)
+    assert err.max() < TOL, err
+    del F
+    mink = idata.minkowski(grid, device=DEV)
+    rhs_into(mink, out, S16, 4, 0.1)
+    torch.cuda.synchronize()
+    assert float(out.abs().max()) < 1e-12

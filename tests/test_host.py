@@ -1,0 +1,138 @@
+"""CPU: the C ABI library loads and exports what include/bssn_hip.h declares, its argument
+checks fire before any device work, and the Python host mirror keeps the reference's
+conventions (scalar pack, output order, grid, initial data, error behaviour)."""
+from __future__ import annotations
+
+import ctypes
+import math
+import re
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import ROOT
+
+from bssn3d_hip import _lib, initial_data as idata
+from bssn3d_hip.grid import Grid, ghost_width
+from bssn3d_hip.rhs import BSSNSolver, as_tensor
+from bssn3d_hip.state import PhysicsParams, VAR_NAMES, pack_S, scalars
+
+HEADER = ROOT / "include" / "bssn_hip.h"
+
+
+def declared_symbols():
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"^(?:int|long|size_t|const char\*)\s+(bssn_\w+)\(", text, re.M)))
+
+
+def test_header_and_binding_agree():
+    assert declared_symbols() == sorted(_lib.SIGNATURES)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(str(_lib.LIB_PATH))
+    for name in declared_symbols():
+        assert hasattr(lib, name), name
+    assert _lib.load().bssn_abi_version() == _lib.ABI_VERSION
+
+
+def test_argument_errors_before_device_work():
+    lib = _lib.load()
+    S = _lib.dbl(pack_S(2.0, 0.06, 0.01, 0.1, 0.6, 20.0, 0.0, (1, 1, 1, 1), (1, 0), 0.1, 0.1))
+    # null pointers, bad order, bad shape, bad sigma: BSSN_E_ARG with a message
+    assert lib.bssn_rhs_f64(None, S, None, 8, 8, 8, 4, 0.0, None) == 1
+    assert b"null" in lib.bssn_last_error()
+    fake = ctypes.c_void_p(16)
+    assert lib.bssn_rhs_f64(fake, S, fake, 8, 8, 8, 5, 0.0, None) == 1
+    assert b"fd_order" in lib.bssn_last_error()
+    assert lib.bssn_rhs_f64(fake, S, fake, 0, 8, 8, 4, 0.0, None) == 1
+    assert lib.bssn_rhs_f64(fake, S, fake, 8, 8, 8, 4, -1.0, None) == 1
+    bad = _lib.dbl([2.0, 0.06, 0.0] + [0.1] * 13)   # dt = 0: CAHD divides by it
+    assert lib.bssn_rhs_f64(fake, bad, fake, 8, 8, 8, 4, 0.0, None) == 1
+    assert b"dt" in lib.bssn_last_error()
+    assert lib.bssn_rk4_step_f64(fake, fake, 8, 8, 8, S, 4, 0.1, 3, 1, 0.0, 0.0, 1e-4, 1e-4, None) == 1
+    assert lib.bssn_periodic_sync_f64(fake, 5, 5, 5, 3, None) == 1   # interior < ng
+    assert lib.bssn_msrk_step_f64(fake, fake, fake, fake, 12, 12, 12, S, 4, 0.1, 3, 1, 0.0, 0.1,
+                                  1e-4, 1e-4, None) == 1
+    assert b"distinct" in lib.bssn_last_error()
+    assert lib.bssn_step_workspace_bytes(0, 10, 10, 10) == 3 * 24 * 1000 * 8
+    assert lib.bssn_step_workspace_bytes(1, 10, 10, 10) == 2 * 24 * 1000 * 8
+
+
+def test_library_errors_raise():
+    with pytest.raises(_lib.BSSNError, match="fd_order"):
+        _lib.check(_lib.load().bssn_rhs_f64(ctypes.c_void_p(16), _lib.dbl([1.0] * 16),
+                                            ctypes.c_void_p(16), 8, 8, 8, 3, 0.0, None))
+
+
+def test_scalar_pack_and_output_map():
+    """16 scalars in the reference's order (fused_rhs_cuda.py:52-55); 24 outputs in state order."""
+    s = scalars(PhysicsParams(), 0.06, 0.07, 0.08, 0.01, 0.5)
+    assert len(s) == 16
+    assert s[:7] == [2.0, 0.06, 0.01, 0.06, 0.6, 20.0, 0.5]
+    assert s[7:13] == [1.0, 1.0, 1.0, 1.0, 1.0, 0.0]
+    assert s[13:] == [0.06, 0.07, 0.08]
+    assert VAR_NAMES[0] == "alpha" and len(VAR_NAMES) == 24
+
+
+def test_missing_library_error(monkeypatch, tmp_path):
+    monkeypatch.setattr(_lib, "LIB_PATH", tmp_path / "nope.so")
+    monkeypatch.setattr(_lib, "_lib", None)
+    with pytest.raises(FileNotFoundError, match="build.py"):
+        _lib.load()
+
+
+def test_host_tensors_rejected():
+    g = Grid.from_domain(8, order=4, ko_order=6)
+    s = idata.minkowski(g)
+    with pytest.raises(ValueError, match="HIP device"):
+        as_tensor(s)
+    with pytest.raises(TypeError):
+        as_tensor(s.float())
+    with pytest.raises(ValueError, match="shape"):
+        as_tensor(torch.zeros(23, 4, 4, 4, dtype=torch.float64))
+
+
+def test_solver_validation():
+    g = Grid.from_domain(8, order=4, ko_order=6)
+    with pytest.raises(ValueError, match="ng"):
+        BSSNSolver(g, order=6)
+    with pytest.raises(ValueError, match="scheme"):
+        BSSNSolver(g, order=4, scheme="nope")
+    with pytest.raises(ValueError, match="precision"):
+        BSSNSolver(g, order=4, precision="fp32_contraction")
+    s = BSSNSolver(g, order=4, scheme="verbatim")
+    assert s.scheme == "twopass" and s.dt == 0.25 * g.dx
+
+
+def test_grid_matches_reference_convention():
+    g = Grid.from_domain(16, order=6)
+    assert g.ng == 4 and g.shape == (24, 24, 24)
+    assert g.x[g.ng] == -0.5 and abs(g.x[1] - g.x[0] - 1 / 16) < 1e-15
+    assert ghost_width(4, 6) == 3
+
+
+def test_puncture_closed_form():
+    g, st, pos = idata.config_single_puncture(seed=1, N=12)
+    d = st.numpy()
+    x, y, z = g.mesh()
+    r = np.sqrt((x - pos[0][0]) ** 2 + (y - pos[0][1]) ** 2 + (z - pos[0][2]) ** 2)
+    psi = 1 + 1.0 / (2 * np.maximum(r, g.dx / 2))
+    # perturbations are at most 1e-4
+    assert np.abs(d[1] - psi ** -4).max() <= 1e-4 + 1e-15
+    assert np.abs(d[0] - psi ** -2).max() <= 1e-4 + 1e-15
+    assert np.abs(d[3] - 1).max() <= 1e-4 and np.abs(d[4]).max() <= 1e-4
+    assert np.abs(d[2]).max() > 0   # every field is perturbed
+    # punctures sit at least h/4 off the grid planes
+    for c, ax in zip(pos[0], (g.x, g.y, g.z)):
+        assert np.min(np.abs(ax - c)) >= 0.25 * g.dx - 1e-12
+    # the perturbation of one field is periodic on the domain (integer wavenumbers)
+    pert = idata.Perturbation.draw(np.random.default_rng(3))
+    assert np.all(np.abs(pert.k).max(axis=2) >= 1) and np.all(np.abs(pert.k) <= 4)
+
+
+def test_binary_config_shape():
+    g, st, pos = idata.config_binary(seed=0, N=16, L=16.0)
+    assert st.shape == (24, 22, 22, 22) and g.dx == 1.0
+    assert abs((pos[1][0] - pos[0][0]) - 6.0) < 1e-12
