@@ -83,6 +83,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-twopass", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy-gib", type=float, default=0.0,
+                    help="also run one plain device copy of this many GiB (profiler calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -185,6 +187,14 @@ def main() -> int:
     roof["fp64_frac"] = round(fp64_frac, 4)
     roof["hbm_frac"] = round(hbm_frac, 4)
     roof["ms_per_launch"] = round(ms_stage, 4)
+
+    if args.copy_gib > 0:
+        n = int(args.copy_gib * (1 << 30)) // 8
+        a = torch.ones(n, dtype=torch.float64, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     value = npts / (ms_step * 1e-3) / 1e6
     res = {

@@ -136,11 +136,11 @@ def offgrid_shift(rng: np.random.Generator, h: float) -> np.ndarray:
 
 # --- the workload configurations -------------------------------------------------------------
 
-def config_single_puncture(seed: int = 0, N: int = 48, device="cpu"):
+def config_single_puncture(seed: int = 0, N: int = 48, device="cpu", ng: int = 3):
     """48^3 interior, 3 ghosts, h = 0.25 on [-6, 6): one puncture m = 1 at a seeded off-grid
     position, 1e-4 perturbations. Returns (grid, state, positions)."""
     h = 12.0 / N
-    grid = Grid(N, N, N, h, h, h, ng=3, xmin=-6.0, ymin=-6.0, zmin=-6.0)
+    grid = Grid(N, N, N, h, h, h, ng=ng, xmin=-6.0, ymin=-6.0, zmin=-6.0)
     rng = np.random.default_rng(seed)
     pos = offgrid_shift(rng, h)
     state = punctures(grid, [1.0], [pos], amp=1e-4, seed=seed + 1000, device=device)

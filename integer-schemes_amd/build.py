@@ -4,6 +4,7 @@
 
 Each HIP source is compiled separately (in parallel) for --offload-arch=gfx950 and linked into
 integer-schemes_amd/bssn3d_hip/libbssn_hip.so, which the Python host layer loads.
+``build(out=..., defines=[...])`` builds a variant with extra -D knobs into another path.
 """
 from __future__ import annotations
 
@@ -18,7 +19,6 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 OUT = PKG / "bssn3d_hip" / "libbssn_hip.so"
-OBJ = PKG / "_build"
 ARCH = os.environ.get("BSSN_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["bssn_fused.hip", "bssn_aux.hip", "bssn_abi.cpp"]
@@ -37,28 +37,32 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found (set HIPCC or install ROCm)")
 
 
-def _compile(src: str) -> Path:
-    obj = OBJ / (Path(src).stem + ".o")
-    cmd = [hipcc(), *HIP_FLAGS, "-c", str(CSRC / src), "-o", str(obj)]
+def _compile(src: str, objdir: Path, defines) -> Path:
+    obj = objdir / (Path(src).stem + ".o")
+    cmd = [hipcc(), *HIP_FLAGS, *[f"-D{d}" for d in defines], "-c", str(CSRC / src), "-o",
+           str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
     return obj
 
 
-def build(verbose: bool = False) -> Path:
-    OBJ.mkdir(exist_ok=True)
+def build(verbose: bool = False, out: Path | None = None, defines=()) -> Path:
+    out = Path(out) if out is not None else OUT
+    objdir = PKG / "_build" / (out.parent.name if out != OUT else "lib")
+    objdir.mkdir(parents=True, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 1)) as ex:
-        objs = list(ex.map(_compile, SOURCES))
-    tmp = OUT.with_suffix(".so.tmp")
+        objs = list(ex.map(lambda s: _compile(s, objdir, defines), SOURCES))
+    out.parent.mkdir(parents=True, exist_ok=True)
+    tmp = out.with_suffix(".so.tmp")
     cmd = [hipcc(), "-shared", f"--offload-arch={ARCH}", "-fPIC", *map(str, objs), "-o", str(tmp)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
-    os.replace(tmp, OUT)
+    os.replace(tmp, out)
     if verbose:
-        print(f"built {OUT}")
-    return OUT
+        print(f"built {out}")
+    return out
 
 
 if __name__ == "__main__":

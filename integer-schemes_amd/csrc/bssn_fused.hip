@@ -114,8 +114,15 @@ struct StageOut {
   }
 };
 
+// Waves per SIMD the register allocator targets (build knob; 1 = the whole 512-entry
+// VGPR+AGPR file per lane).
+#ifndef BSSN_FUSED_WAVES
+#define BSSN_FUSED_WAVES 2
+#endif
+
 template <int ORD, bool KO, int MODE>
-__global__ __launch_bounds__(256) void k_fused(StageArgs args, Dims d, FDW w, Scal sc, int nbx,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BSSN_FUSED_WAVES, BSSN_FUSED_WAVES)))
+void k_fused(StageArgs args, Dims d, FDW w, Scal sc, int nbx,
                                                int nby, int nbz) {
   using G = Geo<ORD, KO>;
   __shared__ double lds[2 * G::LVOL];

@@ -32,6 +32,16 @@
 #include <stdlib.h>
 #include <string.h>
 
+#ifdef ORC_LONG
+#define SQRT sqrtl
+#define EXP expl
+#define POW powl
+#else
+#define SQRT sqrt
+#define EXP exp
+#define POW pow
+#endif
+
 #define NF 24
 #define ND 138
 
@@ -50,18 +60,18 @@ int orc_version(void) { return 1; }
 
 /* ---------------------------------------------------------------- stencils */
 
-int orc_stencil(int kind, int order, double* c) {
-  static const double c1_4[5] = {1, -8, 0, 8, -1};
-  static const double c1_6[7] = {-1, 9, -45, 0, 45, -9, 1};
-  static const double c1_8[9] = {3, -32, 168, -672, 0, 672, -168, 32, -3};
-  static const double c2_4[5] = {-1, 16, -30, 16, -1};
-  static const double c2_6[7] = {2, -27, 270, -490, 270, -27, 2};
-  static const double c2_8[9] = {-9, 128, -1008, 8064, -14350, 8064, -1008, 128, -9};
-  static const double ko_4[5] = {-1, 4, -6, 4, -1};
-  static const double ko_6[7] = {1, -6, 15, -20, 15, -6, 1};
-  static const double ko_8[9] = {-1, 8, -28, 56, -70, 56, -28, 8, -1};
-  const double* src = NULL;
-  double den = 1.0;
+int orc_stencil(int kind, int order, real* c) {
+  static const real c1_4[5] = {1, -8, 0, 8, -1};
+  static const real c1_6[7] = {-1, 9, -45, 0, 45, -9, 1};
+  static const real c1_8[9] = {3, -32, 168, -672, 0, 672, -168, 32, -3};
+  static const real c2_4[5] = {-1, 16, -30, 16, -1};
+  static const real c2_6[7] = {2, -27, 270, -490, 270, -27, 2};
+  static const real c2_8[9] = {-9, 128, -1008, 8064, -14350, 8064, -1008, 128, -9};
+  static const real ko_4[5] = {-1, 4, -6, 4, -1};
+  static const real ko_6[7] = {1, -6, 15, -20, 15, -6, 1};
+  static const real ko_8[9] = {-1, 8, -28, 56, -70, 56, -28, 8, -1};
+  const real* src = NULL;
+  real den = 1.0;
   if (kind == 0) {
     if (order == 4) { src = c1_4; den = 12.0; }
     if (order == 6) { src = c1_6; den = 60.0; }
@@ -92,7 +102,7 @@ static inline long lin(const Dims* d, int x, int y, int z) {
 }
 
 /* value of u at p + off*e_axis, clamped along that axis */
-static inline double tap(const double* u, const Dims* d, const int p[3], int axis, int off) {
+static inline real tap(const real* u, const Dims* d, const int p[3], int axis, int off) {
   int q[3] = {p[0], p[1], p[2]};
   q[axis] = clampi(q[axis] + off, d->S[axis] - 1);
   return u[lin(d, q[0], q[1], q[2])];
@@ -100,19 +110,19 @@ static inline double tap(const double* u, const Dims* d, const int p[3], int axi
 
 /* one-axis stencil at one point: (coeffs * factor) first, then a left-to-right sum that
  * includes every tap (derivatives.py:79,87-88) */
-static double stencil_at(const double* u, const Dims* d, const int p[3], int axis,
-                         const double* w, int ntap) {
+static real stencil_at(const real* u, const Dims* d, const int p[3], int axis,
+                         const real* w, int ntap) {
   const int r = (ntap - 1) / 2;
-  double s = 0.0;
+  real s = 0.0;
   for (int k = 0; k < ntap; ++k) s += w[k] * tap(u, d, p, axis, k - r);
   return s;
 }
 
 /* mixed derivative d1_j(d1_i u) with i < j, each application edge-padded on its own axis */
-static double mixed_at(const double* u, const Dims* d, const int p[3], int i, int j,
-                       const double* wi, const double* wj, int ntap) {
+static real mixed_at(const real* u, const Dims* d, const int p[3], int i, int j,
+                       const real* wi, const real* wj, int ntap) {
   const int r = (ntap - 1) / 2;
-  double s = 0.0;
+  real s = 0.0;
   for (int b = 0; b < ntap; ++b) {
     int q[3] = {p[0], p[1], p[2]};
     q[j] = clampi(q[j] + b - r, d->S[j] - 1);
@@ -121,9 +131,9 @@ static double mixed_at(const double* u, const Dims* d, const int p[3], int i, in
   return s;
 }
 
-void orc_apply_axis(const double* u, double* out, int Sx, int Sy, int Sz, int axis, int kind,
-                    int order, double factor) {
-  double c[9], w[9];
+void orc_apply_axis(const real* u, real* out, int Sx, int Sy, int Sz, int axis, int kind,
+                    int order, real factor) {
+  real c[9], w[9];
   const int nt = orc_stencil(kind, order, c);
   if (nt < 0) return;
   for (int k = 0; k < nt; ++k) w[k] = c[k] * factor;
@@ -139,12 +149,12 @@ void orc_apply_axis(const double* u, double* out, int Sx, int Sy, int Sz, int ax
 
 typedef struct {
   int nt;
-  double w1[3][9]; /* first-derivative weights per axis (C1 / h) */
-  double w2[3][9]; /* second-derivative weights per axis (C2 / h^2) */
+  real w1[3][9]; /* first-derivative weights per axis (C1 / h) */
+  real w2[3][9]; /* second-derivative weights per axis (C2 / h^2) */
 } FDW;
 
-static int fd_weights(FDW* fw, const double* h3, int order) {
-  double c1[9], c2[9];
+static int fd_weights(FDW* fw, const real* h3, int order) {
+  real c1[9], c2[9];
   fw->nt = orc_stencil(0, order, c1);
   if (fw->nt < 0 || orc_stencil(1, order, c2) < 0) return -1;
   for (int a = 0; a < 3; ++a)
@@ -156,18 +166,18 @@ static int fd_weights(FDW* fw, const double* h3, int order) {
 }
 
 /* all 138 derivatives at one point; Dp[d * stride] */
-static void point_derivs(const double* F, const Dims* d, const FDW* fw, const int p[3],
-                         double* Dp, long stride) {
+static void point_derivs(const real* F, const Dims* d, const FDW* fw, const int p[3],
+                         real* Dp, long stride) {
   for (int f = 0; f < NF; ++f) {
-    const double* u = F + (long)f * d->P;
+    const real* u = F + (long)f * d->P;
     for (int a = 0; a < 3; ++a)
       Dp[(3 * f + a) * stride] = stencil_at(u, d, p, a, fw->w1[a], fw->nt);
   }
   for (int s = 0; s < 11; ++s) {
-    const double* u = F + (long)S2F[s] * d->P;
+    const real* u = F + (long)S2F[s] * d->P;
     for (int q = 0; q < 6; ++q) {
       const int i = PAIR_I[q], j = PAIR_J[q];
-      double v;
+      real v;
       if (i == j)
         v = stencil_at(u, d, p, i, fw->w2[i], fw->nt);
       else
@@ -177,7 +187,7 @@ static void point_derivs(const double* F, const Dims* d, const FDW* fw, const in
   }
 }
 
-void orc_deriv_bundle(const double* F, double* D, int Sx, int Sy, int Sz, const double* h3,
+void orc_deriv_bundle(const real* F, real* D, int Sx, int Sy, int Sz, const real* h3,
                       int order) {
   FDW fw;
   if (fd_weights(&fw, h3, order)) return;
@@ -198,8 +208,8 @@ static void unlin(const Dims* d, long idx, int p[3]) {
   p[0] = (int)(idx / d->S[1]);
 }
 
-void orc_deriv_points(const double* F, int Sx, int Sy, int Sz, const double* h3, int order,
-                      long npts, const long* idx, double* Dp) {
+void orc_deriv_points(const real* F, int Sx, int Sy, int Sz, const real* h3, int order,
+                      long npts, const long* idx, real* Dp) {
   FDW fw;
   if (fd_weights(&fw, h3, order)) return;
   Dims d = {{Sx, Sy, Sz}, (long)Sx * Sy * Sz};
@@ -214,10 +224,10 @@ void orc_deriv_points(const double* F, int Sx, int Sy, int Sz, const double* h3,
 /* ---------------------------------------------------------------- algebra */
 
 typedef struct {
-  double eta, lam[4], lamf[2], cahd, dt, dxi, hssl, sigssl, t;
+  real eta, lam[4], lamf[2], cahd, dt, dxi, hssl, sigssl, t;
 } Scal;
 
-static void scal_from13(Scal* s, const double* sc) {
+static void scal_from13(Scal* s, const real* sc) {
   s->eta = sc[0];
   for (int i = 0; i < 4; ++i) s->lam[i] = sc[1 + i];
   s->lamf[0] = sc[5];
@@ -231,7 +241,7 @@ static void scal_from13(Scal* s, const double* sc) {
 }
 
 /* S16 (eta, C, dt, dx_i, h_ssl, sig_ssl, t, lambda0..3, lambda_f0..1, dx, dy, dz) -> 13 */
-static void s16_to_13(const double* S, double* sc) {
+static void s16_to_13(const real* S, real* sc) {
   sc[0] = S[0];
   for (int i = 0; i < 4; ++i) sc[1 + i] = S[7 + i];
   sc[5] = S[11];
@@ -273,14 +283,14 @@ static void s16_to_13(const double* S, double* sc) {
  * (the CAHD+SSL variant the reference evaluates: vendor/bssneqs_SSL_HD_dxsq.cpp output lines
  * a_rhs .. B_rhs2; advection is centred, not upwinded).
  */
-static void algebra_point(const double* v, const double* dv, const Scal* S, double* r,
-                          double* ham_out, double* mom_out) {
+static void algebra_point(const real* v, const real* dv, const Scal* S, real* r,
+                          real* ham_out, real* mom_out) {
 #define D1(f, a) dv[3 * (f) + (a)]
 #define D2(s, i, j) dv[72 + 6 * (s) + SYM[i][j]]
   /* second-derivative slots: 0 alpha, 1 chi, 2..7 gt0..5, 8..10 beta0..2 */
-  const double al = v[ALPHA], chi = v[CHI], K = v[KK];
-  double g[3][3], A[3][3], ig[3][3];
-  double b[3], Gt[3], B[3];
+  const real al = v[ALPHA], chi = v[CHI], K = v[KK];
+  real g[3][3], A[3][3], ig[3][3];
+  real b[3], Gt[3], B[3];
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) {
       g[i][j] = v[GT0 + SYM[i][j]];
@@ -291,7 +301,7 @@ static void algebra_point(const double* v, const double* dv, const Scal* S, doub
     Gt[i] = v[GTI0 + i];
     B[i] = v[BB0 + i];
   }
-  double db[3][3], dg[3][3][3], dchi[3], dal[3], dK[3], dGt[3][3];
+  real db[3][3], dg[3][3][3], dchi[3], dal[3], dK[3], dGt[3][3];
   for (int j = 0; j < 3; ++j) {
     for (int i = 0; i < 3; ++i) db[j][i] = D1(BETA0 + i, j);      /* d_j beta^i */
     for (int k = 0; k < 3; ++k) dGt[j][k] = D1(GTI0 + k, j);      /* d_j Gt^k */
@@ -302,10 +312,10 @@ static void algebra_point(const double* v, const double* dv, const Scal* S, doub
     dK[j] = D1(KK, j);
   }
   /* inverse conformal metric */
-  const double det = g[0][0] * (g[1][1] * g[2][2] - g[1][2] * g[1][2]) -
+  const real det = g[0][0] * (g[1][1] * g[2][2] - g[1][2] * g[1][2]) -
                      g[0][1] * (g[0][1] * g[2][2] - g[1][2] * g[0][2]) +
                      g[0][2] * (g[0][1] * g[1][2] - g[1][1] * g[0][2]);
-  double adj[3][3];
+  real adj[3][3];
   adj[0][0] = g[1][1] * g[2][2] - g[1][2] * g[1][2];
   adj[0][1] = g[0][2] * g[1][2] - g[0][1] * g[2][2];
   adj[0][2] = g[0][1] * g[1][2] - g[0][2] * g[1][1];
@@ -319,7 +329,7 @@ static void algebra_point(const double* v, const double* dv, const Scal* S, doub
     for (int j = 0; j < 3; ++j) ig[i][j] = adj[i][j] / det;
 
   /* Christoffels */
-  double C1[3][3][3], C2[3][3][3], C3[3][3][3], CalGt[3], igdchi[3];
+  real C1[3][3][3], C2[3][3][3], C3[3][3][3], CalGt[3], igdchi[3];
   for (int k = 0; k < 3; ++k)
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j)
@@ -327,12 +337,12 @@ static void algebra_point(const double* v, const double* dv, const Scal* S, doub
   for (int m = 0; m < 3; ++m)
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j) {
-        double s = 0.0;
+        real s = 0.0;
         for (int k = 0; k < 3; ++k) s += ig[m][k] * C1[k][i][j];
         C2[m][i][j] = s;
       }
   for (int m = 0; m < 3; ++m) {
-    double s = 0.0, t = 0.0;
+    real s = 0.0, t = 0.0;
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j) s += ig[i][j] * C2[m][i][j];
     CalGt[m] = s;
@@ -347,19 +357,19 @@ static void algebra_point(const double* v, const double* dv, const Scal* S, doub
                                          g[i][j] * igdchi[m]);
 
   /* conformal Ricci, metric part and chi part */
-  double R[3][3];
-  double lapchi = 0.0;
+  real R[3][3];
+  real lapchi = 0.0;
   for (int k = 0; k < 3; ++k)
     for (int l = 0; l < 3; ++l)
       lapchi += ig[k][l] * (D2(1, k, l) - 1.5 / chi * dchi[k] * dchi[l]);
   for (int m = 0; m < 3; ++m) lapchi -= CalGt[m] * dchi[m];
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) {
-      double s = 0.0;
+      real s = 0.0;
       for (int l = 0; l < 3; ++l)
         for (int m = 0; m < 3; ++m) s += ig[l][m] * D2(2 + SYM[i][j], l, m);
       s *= -0.5;
-      double t = 0.0;
+      real t = 0.0;
       for (int k = 0; k < 3; ++k) t += g[k][i] * dGt[j][k] + g[k][j] * dGt[i][k];
       s += 0.5 * t;
       t = 0.0;
@@ -372,25 +382,25 @@ static void algebra_point(const double* v, const double* dv, const Scal* S, doub
             t += ig[l][m] * (C2[k][l][i] * C1[j][k][m] + C2[k][l][j] * C1[i][k][m] +
                              C2[k][i][m] * C1[k][l][j]);
       s += t;
-      double c = 0.0;
+      real c = 0.0;
       for (int k = 0; k < 3; ++k) c += C2[k][j][i] * dchi[k];
-      const double rchi = 0.5 / chi * (D2(1, i, j) - c) - 0.25 / (chi * chi) * dchi[i] * dchi[j] +
+      const real rchi = 0.5 / chi * (D2(1, i, j) - c) - 0.25 / (chi * chi) * dchi[i] * dchi[j] +
                           0.5 / chi * g[i][j] * lapchi;
       R[i][j] = s + rchi;
     }
 
-  double DDa[3][3];
+  real DDa[3][3];
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) {
-      double s = 0.0;
+      real s = 0.0;
       for (int l = 0; l < 3; ++l) s += C3[l][i][j] * dal[l];
       DDa[i][j] = D2(0, i, j) - s;
     }
-  const double divb = db[0][0] + db[1][1] + db[2][2];
-  double Aup[3][3], AA = 0.0;
+  const real divb = db[0][0] + db[1][1] + db[2][2];
+  real Aup[3][3], AA = 0.0;
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) {
-      double s = 0.0;
+      real s = 0.0;
       for (int k = 0; k < 3; ++k)
         for (int l = 0; l < 3; ++l) s += ig[i][k] * ig[j][l] * A[k][l];
       Aup[i][j] = s;
@@ -400,58 +410,58 @@ static void algebra_point(const double* v, const double* dv, const Scal* S, doub
 
 #define ADV(f) (b[0] * D1(f, 0) + b[1] * D1(f, 1) + b[2] * D1(f, 2))
   /* lapse (with slice locking) */
-  const double sq = sqrt(chi);
+  const real sq = SQRT(chi);
   r[ALPHA] = S->lam[0] * ADV(ALPHA) - 2.0 * al * K -
-             sq * S->hssl * (al - sq) * exp(-0.5 * S->t * S->t / (S->sigssl * S->sigssl));
+             sq * S->hssl * (al - sq) * EXP(-0.5 * S->t * S->t / (S->sigssl * S->sigssl));
   /* shift */
   for (int i = 0; i < 3; ++i)
     r[BETA0 + i] = 0.75 * (S->lamf[0] + S->lamf[1] * al) * B[i] + S->lam[1] * ADV(BETA0 + i);
   /* conformal metric */
   for (int q = 0; q < 6; ++q) {
     const int i = PAIR_I[q], j = PAIR_J[q];
-    double s = 0.0;
+    real s = 0.0;
     for (int k = 0; k < 3; ++k) s += g[i][k] * db[j][k] + g[k][j] * db[i][k];
-    r[GT0 + q] = ADV(GT0 + q) + s - 2.0 / 3.0 * g[i][j] * divb - 2.0 * al * A[i][j];
+    r[GT0 + q] = ADV(GT0 + q) + s - ((real)2 / 3) * g[i][j] * divb - 2.0 * al * A[i][j];
   }
   /* Hamiltonian constraint and the damped conformal factor */
-  double Rs = 0.0;
+  real Rs = 0.0;
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) Rs += ig[i][j] * R[i][j];
-  const double ham = chi * Rs - AA + 2.0 / 3.0 * K * K;
-  const double dx2 = S->dxi * S->dxi;
-  r[CHI] = ADV(CHI) + 2.0 / 3.0 * chi * (al * K - divb) +
+  const real ham = chi * Rs - AA + ((real)2 / 3) * K * K;
+  const real dx2 = S->dxi * S->dxi;
+  r[CHI] = ADV(CHI) + ((real)2 / 3) * chi * (al * K - divb) +
            S->cahd * dx2 * chi * ham / (S->dt * (10.0 * dx2 + 1.0));
   /* trace-free curvature */
-  double X[3][3], trX = 0.0;
+  real X[3][3], trX = 0.0;
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) X[i][j] = al * R[i][j] - DDa[i][j];
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) trX += ig[i][j] * X[i][j];
   for (int q = 0; q < 6; ++q) {
     const int i = PAIR_I[q], j = PAIR_J[q];
-    double lie = 0.0, aa = 0.0;
+    real lie = 0.0, aa = 0.0;
     for (int k = 0; k < 3; ++k) lie += A[i][k] * db[j][k] + A[k][j] * db[i][k];
     for (int k = 0; k < 3; ++k) {
-      double t = 0.0;
+      real t = 0.0;
       for (int l = 0; l < 3; ++l) t += ig[k][l] * A[l][j];
       aa += A[i][k] * t;
     }
-    r[AT0 + q] = ADV(AT0 + q) + lie - 2.0 / 3.0 * A[i][j] * divb +
+    r[AT0 + q] = ADV(AT0 + q) + lie - ((real)2 / 3) * A[i][j] * divb +
                  chi * (X[i][j] - g[i][j] * trX / 3.0) + al * (K * A[i][j] - 2.0 * aa);
   }
   /* trace of the extrinsic curvature */
-  double lap = 0.0;
+  real lap = 0.0;
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) lap += ig[i][j] * DDa[i][j];
   r[KK] = ADV(KK) - chi * lap + al * (K * K / 3.0 + AA);
   /* conformal connection functions and the Gamma-driver auxiliary */
   for (int i = 0; i < 3; ++i) {
-    double s = 0.0;
+    real s = 0.0;
     for (int j = 0; j < 3; ++j) s += b[j] * dGt[j][i];
-    const double advGt = s;
+    const real advGt = s;
     for (int j = 0; j < 3; ++j) s -= CalGt[j] * db[j][i];
-    s += 2.0 / 3.0 * CalGt[i] * divb;
-    double t = 0.0;
+    s += ((real)2 / 3) * CalGt[i] * divb;
+    real t = 0.0;
     for (int j = 0; j < 3; ++j)
       for (int k = 0; k < 3; ++k)
         t += ig[j][k] * D2(8 + i, j, k) + ig[i][j] * D2(8 + k, j, k) / 3.0;
@@ -465,7 +475,7 @@ static void algebra_point(const double* v, const double* dv, const Scal* S, doub
     s += t;
     t = 0.0;
     for (int j = 0; j < 3; ++j)
-      t += al * (3.0 / chi * Aup[i][j] * dchi[j] + 4.0 / 3.0 * ig[i][j] * dK[j]);
+      t += al * (3.0 / chi * Aup[i][j] * dchi[j] + ((real)4 / 3) * ig[i][j] * dK[j]);
     s -= t;
     r[GTI0 + i] = s;
     r[BB0 + i] = s - S->eta * B[i] + S->lam[2] * ADV(BB0 + i) - S->lam[3] * advGt;
@@ -475,21 +485,21 @@ static void algebra_point(const double* v, const double* dv, const Scal* S, doub
     /* momentum constraint M^i = gt^jk (d_j At_ki - Gamma^m_ji At_km) - Gt^j At_ij
      *                           - 3/(2chi) gt^jk At_ki d_j chi - 2/3 d_i K */
     for (int i = 0; i < 3; ++i) {
-      double s = 0.0;
+      real s = 0.0;
       for (int j = 0; j < 3; ++j)
         for (int k = 0; k < 3; ++k) {
-          double c = 0.0;
+          real c = 0.0;
           for (int m = 0; m < 3; ++m) c += C2[m][j][i] * A[k][m];
           s += ig[j][k] * (D1(AT0 + SYM[k][i], j) - c);
         }
-      double t = 0.0;
+      real t = 0.0;
       for (int j = 0; j < 3; ++j) t += Gt[j] * A[i][j];
       s -= t;
       t = 0.0;
       for (int j = 0; j < 3; ++j)
         for (int k = 0; k < 3; ++k) t += ig[j][k] * A[k][i] * dchi[j] / chi;
       s -= 1.5 * t;
-      s -= 2.0 / 3.0 * dK[i];
+      s -= ((real)2 / 3) * dK[i];
       mom_out[i] = s;
     }
   }
@@ -498,12 +508,12 @@ static void algebra_point(const double* v, const double* dv, const Scal* S, doub
 #undef D2
 }
 
-void orc_algebra(long n, const double* F, const double* D, const double* sc13, double* OUT) {
+void orc_algebra(long n, const real* F, const real* D, const real* sc13, real* OUT) {
   Scal S;
   scal_from13(&S, sc13);
 #pragma omp parallel for schedule(static)
   for (long p = 0; p < n; ++p) {
-    double v[NF], dv[ND], r[NF];
+    real v[NF], dv[ND], r[NF];
     for (int f = 0; f < NF; ++f) v[f] = F[f * n + p];
     for (int d = 0; d < ND; ++d) dv[d] = D[d * n + p];
     algebra_point(v, dv, &S, r, NULL, NULL);
@@ -512,14 +522,14 @@ void orc_algebra(long n, const double* F, const double* D, const double* sc13, d
 }
 
 /* Kreiss-Oliger sum over the three axes at one point for one field: ((0 + kx) + ky) + kz */
-static double ko_at(const double* u, const Dims* d, const int p[3], double wko[3][9], int nt) {
-  double acc = 0.0;
+static real ko_at(const real* u, const Dims* d, const int p[3], real wko[3][9], int nt) {
+  real acc = 0.0;
   for (int a = 0; a < 3; ++a) acc = acc + stencil_at(u, d, p, a, wko[a], nt);
   return acc;
 }
 
-static int ko_weights(double wko[3][9], const double* h3, int ko_order, double sigma) {
-  double c[9];
+static int ko_weights(real wko[3][9], const real* h3, int ko_order, real sigma) {
+  real c[9];
   const int nt = orc_stencil(2, ko_order, c);
   if (nt < 0) return -1;
   for (int a = 0; a < 3; ++a)
@@ -527,16 +537,16 @@ static int ko_weights(double wko[3][9], const double* h3, int ko_order, double s
   return nt;
 }
 
-int orc_rhs_points(const double* F, int Sx, int Sy, int Sz, const double* S16, int order,
-                   int ko_order, double ko_sigma, long npts, const long* idx, double* OUT) {
+int orc_rhs_points(const real* F, int Sx, int Sy, int Sz, const real* S16, int order,
+                   int ko_order, real ko_sigma, long npts, const long* idx, real* OUT) {
   FDW fw;
-  const double h3[3] = {S16[13], S16[14], S16[15]};
+  const real h3[3] = {S16[13], S16[14], S16[15]};
   if (ko_order == 0) ko_order = order + 2;
   if (fd_weights(&fw, h3, order)) return -1;
-  double wko[3][9];
+  real wko[3][9];
   const int nko = ko_weights(wko, h3, ko_order, ko_sigma);
   if (nko < 0) return -1;
-  double sc[13];
+  real sc[13];
   s16_to_13(S16, sc);
   Scal S;
   scal_from13(&S, sc);
@@ -545,7 +555,7 @@ int orc_rhs_points(const double* F, int Sx, int Sy, int Sz, const double* S16, i
   for (long n = 0; n < npts; ++n) {
     int p[3];
     unlin(&d, idx[n], p);
-    double v[NF], dv[ND], r[NF];
+    real v[NF], dv[ND], r[NF];
     for (int f = 0; f < NF; ++f) v[f] = F[f * d.P + idx[n]];
     point_derivs(F, &d, &fw, p, dv, 1);
     algebra_point(v, dv, &S, r, NULL, NULL);
@@ -556,16 +566,16 @@ int orc_rhs_points(const double* F, int Sx, int Sy, int Sz, const double* S16, i
   return 0;
 }
 
-int orc_rhs(const double* F, double* OUT, int Sx, int Sy, int Sz, const double* S16, int order,
-            int ko_order, double ko_sigma) {
+int orc_rhs(const real* F, real* OUT, int Sx, int Sy, int Sz, const real* S16, int order,
+            int ko_order, real ko_sigma) {
   FDW fw;
-  const double h3[3] = {S16[13], S16[14], S16[15]};
+  const real h3[3] = {S16[13], S16[14], S16[15]};
   if (ko_order == 0) ko_order = order + 2;
   if (fd_weights(&fw, h3, order)) return -1;
-  double wko[3][9];
+  real wko[3][9];
   const int nko = ko_weights(wko, h3, ko_order, ko_sigma);
   if (nko < 0) return -1;
-  double sc[13];
+  real sc[13];
   s16_to_13(S16, sc);
   Scal S;
   scal_from13(&S, sc);
@@ -576,7 +586,7 @@ int orc_rhs(const double* F, double* OUT, int Sx, int Sy, int Sz, const double* 
       for (int z = 0; z < Sz; ++z) {
         const int p[3] = {x, y, z};
         const long q = lin(&d, x, y, z);
-        double v[NF], dv[ND], r[NF];
+        real v[NF], dv[ND], r[NF];
         for (int f = 0; f < NF; ++f) v[f] = F[f * d.P + q];
         point_derivs(F, &d, &fw, p, dv, 1);
         algebra_point(v, dv, &S, r, NULL, NULL);
@@ -589,14 +599,14 @@ int orc_rhs(const double* F, double* OUT, int Sx, int Sy, int Sz, const double* 
 
 /* ---------------------------------------------------------------- constraints */
 
-void orc_constraints_points(long n, const double* F, const double* D, double* HM) {
+void orc_constraints_points(long n, const real* F, const real* D, real* HM) {
   Scal S;
   memset(&S, 0, sizeof S);
   S.dt = 1.0;
   S.sigssl = 1.0;
 #pragma omp parallel for schedule(static)
   for (long p = 0; p < n; ++p) {
-    double v[NF], dv[ND], r[NF], ham, mom[3];
+    real v[NF], dv[ND], r[NF], ham, mom[3];
     for (int f = 0; f < NF; ++f) v[f] = F[f * n + p];
     for (int d = 0; d < ND; ++d) dv[d] = D[d * n + p];
     algebra_point(v, dv, &S, r, &ham, mom);
@@ -605,7 +615,7 @@ void orc_constraints_points(long n, const double* F, const double* D, double* HM
   }
 }
 
-void orc_constraints(const double* F, double* HM, int Sx, int Sy, int Sz, const double* h3,
+void orc_constraints(const real* F, real* HM, int Sx, int Sy, int Sz, const real* h3,
                      int order) {
   FDW fw;
   if (fd_weights(&fw, h3, order)) return;
@@ -620,7 +630,7 @@ void orc_constraints(const double* F, double* HM, int Sx, int Sy, int Sz, const 
       for (int z = 0; z < Sz; ++z) {
         const int p[3] = {x, y, z};
         const long q = lin(&d, x, y, z);
-        double v[NF], dv[ND], r[NF], ham, mom[3];
+        real v[NF], dv[ND], r[NF], ham, mom[3];
         for (int f = 0; f < NF; ++f) v[f] = F[f * d.P + q];
         point_derivs(F, &d, &fw, p, dv, 1);
         algebra_point(v, dv, &S, r, &ham, mom);
@@ -629,16 +639,16 @@ void orc_constraints(const double* F, double* HM, int Sx, int Sy, int Sz, const 
       }
 }
 
-void orc_constraint_l2(const double* F, int Sx, int Sy, int Sz, const double* h3, int order,
-                       int ng, double* out2) {
+void orc_constraint_l2(const real* F, int Sx, int Sy, int Sz, const real* h3, int order,
+                       int ng, real* out2) {
   const long P = (long)Sx * Sy * Sz;
-  double* HM = (double*)malloc(sizeof(double) * 4 * P);
+  real* HM = (real*)malloc(sizeof(real) * 4 * P);
   if (!HM) {
     out2[0] = out2[1] = NAN;
     return;
   }
   orc_constraints(F, HM, Sx, Sy, Sz, h3, order);
-  double sh = 0.0, sm = 0.0;
+  real sh = 0.0, sm = 0.0;
   long cnt = 0;
   for (int x = ng; x < Sx - ng; ++x)
     for (int y = ng; y < Sy - ng; ++y)
@@ -648,14 +658,14 @@ void orc_constraint_l2(const double* F, int Sx, int Sy, int Sz, const double* h3
         sm += HM[P + q] * HM[P + q] + HM[2 * P + q] * HM[2 * P + q] + HM[3 * P + q] * HM[3 * P + q];
         ++cnt;
       }
-  out2[0] = sqrt(sh / (double)cnt);
-  out2[1] = sqrt(sm / (double)cnt);
+  out2[0] = SQRT(sh / (real)cnt);
+  out2[1] = SQRT(sm / (real)cnt);
   free(HM);
 }
 
 /* ---------------------------------------------------------------- step */
 
-void orc_periodic_sync(double* F, int Sx, int Sy, int Sz, int ng) {
+void orc_periodic_sync(real* F, int Sx, int Sy, int Sz, int ng) {
   const long P = (long)Sx * Sy * Sz;
   const int S[3] = {Sx, Sy, Sz};
   /* x, then y, then z; each copies ng planes from the opposite interior
@@ -663,7 +673,7 @@ void orc_periodic_sync(double* F, int Sx, int Sy, int Sz, int ng) {
   for (int axis = 0; axis < 3; ++axis) {
 #pragma omp parallel for schedule(static)
     for (int f = 0; f < NF; ++f) {
-      double* u = F + f * P;
+      real* u = F + f * P;
       for (int x = 0; x < Sx; ++x)
         for (int y = 0; y < Sy; ++y)
           for (int z = 0; z < Sz; ++z) {
@@ -683,25 +693,25 @@ void orc_periodic_sync(double* F, int Sx, int Sy, int Sz, int ng) {
   }
 }
 
-void orc_enforce(double* F, long P, double chi_floor, double alpha_floor) {
+void orc_enforce(real* F, long P, real chi_floor, real alpha_floor) {
 #pragma omp parallel for schedule(static)
   for (long q = 0; q < P; ++q) {
-    double g[6], a[6];
+    real g[6], a[6];
     for (int i = 0; i < 6; ++i) g[i] = F[(GT0 + i) * P + q];
-    const double det = g[0] * (g[3] * g[5] - g[4] * g[4]) - g[1] * (g[1] * g[5] - g[4] * g[2]) +
+    const real det = g[0] * (g[3] * g[5] - g[4] * g[4]) - g[1] * (g[1] * g[5] - g[4] * g[2]) +
                        g[2] * (g[1] * g[4] - g[3] * g[2]);
-    const double scale = pow(det, -1.0 / 3.0);
+    const real scale = POW(det, -((real)1 / 3));
     for (int i = 0; i < 6; ++i) g[i] = g[i] * scale;
-    const double iuxx = g[3] * g[5] - g[4] * g[4];
-    const double iuxy = g[2] * g[4] - g[1] * g[5];
-    const double iuxz = g[1] * g[4] - g[2] * g[3];
-    const double iuyy = g[0] * g[5] - g[2] * g[2];
-    const double iuyz = g[1] * g[2] - g[0] * g[4];
-    const double iuzz = g[0] * g[3] - g[1] * g[1];
+    const real iuxx = g[3] * g[5] - g[4] * g[4];
+    const real iuxy = g[2] * g[4] - g[1] * g[5];
+    const real iuxz = g[1] * g[4] - g[2] * g[3];
+    const real iuyy = g[0] * g[5] - g[2] * g[2];
+    const real iuyz = g[1] * g[2] - g[0] * g[4];
+    const real iuzz = g[0] * g[3] - g[1] * g[1];
     for (int i = 0; i < 6; ++i) a[i] = F[(AT0 + i) * P + q];
-    const double trA = iuxx * a[0] + iuyy * a[3] + iuzz * a[5] +
+    const real trA = iuxx * a[0] + iuyy * a[3] + iuzz * a[5] +
                        2.0 * (iuxy * a[1] + iuxz * a[2] + iuyz * a[4]);
-    const double third = trA / 3.0;
+    const real third = trA / 3.0;
     for (int i = 0; i < 6; ++i) {
       F[(GT0 + i) * P + q] = g[i];
       F[(AT0 + i) * P + q] = a[i] - g[i] * third;
@@ -711,32 +721,32 @@ void orc_enforce(double* F, long P, double chi_floor, double alpha_floor) {
   }
 }
 
-int orc_stage_rhs(const double* Y, double* Kout, double* scratch, int Sx, int Sy, int Sz,
-                  const double* S16, int order, int ko_order, double ko_sigma, int ng,
-                  int periodic, double t) {
+int orc_stage_rhs(const real* Y, real* Kout, real* scratch, int Sx, int Sy, int Sz,
+                  const real* S16, int order, int ko_order, real ko_sigma, int ng,
+                  int periodic, real t) {
   const long P = (long)Sx * Sy * Sz;
-  double S[16];
+  real S[16];
   memcpy(S, S16, sizeof S);
   S[6] = t;
-  const double* src = Y;
+  const real* src = Y;
   if (periodic) {
-    memcpy(scratch, Y, sizeof(double) * NF * P);
+    memcpy(scratch, Y, sizeof(real) * NF * P);
     orc_periodic_sync(scratch, Sx, Sy, Sz, ng);
     src = scratch;
   }
   return orc_rhs(src, Kout, Sx, Sy, Sz, S, order, ko_order, ko_sigma);
 }
 
-int orc_rk4_step(double* U, int Sx, int Sy, int Sz, const double* S16, int order, int ko_order,
-                 double ko_sigma, int ng, int periodic, double t, double dt, double chi_floor,
-                 double alpha_floor, double* work) {
+int orc_rk4_step(real* U, int Sx, int Sy, int Sz, const real* S16, int order, int ko_order,
+                 real ko_sigma, int ng, int periodic, real t, real dt, real chi_floor,
+                 real alpha_floor, real* work) {
   const long N = (long)NF * Sx * Sy * Sz;
-  double *k1 = work, *k2 = work + N, *k3 = work + 2 * N, *k4 = work + 3 * N;
-  double *Y = work + 4 * N, *scr = work + 5 * N;
-  double S[16];
+  real *k1 = work, *k2 = work + N, *k3 = work + 2 * N, *k4 = work + 3 * N;
+  real *Y = work + 4 * N, *scr = work + 5 * N;
+  real S[16];
   memcpy(S, S16, sizeof S);
   S[2] = dt; /* the CAHD factor uses the full step at every stage (evolve.py:121-124) */
-  const double hdt = 0.5 * dt;
+  const real hdt = 0.5 * dt;
   int rc = orc_stage_rhs(U, k1, scr, Sx, Sy, Sz, S, order, ko_order, ko_sigma, ng, periodic, t);
   if (rc) return rc;
 #pragma omp parallel for schedule(static)
@@ -748,7 +758,7 @@ int orc_rk4_step(double* U, int Sx, int Sy, int Sz, const double* S16, int order
 #pragma omp parallel for schedule(static)
   for (long i = 0; i < N; ++i) Y[i] = U[i] + dt * k3[i];
   orc_stage_rhs(Y, k4, scr, Sx, Sy, Sz, S, order, ko_order, ko_sigma, ng, periodic, t + dt);
-  const double w = dt / 6.0;
+  const real w = dt / 6.0;
 #pragma omp parallel for schedule(static)
   for (long i = 0; i < N; ++i) U[i] = U[i] + w * (k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i]);
   orc_enforce(U, (long)Sx * Sy * Sz, chi_floor, alpha_floor);
@@ -757,23 +767,23 @@ int orc_rk4_step(double* U, int Sx, int Sy, int Sz, const double* S16, int order
 
 /* rk4_2_2 coefficients: Sanches, Brandt, Kalinani, Ji & Schnetter (arXiv:2603.05763), Table 1,
  * solution (2), as the reference enters them (2D/src/mcs2d/msrk.py:47-52). */
-static const double MS_A20 = 1309.0 / 15500.0, MS_A21 = -31999.0 / 15500.0;
-static const double MS_A30 = -241289.0 / 5880000.0, MS_A31 = 22846301.0 / 16170000.0,
+static const real MS_A20 = 1309.0 / 15500.0, MS_A21 = -31999.0 / 15500.0;
+static const real MS_A30 = -241289.0 / 5880000.0, MS_A31 = 22846301.0 / 16170000.0,
                     MS_A32 = -936169.0 / 2587200.0;
-static const double MS_B0 = -191.0 / 882.0, MS_B1 = 48241.0 / 59994.0,
+static const real MS_B0 = -191.0 / 882.0, MS_B1 = 48241.0 / 59994.0,
                     MS_B2 = 193750.0 / 4351347.0, MS_B3 = 100000.0 / 271791.0;
 
-int orc_msrk_step(double* U, double* fprev, int Sx, int Sy, int Sz, const double* S16,
-                  int order, int ko_order, double ko_sigma, int ng, int periodic, double t,
-                  double dt, double chi_floor, double alpha_floor, double* work) {
+int orc_msrk_step(real* U, real* fprev, int Sx, int Sy, int Sz, const real* S16,
+                  int order, int ko_order, real ko_sigma, int ng, int periodic, real t,
+                  real dt, real chi_floor, real alpha_floor, real* work) {
   const long N = (long)NF * Sx * Sy * Sz;
-  double *k1 = work, *k2 = work + N, *k3 = work + 2 * N, *Y = work + 3 * N, *scr = work + 4 * N;
-  double S[16];
+  real *k1 = work, *k2 = work + N, *k3 = work + 2 * N, *Y = work + 3 * N, *scr = work + 4 * N;
+  real S[16];
   memcpy(S, S16, sizeof S);
   S[2] = dt;
-  const double c2 = MS_A20 + MS_A21;
-  const double c3 = MS_A30 + MS_A31 + MS_A32;
-  const double* k0 = fprev;
+  const real c2 = MS_A20 + MS_A21;
+  const real c3 = MS_A30 + MS_A31 + MS_A32;
+  const real* k0 = fprev;
   int rc = orc_stage_rhs(U, k1, scr, Sx, Sy, Sz, S, order, ko_order, ko_sigma, ng, periodic, t);
   if (rc) return rc;
 #pragma omp parallel for schedule(static)

@@ -18,6 +18,13 @@
 #ifndef BSSN_ORACLE_H
 #define BSSN_ORACLE_H
 
+/* Working precision: double by default; the long-double build (liboracle_ld.so, the
+ * accuracy reference of the tests) compiles the same source with ORC_REAL=long double. */
+#ifndef ORC_REAL
+#define ORC_REAL double
+#endif
+typedef ORC_REAL real;
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -27,65 +34,65 @@ int orc_version(void);
 /* Stencil coefficients (3D common/src/mcs_common/derivatives.py:33-49). kind 0 = first
  * derivative, 1 = second derivative, 2 = Kreiss-Oliger. Writes order+1 values, returns the
  * count, or -1 for an unsupported order. */
-int orc_stencil(int kind, int order, double* coeffs);
+int orc_stencil(int kind, int order, real* coeffs);
 
 /* One-axis operator over a whole (Sx,Sy,Sz) array with edge padding by the stencil's own
  * reach (derivatives.py:78-88). factor = 1/h (d1), 1/h^2 (d2) or sigma/h (KO). */
-void orc_apply_axis(const double* u, double* out, int Sx, int Sy, int Sz, int axis,
-                    int kind, int order, double factor);
+void orc_apply_axis(const real* u, real* out, int Sx, int Sy, int Sz, int axis,
+                    int kind, int order, real factor);
 
 /* The 138-derivative bundle at every padded point (derivative_bundle.py:31-48). */
-void orc_deriv_bundle(const double* F, double* D, int Sx, int Sy, int Sz,
-                      const double* h3, int order);
+void orc_deriv_bundle(const real* F, real* D, int Sx, int Sy, int Sz,
+                      const real* h3, int order);
 
 /* The 138 derivatives at a list of points only (same arithmetic as the bundle). */
-void orc_deriv_points(const double* F, int Sx, int Sy, int Sz, const double* h3, int order,
-                      long npts, const long* idx, double* Dp /* 138*npts, d-major */);
+void orc_deriv_points(const real* F, int Sx, int Sy, int Sz, const real* h3, int order,
+                      long npts, const long* idx, real* Dp /* 138*npts, d-major */);
 
 /* Pointwise BSSN algebra over n points. sc13 = eta, lambda0..3, lambda_f0..1, C_CAHD, dt,
  * dx_i, h_ssl, sig_ssl, t. */
-void orc_algebra(long n, const double* F, const double* D, const double* sc13, double* OUT);
+void orc_algebra(long n, const real* F, const real* D, const real* sc13, real* OUT);
 
 /* Full padded RHS: bundle + algebra (rhs.py:102-139), optional KO added when ko_sigma > 0
  * (evolve.py:76-90). ko_order 0 selects the default pairing (order + 2). */
-int orc_rhs(const double* F, double* OUT, int Sx, int Sy, int Sz, const double* S16,
-            int order, int ko_order, double ko_sigma);
+int orc_rhs(const real* F, real* OUT, int Sx, int Sy, int Sz, const real* S16,
+            int order, int ko_order, real ko_sigma);
 
 /* RHS (+KO) at listed points only. OUT is 24*npts, field-major. */
-int orc_rhs_points(const double* F, int Sx, int Sy, int Sz, const double* S16, int order,
-                   int ko_order, double ko_sigma, long npts, const long* idx, double* OUT);
+int orc_rhs_points(const real* F, int Sx, int Sy, int Sz, const real* S16, int order,
+                   int ko_order, real ko_sigma, long npts, const long* idx, real* OUT);
 
 /* Hamiltonian + momentum constraints at every padded point (constraints.py:38-51):
  * HM = 4 arrays (H, M0, M1, M2). */
-void orc_constraints(const double* F, double* HM, int Sx, int Sy, int Sz, const double* h3,
+void orc_constraints(const real* F, real* HM, int Sx, int Sy, int Sz, const real* h3,
                      int order);
 
 /* Constraints from given fields and derivatives at n points: HM is 4*n. */
-void orc_constraints_points(long n, const double* F, const double* D, double* HM);
+void orc_constraints_points(long n, const real* F, const real* D, real* HM);
 
 /* Interior RMS norms (constraints.py:53-62): out2 = {H_l2, M_l2}. */
-void orc_constraint_l2(const double* F, int Sx, int Sy, int Sz, const double* h3, int order,
-                       int ng, double* out2);
+void orc_constraint_l2(const real* F, int Sx, int Sy, int Sz, const real* h3, int order,
+                       int ng, real* out2);
 
-void orc_periodic_sync(double* F, int Sx, int Sy, int Sz, int ng);
-void orc_enforce(double* F, long P, double chi_floor, double alpha_floor);
+void orc_periodic_sync(real* F, int Sx, int Sy, int Sz, int ng);
+void orc_enforce(real* F, long P, real chi_floor, real alpha_floor);
 
 /* One RK4 step in place (evolve.py:125-132), with periodic ghost sync before every RHS
  * evaluation when periodic != 0. work must hold 6 states. */
-int orc_rk4_step(double* U, int Sx, int Sy, int Sz, const double* S16, int order, int ko_order,
-                 double ko_sigma, int ng, int periodic, double t, double dt,
-                 double chi_floor, double alpha_floor, double* work);
+int orc_rk4_step(real* U, int Sx, int Sy, int Sz, const real* S16, int order, int ko_order,
+                 real ko_sigma, int ng, int periodic, real t, real dt,
+                 real chi_floor, real alpha_floor, real* work);
 
 /* One MSRK rk4_2_2 step (evolve.py:139-153). fprev holds f(y_{n-1}) on entry and f(y_n) on
  * exit. work must hold 6 states. */
-int orc_msrk_step(double* U, double* fprev, int Sx, int Sy, int Sz, const double* S16,
-                  int order, int ko_order, double ko_sigma, int ng, int periodic, double t,
-                  double dt, double chi_floor, double alpha_floor, double* work);
+int orc_msrk_step(real* U, real* fprev, int Sx, int Sy, int Sz, const real* S16,
+                  int order, int ko_order, real ko_sigma, int ng, int periodic, real t,
+                  real dt, real chi_floor, real alpha_floor, real* work);
 
 /* Full RHS (+KO, +sync) used as the stage function f(y, t) by the steppers. */
-int orc_stage_rhs(const double* Y, double* K, double* scratch, int Sx, int Sy, int Sz,
-                  const double* S16, int order, int ko_order, double ko_sigma, int ng,
-                  int periodic, double t);
+int orc_stage_rhs(const real* Y, real* K, real* scratch, int Sx, int Sy, int Sz,
+                  const real* S16, int order, int ko_order, real ko_sigma, int ng,
+                  int periodic, real t);
 
 #ifdef __cplusplus
 }

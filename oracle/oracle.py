@@ -215,6 +215,40 @@ def rhs_points(F: np.ndarray, S16, idx, order: int = 4, ko_order: int = 0,
     return out
 
 
+LD_PATH = HERE / "liboracle_ld.so"
+_lib_ld = None
+
+
+def lib_ld():
+    """The long-double build of the same restatement (x87, 64-bit significand)."""
+    global _lib_ld
+    if _lib_ld is None:
+        if not LD_PATH.exists():
+            build()
+        _lib_ld = ctypes.CDLL(str(LD_PATH))
+    return _lib_ld
+
+
+def rhs_points_ld(F: np.ndarray, S16, idx, order: int = 4, ko_sigma: float = 0.0) -> np.ndarray:
+    """rhs_points evaluated in long double from the same (double) inputs: the accuracy
+    reference for FP64 evaluations of the RHS. Returns float64 (24, len(idx)) rounded from the
+    long-double result and the rounding residual, as (hi, lo)."""
+    _PLD = ctypes.POINTER(ctypes.c_longdouble)
+    Fl = np.ascontiguousarray(np.asarray(F, dtype=np.float64).astype(np.longdouble))
+    Sl = np.ascontiguousarray(np.asarray(S16, dtype=np.float64).astype(np.longdouble))
+    _, Sx, Sy, Sz = Fl.shape
+    idx = np.ascontiguousarray(idx, dtype=np.int64)
+    out = np.zeros((24, idx.size), dtype=np.longdouble)
+    rc = lib_ld().orc_rhs_points(Fl.ctypes.data_as(_PLD), Sx, Sy, Sz, Sl.ctypes.data_as(_PLD),
+                                 order, 0, ctypes.c_longdouble(ko_sigma),
+                                 ctypes.c_long(idx.size), idx.ctypes.data_as(_PL),
+                                 out.ctypes.data_as(_PLD))
+    if rc:
+        raise ValueError("orc_rhs_points (long double) failed")
+    hi = out.astype(np.float64)
+    return hi, (out - hi).astype(np.float64)
+
+
 def constraints(F: np.ndarray, h3, order: int = 4) -> np.ndarray:
     F = _c(F)
     _, Sx, Sy, Sz = F.shape

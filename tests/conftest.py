@@ -25,12 +25,17 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "slow: longer-running checks")
 
 
-def normwise(a: np.ndarray, b: np.ndarray) -> np.ndarray:
-    """Per-field normwise relative difference max|a-b| / max|b| over axis 0 fields."""
+def normwise(a: np.ndarray, b: np.ndarray, den=None) -> np.ndarray:
+    """Per-field normwise relative difference max|a-b| / max(max|b|, 1e-12) over the fields of
+    axis 0. The 1e-12 floor is the reference oracle's (3D/src/bssn3d/oracle.py:187): a field
+    whose exact right-hand side vanishes (e.g. At_xy of a 1D gauge wave) carries only round-off
+    and must not be divided by it. ``den`` overrides max|b| (e.g. a full-field maximum when a
+    and b are samples)."""
     a = np.asarray(a).reshape(a.shape[0], -1)
     b = np.asarray(b).reshape(b.shape[0], -1)
-    den = np.max(np.abs(b), axis=1)
-    den = np.where(den == 0.0, 1.0, den)
+    if den is None:
+        den = np.max(np.abs(b), axis=1)
+    den = np.maximum(np.asarray(den, dtype=np.float64), 1e-12)
     return np.max(np.abs(a - b), axis=1) / den
 
 

@@ -113,7 +113,23 @@ def test_twopass_matches_oracle():
     out = torch.empty_like(Fd)
     rhs_twopass_into(Fd, out, D, S16, 4, 0.1)
     Dref = O.deriv_bundle(F, h3, 4)
-    assert normwise(to_host(D), Dref).max() < TOL
+    # A derivative of a nearly constant field (gt_ii = 1 + 1e-4 waves) is a sum of O(1/h)
+    # terms cancelling to O(1e-4): its rounding error scales with sum_k |w_k| max|u|, not with
+    # the result. Bound each array by 64 eps times that stencil magnitude (or 1e-12 normwise).
+    Dg = to_host(D)
+    c1 = np.abs(O.stencil(0, 4)).sum()
+    c2 = np.abs(O.stencil(1, 4)).sum()
+    for d in range(138):
+        if d < 72:
+            f, a = divmod(d, 3)
+            mag = c1 / h3[a]
+        else:
+            s, q = divmod(d - 72, 6)
+            f = O.IDX[O.S2_FIELDS[s]]
+            i, j = O.PAIRS[q]
+            mag = c2 / h3[i] ** 2 if i == j else c1 * c1 / (h3[i] * h3[j])
+        bound = max(TOL * np.abs(Dref[d]).max(), 64 * 2.2e-16 * mag * np.abs(F[f]).max())
+        assert np.abs(Dg[d] - Dref[d]).max() <= bound, d
     err = normwise(to_host(out), O.rhs(F, S16, 4, ko_sigma=0.1))
     assert err.max() < TOL, err
 
@@ -160,10 +176,9 @@ def test_step_edge_padded():
 
 
 def test_order6_step():
-    say("rk4 step, order 6 with KO 8, ng 4")
-    N = 12
-    grid = Grid.from_domain(N, order=6, ko_order=8)
-    F = idata.gauge_wave(grid, amplitude=0.02).numpy()
+    say("rk4 step, order 6 with KO 8, ng 4, 16^3 single puncture")
+    grid, st, _ = idata.config_single_puncture(seed=3, N=16, ng=4)
+    F = st.numpy()
     dt = 0.25 * grid.dx
     p = PhysicsParams()
     evo = BSSNEvolution(grid, p, order=6, ko_sigma=0.1, bc="periodic")
@@ -214,10 +229,11 @@ def test_solver_interface():
 
 
 def test_full_size_binary_sampled():
-    """The 256^3 binary at full size: sampled points against the oracle (edges, faces and
-    corners included), Minkowski staticity and bitwise determinism."""
+    """The 256^3 binary at full size: sampled points against the oracle (random points, the
+    7^3 neighbourhoods of both punctures where the fields vary most, edges, faces and corners),
+    normalised by each field's full-grid maximum; Minkowski staticity and bitwise determinism."""
     say("256^3 binary workload: sampled parity + properties")
-    grid, st, _ = idata.config_binary(seed=0, N=256, device=DEV)
+    grid, st, pos = idata.config_binary(seed=0, N=256, device=DEV)
     S16 = scalars(PhysicsParams(), grid.dx, grid.dy, grid.dz, 0.25 * grid.dx, 0.0)
     out = torch.empty_like(st)
     rhs_into(st, out, S16, 4)
@@ -225,18 +241,34 @@ def test_full_size_binary_sampled():
     rhs_into(st, out2, S16, 4)
     torch.cuda.synchronize()
     assert torch.equal(out, out2)
+    del out2
+    fullmax = out.abs().reshape(24, -1).max(dim=1).values.cpu().numpy()
     Sx, Sy, Sz = grid.shape
     rng = np.random.default_rng(1)
-    pts = [rng.integers(0, n, size=3000) for n in (Sx, Sy, Sz)]
-    corners = np.array([[x, y, z] for x in (0, 1, Sx - 2, Sx - 1) for y in (0, 2, Sy - 1)
-                        for z in (0, 3, Sz - 1)]).T
-    P = np.concatenate([np.stack(pts), corners], axis=1)
+    pts = [np.stack([rng.integers(0, n, size=3000) for n in (Sx, Sy, Sz)])]
+    pts.append(np.array([[x, y, z] for x in (0, 1, Sx - 2, Sx - 1) for y in (0, 2, Sy - 1)
+                         for z in (0, 3, Sz - 1)]).T)
+    for p in pos:
+        c = [int(round((p[a] - lo) / grid.dx)) + grid.ng
+             for a, lo in enumerate((grid.xmin, grid.ymin, grid.zmin))]
+        r = np.arange(-3, 4)
+        X, Y, Z = np.meshgrid(c[0] + r, c[1] + r, c[2] + r, indexing="ij")
+        pts.append(np.stack([X.ravel(), Y.ravel(), Z.ravel()]))
+    P = np.concatenate(pts, axis=1)
     idx = (P[0] * Sy + P[1]) * Sz + P[2]
     F = to_host(st)
     ref = O.rhs_points(F, S16, idx, 4)
+    truth, _ = O.rhs_points_ld(F, S16, idx, 4)
     got = to_host(out).reshape(24, -1)[:, idx]
-    err = normwise(got, ref)
-    assert err.max() < TOL, err
+    # Near the punctures (chi ~ 1e-4) the RHS is a difference of terms ~1e8 times larger than
+    # its value, so FP64 itself is good to only ~1e-11 there: the reference algorithm in FP64
+    # (the oracle) misses the long-double evaluation by e_ref. The HIP kernel must be as
+    # accurate as that: e_hip <= max(1e-12, 2 e_ref), normwise by each field's full-grid max.
+    e_ref = normwise(ref, truth, den=fullmax)
+    e_hip = normwise(got, truth, den=fullmax)
+    print("e_ref", np.array2string(e_ref, precision=2), "\ne_hip",
+          np.array2string(e_hip, precision=2), flush=True)
+    assert np.all(e_hip <= np.maximum(TOL, 2.0 * e_ref)), (e_hip, e_ref)
     del F
     mink = idata.minkowski(grid, device=DEV)
     rhs_into(mink, out, S16, 4, 0.1)
