@@ -192,7 +192,7 @@ def main() -> int:
         n = int(args.copy_gib * (1 << 30)) // 8
         a = torch.ones(n, dtype=torch.float64, device=dev)
         b = torch.empty_like(a)
-        b.copy_(a)
+        torch.neg(a, out=b)          # an elementwise kernel: reads and writes copy_gib GiB
         torch.cuda.synchronize()
         del a, b
 
