@@ -28,38 +28,24 @@ int check_hip(hipError_t e, const char* what) {
   return fail(BSSN_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-// Centred stencils (common/src/mcs_common/derivatives.py:33-49): integer numerators and the
-// common denominator, per order.
-struct Stencil {
-  int n;
-  double num[BSSN_MAXTAP];
-  double den;
-};
-const Stencil C1_4 = {5, {1, -8, 0, 8, -1}, 12.0};
-const Stencil C1_6 = {7, {-1, 9, -45, 0, 45, -9, 1}, 60.0};
-const Stencil C2_4 = {5, {-1, 16, -30, 16, -1}, 12.0};
-const Stencil C2_6 = {7, {2, -27, 270, -490, 270, -27, 2}, 180.0};
-const Stencil KO_6 = {7, {1, -6, 15, -20, 15, -6, 1}, 64.0};
-const Stencil KO_8 = {9, {-1, 8, -28, 56, -70, 56, -28, 8, -1}, 256.0};
-
 bool valid_order(int ord) { return ord == 4 || ord == 6; }
 
-// stencil = (num / den) * factor, the order the reference forms it in (derivatives.py:79)
-void weights(const Stencil& s, double factor, double* w) {
-  for (int k = 0; k < BSSN_MAXTAP; ++k) w[k] = k < s.n ? (s.num[k] / s.den) * factor : 0.0;
-}
-
-FDW make_fdw(const double h3[3], int ord, double sigma) {
-  FDW w;
-  const Stencil& c1 = ord == 4 ? C1_4 : C1_6;
-  const Stencil& c2 = ord == 4 ? C2_4 : C2_6;
-  const Stencil& ko = ord == 4 ? KO_6 : KO_8;
+// Scales of the integer stencils (bssn_brick.h): the denominators of the centred and
+// Kreiss-Oliger coefficients (common/src/mcs_common/derivatives.py:33-49) times the spacings.
+Scales make_scales(const double h3[3], int ord, double sigma) {
+  Scales s;
+  const double den1 = ord == 4 ? 12.0 : 60.0;    // first derivative
+  const double den2 = ord == 4 ? 12.0 : 180.0;   // second derivative
+  const double denk = ord == 4 ? 64.0 : 256.0;   // KO of order ord + 2
   for (int a = 0; a < 3; ++a) {
-    weights(c1, 1.0 / h3[a], w.w1[a]);
-    weights(c2, 1.0 / (h3[a] * h3[a]), w.w2[a]);
-    weights(ko, sigma / h3[a], w.wko[a]);
+    s.s1[a] = 1.0 / (den1 * h3[a]);
+    s.s2[a] = 1.0 / (den2 * (h3[a] * h3[a]));
+    s.sko[a] = sigma / (denk * h3[a]);
   }
-  return w;
+  s.sm[0] = 1.0 / (den1 * den1 * (h3[0] * h3[1]));
+  s.sm[1] = 1.0 / (den1 * den1 * (h3[0] * h3[2]));
+  s.sm[2] = 1.0 / (den1 * den1 * (h3[1] * h3[2]));
+  return s;
 }
 
 // S16 = eta, C_CAHD, dt, dx_i, h_ssl, sig_ssl, t, lambda0..3, lambda_f0..1, dx, dy, dz
@@ -138,7 +124,7 @@ int check_ng(int Sx, int Sy, int Sz, int ng) {
     if (rc_ != BSSN_OK) return rc_; \
   } while (0)
 
-int run_stage(const StageArgs& a, const Dims& d, const FDW& w, const Scal& sc, int ord,
+int run_stage(const StageArgs& a, const Dims& d, const Scales& w, const Scal& sc, int ord,
               bool ko, int mode, hipStream_t st) {
   return check_hip(launch_fused(a, d, w, sc, ord, ko, mode, st), "fused stage launch");
 }
@@ -159,7 +145,7 @@ int bssn_rhs_f64(const double* F, const double* S16, double* OUT, int Sx, int Sy
   TRY(check_shape(Sx, Sy, Sz));
   TRY(check_s16(S16));
   const double h3[3] = {S16[13], S16[14], S16[15]};
-  const FDW w = make_fdw(h3, fd_order, ko_sigma);
+  const Scales w = make_scales(h3, fd_order, ko_sigma);
   const Scal sc = make_scal(S16, S16[2], S16[6]);
   const Dims d = full_dims(Sx, Sy, Sz, 0);
   StageArgs a{};
@@ -178,7 +164,7 @@ int bssn_stage_rhs_f64(const double* Y, double* OUT, int Sx, int Sy, int Sz, con
   TRY(check_s16(S16));
   if (periodic) TRY(check_ng(Sx, Sy, Sz, ng));
   const double h3[3] = {S16[13], S16[14], S16[15]};
-  const FDW w = make_fdw(h3, fd_order, ko_sigma);
+  const Scales w = make_scales(h3, fd_order, ko_sigma);
   const Scal sc = make_scal(S16, S16[2], S16[6]);
   const Dims d = periodic ? interior_dims(Sx, Sy, Sz, ng) : full_dims(Sx, Sy, Sz, ng);
   StageArgs a{};
@@ -195,7 +181,7 @@ int bssn_deriv_bundle_f64(const double* F, double* D, int Sx, int Sy, int Sz, in
   if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4 or 6");
   TRY(check_shape(Sx, Sy, Sz));
   TRY(check_h(h3));
-  const FDW w = make_fdw(h3, fd_order, 0.0);
+  const Scales w = make_scales(h3, fd_order, 0.0);
   return check_hip(launch_deriv_bundle(F, D, full_dims(Sx, Sy, Sz, 0), w, fd_order,
                                        (hipStream_t)stream),
                    "deriv bundle launch");
@@ -218,7 +204,7 @@ int bssn_ko_add_f64(const double* F, double* OUT, int Sx, int Sy, int Sz, int fd
   TRY(check_h(h3));
   if (periodic) TRY(check_ng(Sx, Sy, Sz, ng));
   if (!(sigma > 0.0)) return BSSN_OK;   // the reference adds 0 for sigma <= 0
-  const FDW w = make_fdw(h3, fd_order, sigma);
+  const Scales w = make_scales(h3, fd_order, sigma);
   const Dims d = periodic ? interior_dims(Sx, Sy, Sz, ng) : full_dims(Sx, Sy, Sz, ng);
   return check_hip(launch_ko_add(F, OUT, d, w, fd_order, periodic ? WRAP : CLAMP,
                                  (hipStream_t)stream),
@@ -265,7 +251,7 @@ int bssn_rk4_step_f64(double* U, double* work, int Sx, int Sy, int Sz, const dou
   double* Yb = work + NS;
   double* KS = work + 2 * NS;
   const double h3[3] = {S16[13], S16[14], S16[15]};
-  const FDW w = make_fdw(h3, fd_order, ko_sigma);
+  const Scales w = make_scales(h3, fd_order, ko_sigma);
   const Dims d = periodic ? interior_dims(Sx, Sy, Sz, ng) : full_dims(Sx, Sy, Sz, ng);
   const int mode = periodic ? WRAP : CLAMP;
   const bool ko = ko_sigma > 0.0;
@@ -313,7 +299,7 @@ int bssn_msrk_step_f64(double* U, double* fprev, double* fnext, double* work, in
   double* Ya = work;
   double* Yb = work + NS;
   const double h3[3] = {S16[13], S16[14], S16[15]};
-  const FDW w = make_fdw(h3, fd_order, ko_sigma);
+  const Scales w = make_scales(h3, fd_order, ko_sigma);
   const Dims d = periodic ? interior_dims(Sx, Sy, Sz, ng) : full_dims(Sx, Sy, Sz, ng);
   const int mode = periodic ? WRAP : CLAMP;
   const bool ko = ko_sigma > 0.0;
@@ -347,7 +333,7 @@ int bssn_constraints_f64(const double* F, double* HM, int Sx, int Sy, int Sz, in
   if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4 or 6");
   TRY(check_shape(Sx, Sy, Sz));
   TRY(check_h(h3));
-  const FDW w = make_fdw(h3, fd_order, 0.0);
+  const Scales w = make_scales(h3, fd_order, 0.0);
   return check_hip(launch_constraints(F, HM, full_dims(Sx, Sy, Sz, 0), w, fd_order,
                                       (hipStream_t)stream),
                    "constraints launch");

@@ -1,5 +1,6 @@
 // The BSSN right-hand side at one grid point, ordered field by field so that each derivative
-// is consumed as soon as it is formed (gfx950 register budget).
+// is consumed as soon as it is formed and the big intermediates (the Christoffels, the Ricci
+// tensor) die early (gfx950 register budget).
 //
 // Equations: the CAHD + SSL moving-puncture BSSN system the reference evaluates
 // (3D/src/bssn3d/vendor/bssneqs_SSL_HD_dxsq.cpp, output statements a_rhs .. B_rhs2, through
@@ -9,13 +10,14 @@
 //   R_ij = Rt_ij + Rchi_ij (conformal Ricci: metric part with the evolved Gt^k in the
 //          d_(i Gt_j) term, chi part with CalGt), H = chi gt^ij R_ij - At.At + 2K^2/3,
 //   DDalpha_ij = d_i d_j alpha - Gamma3^l_ij d_l alpha (physical Christoffels),
-// and the evolution equations for alpha (1+log-free harmonic-K form with slice locking),
-// beta (Gamma-driver), gt, chi (with Hamiltonian damping), At, K, Gt, B. Advection is
-// centred. Parity: every output agrees with the compiled reference CSE to round-off
-// (tests/test_oracle_parity.py, tests/test_hip_parity.py).
+// and the evolution equations for alpha (with slice locking), beta (Gamma-driver), gt, chi (with
+// Hamiltonian damping), At, K, Gt, B. Advection is centred. Parity: every output agrees with the
+// compiled reference CSE to round-off (tests/test_oracle.py, tests/test_hip_parity.py).
 //
-// Src must provide (all calls uniform across the workgroup):
-//   void   stage(int f)        make field f current (may synchronise the workgroup)
+// Phase order (fields staged):  gt0..5 | chi | Gt0..2 | alpha | beta0..2 | At0..5 | K | B0..2
+//
+// Src must provide (all calls uniform across the wave):
+//   void   stage(int f)        make field f current
 //   double val()               current field at the point
 //   double d1(int a)           first derivative of the current field along axis a
 //   double d2(int i, int j)    second derivative, i <= j (mixed = d1_j(d1_i))
@@ -44,6 +46,40 @@ __device__ __forceinline__ double contract_sym(const double* a, const double* b)
   return a[0] * b[0] + a[3] * b[3] + a[5] * b[5] + 2.0 * (a[1] * b[1] + a[2] * b[2] + a[4] * b[4]);
 }
 
+// inverse of a packed symmetric 3x3 matrix; returns the determinant
+__device__ __forceinline__ double inverse_sym(const double* g, double* ig) {
+  const double a00 = g[3] * g[5] - g[4] * g[4];
+  const double a01 = g[2] * g[4] - g[1] * g[5];
+  const double a02 = g[1] * g[4] - g[2] * g[3];
+  const double a11 = g[0] * g[5] - g[2] * g[2];
+  const double a12 = g[1] * g[2] - g[0] * g[4];
+  const double a22 = g[0] * g[3] - g[1] * g[1];
+  const double det = g[0] * a00 + g[1] * a01 + g[2] * a02;
+  const double idet = 1.0 / det;
+  ig[0] = a00 * idet; ig[1] = a01 * idet; ig[2] = a02 * idet;
+  ig[3] = a11 * idet; ig[4] = a12 * idet; ig[5] = a22 * idet;
+  return det;
+}
+
+// At^ij (packed), At_ij At^ij and At_ik gt^kl At_lj from At_ij and gt^ij
+__device__ __forceinline__ void raise_A(const double* A, const double* ig, double* Aup,
+                                        double* AA, double* AkA) {
+  double IA[3][3];   // IA[k][j] = gt^kl At_lj
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      IA[k][j] = ig[sym(k, 0)] * A[sym(0, j)] + ig[sym(k, 1)] * A[sym(1, j)] +
+                 ig[sym(k, 2)] * A[sym(2, j)];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const int i = pi_(q), j = pj_(q);
+    Aup[q] = IA[i][0] * ig[sym(0, j)] + IA[i][1] * ig[sym(1, j)] + IA[i][2] * ig[sym(2, j)];
+    if (AkA) AkA[q] = A[sym(i, 0)] * IA[0][j] + A[sym(i, 1)] * IA[1][j] + A[sym(i, 2)] * IA[2][j];
+  }
+  *AA = contract_sym(Aup, A);
+}
+
 template <bool KO, class Src, class Out>
 __device__ __forceinline__ void bssn_rhs_point(Src& s, const Scal& P, Out& o) {
   // ---------------------------------------------------------------- metric and shift centres
@@ -52,39 +88,28 @@ __device__ __forceinline__ void bssn_rhs_point(Src& s, const Scal& P, Out& o) {
   for (int q = 0; q < 6; ++q) g[q] = s.center(GT0 + q);
 #pragma unroll
   for (int i = 0; i < 3; ++i) b[i] = s.center(BETA0 + i);
-  {
-    const double a00 = g[3] * g[5] - g[4] * g[4];
-    const double a01 = g[2] * g[4] - g[1] * g[5];
-    const double a02 = g[1] * g[4] - g[2] * g[3];
-    const double a11 = g[0] * g[5] - g[2] * g[2];
-    const double a12 = g[1] * g[2] - g[0] * g[4];
-    const double a22 = g[0] * g[3] - g[1] * g[1];
-    const double det = g[0] * a00 + g[1] * a01 + g[2] * a02;
-    const double idet = 1.0 / det;
-    ig[0] = a00 * idet; ig[1] = a01 * idet; ig[2] = a02 * idet;
-    ig[3] = a11 * idet; ig[4] = a12 * idet; ig[5] = a22 * idet;
-  }
+  inverse_sym(g, ig);
 #pragma unroll
   for (int q = 0; q < 6; ++q) BSSN_PIN(ig[q]);
 
   // ---------------------------------------------------------------- gt_ij: Christoffels, Rt
-  double dg[3][6];     // d_k gt_q
   double Rt[6];        // accumulates the Ricci tensor
-  double rg[6];        // gt right-hand sides (partial)
-#pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    s.stage(GT0 + q);
-#pragma unroll
-    for (int a = 0; a < 3; ++a) dg[a][q] = s.d1(a);
-    const double lapg = ig[0] * s.d2(0, 0) + ig[3] * s.d2(1, 1) + ig[5] * s.d2(2, 2) +
-                        2.0 * (ig[1] * s.d2(0, 1) + ig[2] * s.d2(0, 2) + ig[4] * s.d2(1, 2));
-    Rt[q] = -0.5 * lapg;
-    double r = b[0] * dg[0][q] + b[1] * dg[1][q] + b[2] * dg[2][q];
-    if (KO) r += s.ko();
-    rg[q] = r;
-  }
+  double rg[6];        // gt right-hand sides (advection + KO so far)
   double C2[3][6], CalGt[3];
   {
+    double dg[3][6];   // d_k gt_q
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      s.stage(GT0 + q);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) dg[a][q] = s.d1(a);
+      const double lapg = ig[0] * s.d2(0, 0) + ig[3] * s.d2(1, 1) + ig[5] * s.d2(2, 2) +
+                          2.0 * (ig[1] * s.d2(0, 1) + ig[2] * s.d2(0, 2) + ig[4] * s.d2(1, 2));
+      Rt[q] = -0.5 * lapg;
+      double r = b[0] * dg[0][q] + b[1] * dg[1][q] + b[2] * dg[2][q];
+      if (KO) r += s.ko();
+      rg[q] = pin(r);
+    }
     double C1[3][6];   // Gamma_k(ij), first index lowered
 #pragma unroll
     for (int k = 0; k < 3; ++k)
@@ -100,9 +125,9 @@ __device__ __forceinline__ void bssn_rhs_point(Src& s, const Scal& P, Out& o) {
         C2[m][q] = ig[sym(m, 0)] * C1[0][q] + ig[sym(m, 1)] * C1[1][q] + ig[sym(m, 2)] * C1[2][q];
 #pragma unroll
     for (int m = 0; m < 3; ++m) CalGt[m] = contract_sym(ig, C2[m]);
-    // raised Christoffels: Gu[i][k][m] = gt^ml Gamma^k_li,  Gd[k][j][m] = gt^ml Gamma_klj
     //   Rt_ij += CalGt^k (Gamma_ijk + Gamma_jik)/2
-    //          + sum_km (Gu[i][k][m] Gamma_jkm + Gu[j][k][m] Gamma_ikm + Gamma^k_im Gd[k][j][m])
+    //          + gt^lm (Gamma^k_li Gamma_jkm + Gamma^k_lj Gamma_ikm + Gamma^k_im Gamma_klj)
+    // with the raised forms Gu[i][k][m] = gt^ml Gamma^k_li and gt^ml Gamma_klj.
     double Gu[3][3][3];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -139,14 +164,28 @@ __device__ __forceinline__ void bssn_rhs_point(Src& s, const Scal& P, Out& o) {
 #pragma unroll
   for (int m = 0; m < 3; ++m) BSSN_PIN(CalGt[m]);
 
+  // At^ij from the At centres, for the two contractions needed before the Christoffels die:
+  // Gamma^i_jk At^jk and At_ij At^ij.
+  double CA[3], AA;
+  {
+    double A[6], Aup[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) A[q] = s.center(AT0 + q);
+    raise_A(A, ig, Aup, &AA, nullptr);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) CA[i] = pin(contract_sym(C2[i], Aup));
+    BSSN_PIN(AA);
+  }
+
   // ---------------------------------------------------------------- chi: Ricci chi part
-  double chi, dchi[3], igdchi[3];
+  double chi, dchi[3];
   double rchi;
   {
     s.stage(CHI);
     chi = s.val();
 #pragma unroll
     for (int a = 0; a < 3; ++a) dchi[a] = s.d1(a);
+    double igdchi[3];
 #pragma unroll
     for (int m = 0; m < 3; ++m)
       igdchi[m] = ig[sym(m, 0)] * dchi[0] + ig[sym(m, 1)] * dchi[1] + ig[sym(m, 2)] * dchi[2];
@@ -168,8 +207,7 @@ __device__ __forceinline__ void bssn_rhs_point(Src& s, const Scal& P, Out& o) {
     if (KO) rchi += s.ko();
 #pragma unroll
     for (int q = 0; q < 6; ++q) BSSN_PIN(Rt[q]);
-#pragma unroll
-    for (int m = 0; m < 3; ++m) BSSN_PIN(igdchi[m]);
+    BSSN_PIN(rchi);
   }
 
   // ---------------------------------------------------------------- Gt^k: d_(i Gt_j) term
@@ -185,52 +223,25 @@ __device__ __forceinline__ void bssn_rhs_point(Src& s, const Scal& P, Out& o) {
       const int i = pi_(q), j = pj_(q);
       Rt[q] += 0.5 * (g[sym(k, i)] * dG[j] + g[sym(k, j)] * dG[i]);
     }
-    advGt[k] = b[0] * dG[0] + b[1] * dG[1] + b[2] * dG[2];
+    advGt[k] = pin(b[0] * dG[0] + b[1] * dG[1] + b[2] * dG[2]);
     koGt[k] = KO ? s.ko() : 0.0;
   }
   // R_ij is complete from here on.
 
-  // ---------------------------------------------------------------- At_ij
-  double A[6], rA[6];
-#pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    s.stage(AT0 + q);
-    A[q] = s.val();
-    double r = b[0] * s.d1(0) + b[1] * s.d1(1) + b[2] * s.d1(2);
-    if (KO) r += s.ko();
-    rA[q] = r;
-  }
-  double Aup[6], AA, AkA[6];   // At^ij, At_ij At^ij, At_ik gt^kl At_lj
-  {
-    double IA[3][3];   // IA[k][j] = gt^kl At_lj
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-#pragma unroll
-      for (int j = 0; j < 3; ++j)
-        IA[k][j] = ig[sym(k, 0)] * A[sym(0, j)] + ig[sym(k, 1)] * A[sym(1, j)] +
-                   ig[sym(k, 2)] * A[sym(2, j)];
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-      const int i = pi_(q), j = pj_(q);
-      Aup[q] = IA[i][0] * ig[sym(0, j)] + IA[i][1] * ig[sym(1, j)] + IA[i][2] * ig[sym(2, j)];
-      AkA[q] = A[sym(i, 0)] * IA[0][j] + A[sym(i, 1)] * IA[1][j] + A[sym(i, 2)] * IA[2][j];
-    }
-    AA = contract_sym(Aup, A);
-  }
-#pragma unroll
-  for (int q = 0; q < 6; ++q) { BSSN_PIN(Aup[q]); BSSN_PIN(AkA[q]); }
-  BSSN_PIN(AA);
-
   // ---------------------------------------------------------------- alpha: D_iD_j alpha
   const double K = s.center(KTR);
-  double al, dal[3], rGt[3], rK;
+  double al, dal[3], rGt[3], rK, rA[6];
   {
     s.stage(ALPHA);
     al = s.val();
 #pragma unroll
     for (int a = 0; a < 3; ++a) dal[a] = s.d1(a);
     const double ichi = 1.0 / chi;
-    const double igdd = igdchi[0] * dal[0] + igdchi[1] * dal[1] + igdchi[2] * dal[2];
+    double igdd = 0.0;
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+      igdd += (ig[sym(m, 0)] * dchi[0] + ig[sym(m, 1)] * dchi[1] + ig[sym(m, 2)] * dchi[2]) *
+              dal[m];
     double DD[6];
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
@@ -245,9 +256,8 @@ __device__ __forceinline__ void bssn_rhs_point(Src& s, const Scal& P, Out& o) {
     for (int q = 0; q < 6; ++q) X[q] = al * Rt[q] - DD[q];
     const double trX3 = contract_sym(ig, X) * (1.0 / 3.0);
 #pragma unroll
-    for (int q = 0; q < 6; ++q) rA[q] += chi * (X[q] - g[q] * trX3);
-    const double lap = contract_sym(ig, DD);
-    rK = -chi * lap;
+    for (int q = 0; q < 6; ++q) rA[q] = pin(chi * (X[q] - g[q] * trX3));
+    rK = pin(-chi * contract_sym(ig, DD));
     // Hamiltonian constraint and its damping of chi (CAHD)
     const double ham = chi * Rs - AA + (2.0 / 3.0) * K * K;
     rchi += P.cahd * chi * ham;
@@ -257,19 +267,10 @@ __device__ __forceinline__ void bssn_rhs_point(Src& s, const Scal& P, Out& o) {
                 sq * P.ssl_h * (al - sq) * P.ssl_e;
     if (KO) ra += s.ko();
     o.put(ALPHA, ra);
-    // connection-function terms that need alpha
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      double t = 0.0, u = 0.0, w = 0.0;
-#pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        t += Aup[sym(i, j)] * dal[j];
-        w += Aup[sym(i, j)] * dchi[j];
-      }
-      u = contract_sym(C2[i], Aup);
-      rGt[i] = advGt[i] - 2.0 * t + 2.0 * al * u - 3.0 * al * ichi * w;
-    }
+    for (int i = 0; i < 3; ++i) rGt[i] = pin(advGt[i] + 2.0 * al * CA[i]);
   }
+  // The Christoffels and the Ricci tensor are dead from here on.
 
   // ---------------------------------------------------------------- beta^i
   double db[3][3];   // db[j][i] = d_j beta^i
@@ -300,17 +301,46 @@ __device__ __forceinline__ void bssn_rhs_point(Src& s, const Scal& P, Out& o) {
   for (int i = 0; i < 3; ++i)
     rGt[i] += -(CalGt[0] * db[0][i] + CalGt[1] * db[1][i] + CalGt[2] * db[2][i]) +
               (2.0 / 3.0) * CalGt[i] * divb;
-  // conformal metric
-#pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    const int i = pi_(q), j = pj_(q);
-    double lie = 0.0;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) lie += g[sym(i, k)] * db[j][k] + g[sym(k, j)] * db[i][k];
-    o.put(GT0 + q, rg[q] + lie - (2.0 / 3.0) * g[q] * divb - 2.0 * al * A[q]);
-  }
   rchi += (2.0 / 3.0) * chi * (al * K - divb);
   o.put(CHI, rchi);
+
+  // ---------------------------------------------------------------- At_ij, and gt_ij
+  {
+    double A[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      s.stage(AT0 + q);
+      A[q] = s.val();
+      double r = b[0] * s.d1(0) + b[1] * s.d1(1) + b[2] * s.d1(2);
+      if (KO) r += s.ko();
+      rA[q] += r;
+    }
+    double Aup[6], AA2, AkA[6];
+    raise_A(A, ig, Aup, &AA2, AkA);
+    const double ichi = 1.0 / chi;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double t = 0.0, w = 0.0;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        t += Aup[sym(i, j)] * dal[j];
+        w += Aup[sym(i, j)] * dchi[j];
+      }
+      rGt[i] += -2.0 * t - 3.0 * al * ichi * w;
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const int i = pi_(q), j = pj_(q);
+      double lie = 0.0, lieg = 0.0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        lie += A[sym(i, k)] * db[j][k] + A[sym(k, j)] * db[i][k];
+        lieg += g[sym(i, k)] * db[j][k] + g[sym(k, j)] * db[i][k];
+      }
+      o.put(AT0 + q, rA[q] + lie - (2.0 / 3.0) * A[q] * divb + al * (K * A[q] - 2.0 * AkA[q]));
+      o.put(GT0 + q, rg[q] + lieg - (2.0 / 3.0) * g[q] * divb - 2.0 * al * A[q]);
+    }
+  }
 
   // ---------------------------------------------------------------- K
   {
@@ -326,15 +356,6 @@ __device__ __forceinline__ void bssn_rhs_point(Src& s, const Scal& P, Out& o) {
       rGt[i] -= (4.0 / 3.0) * al *
                 (ig[sym(i, 0)] * dK[0] + ig[sym(i, 1)] * dK[1] + ig[sym(i, 2)] * dK[2]);
   }
-  // At
-#pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    const int i = pi_(q), j = pj_(q);
-    double lie = 0.0;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) lie += A[sym(i, k)] * db[j][k] + A[sym(k, j)] * db[i][k];
-    o.put(AT0 + q, rA[q] + lie - (2.0 / 3.0) * A[q] * divb + al * (K * A[q] - 2.0 * AkA[q]));
-  }
 
   // ---------------------------------------------------------------- B^i and Gt^i
 #pragma unroll
@@ -347,7 +368,6 @@ __device__ __forceinline__ void bssn_rhs_point(Src& s, const Scal& P, Out& o) {
     o.put(GTI0 + i, rGt[i] + koGt[i]);
   }
 }
-
 
 // Hamiltonian and momentum constraints at one point (3D/src/bssn3d/constraints.py:38-51, the
 // reference's physcon CSE 3D/src/bssn3d/vendor/physcon.cpp:360-1287):

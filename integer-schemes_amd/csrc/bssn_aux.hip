@@ -26,42 +26,17 @@ __host__ __device__ constexpr int s2slot(int f) {
                                           : (f >= BETA0 && f <= BETA2 ? 8 + f - BETA0 : -1)));
 }
 
-template <class G>
-__device__ __forceinline__ bool brick_point(const Dims& d, int* ox, int* oy, int* oz, int* tx,
-                                            int* ty, int* tz) {
-  const int nbx = (d.n[0] + G::BX - 1) / G::BX;
-  const int nby = (d.n[1] + G::BY - 1) / G::BY;
-  const int nbz = (d.n[2] + G::BZ - 1) / G::BZ;
-  int bx, by, bz;
-  if (!brick_of_block(nbx, nby, nbz, &bx, &by, &bz)) return false;
-  const int t = (int)threadIdx.x;
-  *tz = t % G::BZ;
-  *ty = (t / G::BZ) % G::BY;
-  *tx = t / (G::BZ * G::BY);
-  *ox = d.lo[0] + bx * G::BX;
-  *oy = d.lo[1] + by * G::BY;
-  *oz = d.lo[2] + bz * G::BZ;
-  return true;
-}
-
-template <class G>
-static unsigned brick_blocks(const Dims& d) {
-  const long nb = (long)((d.n[0] + G::BX - 1) / G::BX) * ((d.n[1] + G::BY - 1) / G::BY) *
-                  ((d.n[2] + G::BZ - 1) / G::BZ);
-  return (unsigned)((nb + 7) & ~7L);
-}
-
 // ------------------------------------------------------------------------- pass 1: bundle
 template <int ORD>
-__global__ __launch_bounds__(256) void k_deriv_bundle(const double* __restrict__ F,
-                                                      double* __restrict__ D, Dims d, FDW w) {
+__global__ __launch_bounds__(64) void k_deriv_bundle(const double* __restrict__ F,
+                                                     double* __restrict__ D, Dims d, Scales w) {
   using G = Geo<ORD, false>;
-  __shared__ double lds[2 * G::LVOL];
-  int ox, oy, oz, tx, ty, tz;
-  if (!brick_point<G>(d, &ox, &oy, &oz, &tx, &ty, &tz)) return;
-  const bool valid = (ox + tx < d.lo[0] + d.n[0]) && (oy + ty < d.lo[1] + d.n[1]) &&
-                     (oz + tz < d.lo[2] + d.n[2]);
-  BrickSrc<ORD, false, CLAMP> s(lds, F, w, d, ox, oy, oz, tx, ty, tz);
+  __shared__ double lds[G::VOL];
+  int ox, oy, oz;
+  if (!brick_origin<G>(d, &ox, &oy, &oz)) return;
+  const int lane = (int)threadIdx.x;
+  const bool valid = lane_valid<G>(d, ox, oy, oz, lane);
+  WaveSrc<ORD, false, CLAMP> s(lds, F, w, d, ox, oy, oz, lane);
 #pragma unroll 1
   for (int f = 0; f < BSSN_NF; ++f) {
     s.stage(f);
@@ -81,16 +56,17 @@ __global__ __launch_bounds__(256) void k_deriv_bundle(const double* __restrict__
         for (int q = 0; q < 6; ++q) D[(long)(72 + 6 * sl + q) * d.P + s.gc] = v[3 + q];
       }
     }
+    __builtin_amdgcn_wave_barrier();
   }
 }
 
-hipError_t launch_deriv_bundle(const double* F, double* D, const Dims& d, const FDW& w, int ord,
-                               hipStream_t st) {
+hipError_t launch_deriv_bundle(const double* F, double* D, const Dims& d, const Scales& w,
+                               int ord, hipStream_t st) {
   if (ord == 4)
-    hipLaunchKernelGGL(k_deriv_bundle<4>, dim3(brick_blocks<Geo<4, false>>(d)), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_deriv_bundle<4>, dim3(brick_blocks<Geo<4, false>>(d)), dim3(64), 0, st,
                        F, D, d, w);
   else
-    hipLaunchKernelGGL(k_deriv_bundle<6>, dim3(brick_blocks<Geo<6, false>>(d)), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_deriv_bundle<6>, dim3(brick_blocks<Geo<6, false>>(d)), dim3(64), 0, st,
                        F, D, d, w);
   return hipGetLastError();
 }
@@ -136,15 +112,15 @@ hipError_t launch_algebra(const double* F, const double* D, double* OUT, long np
 
 // ------------------------------------------------------------------------- KO pass
 template <int ORD, int MODE>
-__global__ __launch_bounds__(256) void k_ko_add(const double* __restrict__ F,
-                                                double* __restrict__ OUT, Dims d, FDW w) {
+__global__ __launch_bounds__(64) void k_ko_add(const double* __restrict__ F,
+                                               double* __restrict__ OUT, Dims d, Scales w) {
   using G = Geo<ORD, true>;
-  __shared__ double lds[2 * G::LVOL];
-  int ox, oy, oz, tx, ty, tz;
-  if (!brick_point<G>(d, &ox, &oy, &oz, &tx, &ty, &tz)) return;
-  const bool valid = (ox + tx < d.lo[0] + d.n[0]) && (oy + ty < d.lo[1] + d.n[1]) &&
-                     (oz + tz < d.lo[2] + d.n[2]);
-  BrickSrc<ORD, true, MODE> s(lds, F, w, d, ox, oy, oz, tx, ty, tz);
+  __shared__ double lds[G::VOL];
+  int ox, oy, oz;
+  if (!brick_origin<G>(d, &ox, &oy, &oz)) return;
+  const int lane = (int)threadIdx.x;
+  const bool valid = lane_valid<G>(d, ox, oy, oz, lane);
+  WaveSrc<ORD, true, MODE> s(lds, F, w, d, ox, oy, oz, lane);
 #pragma unroll 1
   for (int f = 0; f < BSSN_NF; ++f) {
     s.stage(f);
@@ -153,24 +129,25 @@ __global__ __launch_bounds__(256) void k_ko_add(const double* __restrict__ F,
       const long i = (long)f * d.P + s.gc;
       OUT[i] = OUT[i] + k;
     }
+    __builtin_amdgcn_wave_barrier();
   }
 }
 
-hipError_t launch_ko_add(const double* F, double* OUT, const Dims& d, const FDW& w, int ord,
+hipError_t launch_ko_add(const double* F, double* OUT, const Dims& d, const Scales& w, int ord,
                          int mode, hipStream_t st) {
   if (ord == 4) {
     if (mode == WRAP)
-      hipLaunchKernelGGL((k_ko_add<4, WRAP>), dim3(brick_blocks<Geo<4, true>>(d)), dim3(256), 0,
+      hipLaunchKernelGGL((k_ko_add<4, WRAP>), dim3(brick_blocks<Geo<4, true>>(d)), dim3(64), 0,
                          st, F, OUT, d, w);
     else
-      hipLaunchKernelGGL((k_ko_add<4, CLAMP>), dim3(brick_blocks<Geo<4, true>>(d)), dim3(256), 0,
+      hipLaunchKernelGGL((k_ko_add<4, CLAMP>), dim3(brick_blocks<Geo<4, true>>(d)), dim3(64), 0,
                          st, F, OUT, d, w);
   } else {
     if (mode == WRAP)
-      hipLaunchKernelGGL((k_ko_add<6, WRAP>), dim3(brick_blocks<Geo<6, true>>(d)), dim3(256), 0,
+      hipLaunchKernelGGL((k_ko_add<6, WRAP>), dim3(brick_blocks<Geo<6, true>>(d)), dim3(64), 0,
                          st, F, OUT, d, w);
     else
-      hipLaunchKernelGGL((k_ko_add<6, CLAMP>), dim3(brick_blocks<Geo<6, true>>(d)), dim3(256), 0,
+      hipLaunchKernelGGL((k_ko_add<6, CLAMP>), dim3(brick_blocks<Geo<6, true>>(d)), dim3(64), 0,
                          st, F, OUT, d, w);
   }
   return hipGetLastError();
@@ -268,26 +245,25 @@ struct ConsOut {
 };
 
 template <int ORD>
-__global__ __launch_bounds__(256) void k_constraints(const double* __restrict__ F,
-                                                     double* __restrict__ HM, Dims d, FDW w) {
+__global__ __launch_bounds__(64) void k_constraints(const double* __restrict__ F,
+                                                    double* __restrict__ HM, Dims d, Scales w) {
   using G = Geo<ORD, false>;
-  __shared__ double lds[2 * G::LVOL];
-  int ox, oy, oz, tx, ty, tz;
-  if (!brick_point<G>(d, &ox, &oy, &oz, &tx, &ty, &tz)) return;
-  const bool valid = (ox + tx < d.lo[0] + d.n[0]) && (oy + ty < d.lo[1] + d.n[1]) &&
-                     (oz + tz < d.lo[2] + d.n[2]);
-  BrickSrc<ORD, false, CLAMP> s(lds, F, w, d, ox, oy, oz, tx, ty, tz);
-  ConsOut o{HM, d.P, s.gc, valid};
+  __shared__ double lds[G::VOL];
+  int ox, oy, oz;
+  if (!brick_origin<G>(d, &ox, &oy, &oz)) return;
+  const int lane = (int)threadIdx.x;
+  WaveSrc<ORD, false, CLAMP> s(lds, F, w, d, ox, oy, oz, lane);
+  ConsOut o{HM, d.P, s.gc, lane_valid<G>(d, ox, oy, oz, lane)};
   bssn_constraints_point(s, o);
 }
 
-hipError_t launch_constraints(const double* F, double* HM, const Dims& d, const FDW& w, int ord,
-                              hipStream_t st) {
+hipError_t launch_constraints(const double* F, double* HM, const Dims& d, const Scales& w,
+                              int ord, hipStream_t st) {
   if (ord == 4)
-    hipLaunchKernelGGL(k_constraints<4>, dim3(brick_blocks<Geo<4, false>>(d)), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_constraints<4>, dim3(brick_blocks<Geo<4, false>>(d)), dim3(64), 0, st,
                        F, HM, d, w);
   else
-    hipLaunchKernelGGL(k_constraints<6>, dim3(brick_blocks<Geo<6, false>>(d)), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_constraints<6>, dim3(brick_blocks<Geo<6, false>>(d)), dim3(64), 0, st,
                        F, HM, d, w);
   return hipGetLastError();
 }

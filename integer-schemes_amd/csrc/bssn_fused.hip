@@ -1,7 +1,7 @@
 // Fused BSSN right-hand side and fused RK4 / MSRK stage kernels for gfx950.
 //
-// One kernel evaluates, per output point, the 138 finite-difference derivatives (from LDS
-// bricks, bssn_brick.h), the Kreiss-Oliger terms, the pointwise BSSN algebra
+// One kernel evaluates, per output point, the 138 finite-difference derivatives (from
+// wave-private LDS bricks, bssn_brick.h), the Kreiss-Oliger terms, the pointwise BSSN algebra
 // (bssn_algebra.h) and the write-back: either the bare RHS, or an RK stage update
 // (next stage input + running combination), with the algebraic enforcement of the last stage
 // folded into the final store.
@@ -121,40 +121,30 @@ struct StageOut {
 #endif
 
 template <int ORD, bool KO, int MODE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BSSN_FUSED_WAVES, BSSN_FUSED_WAVES)))
-void k_fused(StageArgs args, Dims d, FDW w, Scal sc, int nbx,
-                                               int nby, int nbz) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BSSN_FUSED_WAVES, BSSN_FUSED_WAVES)))
+void k_fused(StageArgs args, Dims d, Scales w, Scal sc) {
   using G = Geo<ORD, KO>;
-  __shared__ double lds[2 * G::LVOL];
-  int bx, by, bz;
-  if (!brick_of_block(nbx, nby, nbz, &bx, &by, &bz)) return;
-  const int t = (int)threadIdx.x;
-  const int tz = t % G::BZ, ty = (t / G::BZ) % G::BY, tx = t / (G::BZ * G::BY);
-  const int ox = d.lo[0] + bx * G::BX, oy = d.lo[1] + by * G::BY, oz = d.lo[2] + bz * G::BZ;
-  const bool valid = (ox + tx < d.lo[0] + d.n[0]) && (oy + ty < d.lo[1] + d.n[1]) &&
-                     (oz + tz < d.lo[2] + d.n[2]);
-  BrickSrc<ORD, KO, MODE> src(lds, args.Yin, w, d, ox, oy, oz, tx, ty, tz);
-  StageOut out(args, d.P, src.gc, valid);
+  __shared__ double lds[G::VOL];
+  int ox, oy, oz;
+  if (!brick_origin<G>(d, &ox, &oy, &oz)) return;
+  const int lane = (int)threadIdx.x;
+  WaveSrc<ORD, KO, MODE> src(lds, args.Yin, w, d, ox, oy, oz, lane);
+  StageOut out(args, d.P, src.gc, lane_valid<G>(d, ox, oy, oz, lane));
   bssn_rhs_point<KO>(src, sc, out);
   out.finish();
 }
 
 template <int ORD, bool KO, int MODE>
-static hipError_t launch_fused_t(const StageArgs& a, const Dims& d, const FDW& w, const Scal& sc,
-                                 hipStream_t st) {
+static hipError_t launch_fused_t(const StageArgs& a, const Dims& d, const Scales& w,
+                                 const Scal& sc, hipStream_t st) {
   using G = Geo<ORD, KO>;
-  const int nbx = (d.n[0] + G::BX - 1) / G::BX;
-  const int nby = (d.n[1] + G::BY - 1) / G::BY;
-  const int nbz = (d.n[2] + G::BZ - 1) / G::BZ;
-  const long nb = (long)nbx * nby * nbz;
-  const long nb8 = (nb + 7) & ~7L;
-  hipLaunchKernelGGL((k_fused<ORD, KO, MODE>), dim3((unsigned)nb8), dim3(G::NT), 0, st, a, d, w,
-                     sc, nbx, nby, nbz);
+  hipLaunchKernelGGL((k_fused<ORD, KO, MODE>), dim3(brick_blocks<G>(d)), dim3(G::NT), 0, st, a,
+                     d, w, sc);
   return hipGetLastError();
 }
 
-hipError_t launch_fused(const StageArgs& a, const Dims& d, const FDW& w, const Scal& sc, int ord,
-                        bool ko, int mode, hipStream_t st) {
+hipError_t launch_fused(const StageArgs& a, const Dims& d, const Scales& w, const Scal& sc,
+                        int ord, bool ko, int mode, hipStream_t st) {
   if (ord == 4) {
     if (ko) return mode == WRAP ? launch_fused_t<4, true, WRAP>(a, d, w, sc, st)
                                 : launch_fused_t<4, true, CLAMP>(a, d, w, sc, st);
@@ -165,15 +155,6 @@ hipError_t launch_fused(const StageArgs& a, const Dims& d, const FDW& w, const S
                               : launch_fused_t<6, true, CLAMP>(a, d, w, sc, st);
   return mode == WRAP ? launch_fused_t<6, false, WRAP>(a, d, w, sc, st)
                       : launch_fused_t<6, false, CLAMP>(a, d, w, sc, st);
-}
-
-int fused_halo(int ord, bool ko) {
-  if (ord == 4) return ko ? Geo<4, true>::H : Geo<4, false>::H;
-  return ko ? Geo<6, true>::H : Geo<6, false>::H;
-}
-
-int fused_brick(int axis) {
-  return axis == 0 ? Geo<4, true>::BX : (axis == 1 ? Geo<4, true>::BY : Geo<4, true>::BZ);
 }
 
 }  // namespace bssn

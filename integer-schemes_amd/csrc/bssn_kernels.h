@@ -35,23 +35,22 @@ struct StageArgs {
   double chi_floor, alpha_floor;
 };
 
-hipError_t launch_fused(const StageArgs& a, const Dims& d, const FDW& w, const Scal& sc, int ord,
+hipError_t launch_fused(const StageArgs& a, const Dims& d, const Scales& w, const Scal& sc, int ord,
                         bool ko, int mode, hipStream_t st);
-int fused_halo(int ord, bool ko);
 
 // two-pass (materialised derivative bundle) path
-hipError_t launch_deriv_bundle(const double* F, double* D, const Dims& d, const FDW& w, int ord,
+hipError_t launch_deriv_bundle(const double* F, double* D, const Dims& d, const Scales& w, int ord,
                                hipStream_t st);
 hipError_t launch_algebra(const double* F, const double* D, double* OUT, long npts,
                           const Scal& sc, hipStream_t st);
-hipError_t launch_ko_add(const double* F, double* OUT, const Dims& d, const FDW& w, int ord,
+hipError_t launch_ko_add(const double* F, double* OUT, const Dims& d, const Scales& w, int ord,
                          int mode, hipStream_t st);
 
 // auxiliary
 hipError_t launch_periodic_sync(double* F, const Dims& d, hipStream_t st);
 hipError_t launch_enforce(double* F, long P, double chi_floor, double alpha_floor,
                           hipStream_t st);
-hipError_t launch_constraints(const double* F, double* HM, const Dims& d, const FDW& w, int ord,
+hipError_t launch_constraints(const double* F, double* HM, const Dims& d, const Scales& w, int ord,
                               hipStream_t st);
 hipError_t launch_sumsq(const double* HM, const Dims& d, double* partial, int nblk,
                         hipStream_t st);

@@ -40,13 +40,13 @@ struct Scal {
   double cahd;         // C_CAHD * dx_i^2 / (dt * (10 dx_i^2 + 1))
 };
 
-// Finite-difference weights per axis, already divided by h (d1), h^2 (d2) or multiplied by
-// sigma/h (KO), as the reference forms stencil = coeffs * factor before summing
-// (common/src/mcs_common/derivatives.py:79).
-struct FDW {
-  double w1[3][BSSN_MAXTAP];
-  double w2[3][BSSN_MAXTAP];
-  double wko[3][BSSN_MAXTAP];
+// Per-axis scales of the integer stencils: the centred coefficients are applied as their
+// integer numerators and the result is multiplied once by these.
+struct Scales {
+  double s1[3];     // 1 / (den1 h_a)
+  double s2[3];     // 1 / (den2 h_a^2)
+  double sm[3];     // mixed (01, 02, 12): 1 / (den1^2 h_i h_j)
+  double sko[3];    // sigma / (den_ko h_a)
 };
 
 struct Dims {
