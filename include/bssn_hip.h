@@ -112,6 +112,74 @@ long bssn_l2_partial_doubles(int Sx, int Sy, int Sz, int ng);
 int bssn_constraint_l2_f64(const double* F, double* HM, double* partial, int Sx, int Sy, int Sz,
                            int fd_order, const double* h3, int ng, double* out2, void* stream);
 
+/* ------------------------------------------------------------------------- block batches
+ * A batch is a set of blocks of different extents and spacings stored in one pool of doubles:
+ * block b is a (24, S[3b], S[3b+1], S[3b+2]) state at pool offset off[b], in the single-grid
+ * layout above, with spacings h[3b..3b+2]; all blocks share the ghost width ng. One launch
+ * evaluates every block. The plan (device memory, caller-owned, bssn_batch_plan_bytes long)
+ * holds the block table and the brick lists; the handle (host memory) holds their sizes and the
+ * launch setting. A block's results are bit-identical to the single-grid entry points run on
+ * that block alone with dx_i = dx, dy, dz set to its spacings.
+ * Replaces the slot-batched block-AMR step: fixed-shape slot arrays per level with the RHS
+ * vmapped over every slot, active or not (2D/src/mcs2d/amr/evolve.py:659-775,
+ * amr/state.py:27-31); here blocks have any extent and only existing blocks are evaluated. */
+typedef struct bssn_batch bssn_batch;
+
+/* One ghost point filled from another point of the pool, all 24 fields:
+ * pool[dst + f * dst_stride] = pool[src + f * src_stride] (strides = P of each block).
+ * Sources must be interior points. */
+typedef struct {
+  long long dst, src;
+  long long dst_stride, src_stride;
+} bssn_halo_copy;
+
+/* Plan size for nblk blocks of padded extents S (3 ints per block); 0 for invalid extents. */
+size_t bssn_batch_plan_bytes(int nblk, const int* S, int ng);
+
+/* Build the plan (uploads the block table and brick lists, synchronises the stream) and a
+ * handle to it. off: nblk pool offsets; S: 3 * nblk extents (each at most 1024); h: 3 * nblk
+ * spacings. Blocks must lie inside the pool and not overlap. */
+int bssn_batch_create(bssn_batch** out, void* plan, size_t plan_bytes, int nblk,
+                      const long long* off, const int* S, const double* h, int ng,
+                      long long pool_doubles, void* stream);
+void bssn_batch_destroy(bssn_batch* b);
+
+/* Launch shape of the batched stage kernels: 0 = one workgroup per brick (default); N > 0
+ * (a multiple of 8) = N persistent waves pulling bricks from per-XCD queues. */
+int bssn_batch_set_launch(bssn_batch* b, int persistent_waves);
+
+/* Interior (interior = 1) or padded point count of the batch; -1 for a null handle. */
+long long bssn_batch_points(const bssn_batch* b, int interior);
+
+/* RHS (+ Kreiss-Oliger if ko_sigma > 0) of every block at every padded point, each block
+ * edge-padded: bssn_rhs_f64 per block. S16's spacing entries (3, 13..15) are ignored; each
+ * block uses its own. */
+int bssn_batch_rhs_f64(const bssn_batch* b, const double* F, const double* S16, double* OUT,
+                       int fd_order, double ko_sigma, void* stream);
+
+/* Workspace of a batched step: integrator 0 = RK4 (3 pools). */
+size_t bssn_batch_step_workspace_bytes(const bssn_batch* b, int integrator);
+
+/* Each block periodic on its own interior: ghost refill of every block. */
+int bssn_batch_periodic_sync_f64(const bssn_batch* b, double* U, void* stream);
+
+/* Apply n halo copies (device array) to a pool. */
+int bssn_batch_halo_f64(double* pool, const bssn_halo_copy* map, long long n, void* stream);
+
+/* One RK4 step of every block of the pool U in place (the reference's per-stage pattern:
+ * RHS of all blocks, interior update, ghost sync; 2D/src/mcs2d/amr/evolve.py:745-771).
+ * bc = 0: blocks edge-padded (bssn_rk4_step_f64 with periodic = 0, per block);
+ * bc = 1: blocks periodic on their own interiors (bssn_rk4_step_f64 with periodic = 1);
+ * bc = 2: ghosts come from other blocks: interiors advance with stencils reading the ghosts,
+ *         and the halo map (nhalo device entries) refills the ghosts of every stage input and
+ *         of U at the end.
+ * tdev: if not null, the step time is read from device memory (*tdev, t ignored) and
+ * advanced by dt at the end, so a captured step can be replayed step after step. */
+int bssn_batch_rk4_step_f64(const bssn_batch* b, double* U, double* work, const double* S16,
+                            int fd_order, double ko_sigma, int bc, const bssn_halo_copy* halo,
+                            long long nhalo, double t, double dt, double chi_floor,
+                            double alpha_floor, double* tdev, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

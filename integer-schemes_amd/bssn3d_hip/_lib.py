@@ -18,6 +18,9 @@ ABI_VERSION = 1
 _c_int, _c_long, _c_double, _c_void_p, _c_size_t = (
     ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_void_p, ctypes.c_size_t)
 _PD = ctypes.POINTER(ctypes.c_double)
+_c_ll = ctypes.c_longlong
+_PI = ctypes.POINTER(ctypes.c_int)
+_PLL = ctypes.POINTER(ctypes.c_longlong)
 
 # name -> (restype, argtypes); mirrors include/bssn_hip.h
 SIGNATURES = {
@@ -46,6 +49,21 @@ SIGNATURES = {
     "bssn_l2_partial_doubles": (_c_long, [_c_int, _c_int, _c_int, _c_int]),
     "bssn_constraint_l2_f64": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_int, _c_int, _c_int,
                                         _c_int, _PD, _c_int, _PD, _c_void_p]),
+    # block batches
+    "bssn_batch_plan_bytes": (_c_size_t, [_c_int, _PI, _c_int]),
+    "bssn_batch_create": (_c_int, [ctypes.POINTER(_c_void_p), _c_void_p, _c_size_t, _c_int, _PLL,
+                                   _PI, _PD, _c_int, _c_ll, _c_void_p]),
+    "bssn_batch_destroy": (None, [_c_void_p]),
+    "bssn_batch_set_launch": (_c_int, [_c_void_p, _c_int]),
+    "bssn_batch_points": (_c_ll, [_c_void_p, _c_int]),
+    "bssn_batch_rhs_f64": (_c_int, [_c_void_p, _c_void_p, _PD, _c_void_p, _c_int, _c_double,
+                                    _c_void_p]),
+    "bssn_batch_step_workspace_bytes": (_c_size_t, [_c_void_p, _c_int]),
+    "bssn_batch_periodic_sync_f64": (_c_int, [_c_void_p, _c_void_p, _c_void_p]),
+    "bssn_batch_halo_f64": (_c_int, [_c_void_p, _c_void_p, _c_ll, _c_void_p]),
+    "bssn_batch_rk4_step_f64": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _PD, _c_int, _c_double,
+                                         _c_int, _c_void_p, _c_ll, _c_double, _c_double,
+                                         _c_double, _c_double, _c_void_p, _c_void_p]),
 }
 
 _lib = None

@@ -4,8 +4,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -30,24 +32,6 @@ int check_hip(hipError_t e, const char* what) {
 
 bool valid_order(int ord) { return ord == 4 || ord == 6; }
 
-// Scales of the integer stencils (bssn_brick.h): the denominators of the centred and
-// Kreiss-Oliger coefficients (common/src/mcs_common/derivatives.py:33-49) times the spacings.
-Scales make_scales(const double h3[3], int ord, double sigma) {
-  Scales s;
-  const double den1 = ord == 4 ? 12.0 : 60.0;    // first derivative
-  const double den2 = ord == 4 ? 12.0 : 180.0;   // second derivative
-  const double denk = ord == 4 ? 64.0 : 256.0;   // KO of order ord + 2
-  for (int a = 0; a < 3; ++a) {
-    s.s1[a] = 1.0 / (den1 * h3[a]);
-    s.s2[a] = 1.0 / (den2 * (h3[a] * h3[a]));
-    s.sko[a] = sigma / (denk * h3[a]);
-  }
-  s.sm[0] = 1.0 / (den1 * den1 * (h3[0] * h3[1]));
-  s.sm[1] = 1.0 / (den1 * den1 * (h3[0] * h3[2]));
-  s.sm[2] = 1.0 / (den1 * den1 * (h3[1] * h3[2]));
-  return s;
-}
-
 // S16 = eta, C_CAHD, dt, dx_i, h_ssl, sig_ssl, t, lambda0..3, lambda_f0..1, dx, dy, dz
 Scal make_scal(const double* S, double dt, double t) {
   Scal s;
@@ -57,8 +41,7 @@ Scal make_scal(const double* S, double dt, double t) {
   s.lamf[1] = S[12];
   s.ssl_h = S[4];
   s.ssl_e = std::exp(-0.5 * t * t / (S[5] * S[5]));
-  const double dx2 = S[3] * S[3];
-  s.cahd = S[1] * dx2 / (dt * (10.0 * dx2 + 1.0));
+  s.cahd = cahd_factor(S[1], S[3], dt);
   return s;
 }
 
@@ -367,6 +350,364 @@ int bssn_constraint_l2_f64(const double* F, double* HM, double* partial, int Sx,
   const double cnt = (double)d.n[0] * d.n[1] * d.n[2];
   out2[0] = std::sqrt(sh / cnt);
   out2[1] = std::sqrt(sm / cnt);
+  return BSSN_OK;
+}
+
+}  // extern "C"
+
+// ================================================================================ batches
+
+struct bssn_batch {
+  int nblk = 0, ng = 0;
+  long long pool_doubles = 0, interior_pts = 0, padded_pts = 0;
+  BlockDesc* blk = nullptr;          // device block table
+  BrickList full{}, inner{}, ghost{};
+  int grid_full = 0, grid_inner = 0, grid_ghost = 0;   // one workgroup per brick
+  int* ctr = nullptr;                // device queue counters
+  int persistent = 0;                // 0, or the persistent wave count
+};
+
+namespace {
+
+constexpr size_t kAlign = 256;
+
+size_t align_up(size_t n) { return (n + kAlign - 1) & ~(kAlign - 1); }
+
+int cdiv4(int n) { return (n + 3) / 4; }
+
+// Brick lists of one batch, built on the host. A brick of the padded block holds a ghost point
+// unless it lies inside [ng, S - ng) on every axis.
+struct HostLists {
+  std::vector<BrickItem> full, inner, ghost;
+  int cb_full[BATCH_CHUNKS + 1], cb_inner[BATCH_CHUNKS + 1], cb_ghost[BATCH_CHUNKS + 1];
+};
+
+void block_bricks(int b, const int* S, int ng, std::vector<BrickItem>* full,
+                  std::vector<BrickItem>* inner, std::vector<BrickItem>* ghost) {
+  auto pack = [](int x, int y, int z) { return x | (y << 10) | (z << 20); };
+  for (int x = 0; x < S[0]; x += 4)
+    for (int y = 0; y < S[1]; y += 4)
+      for (int z = 0; z < S[2]; z += 4) {
+        if (full) full->push_back({b, pack(x, y, z)});
+        if (ghost) {
+          const int o[3] = {x, y, z};
+          bool inside = true;
+          for (int a = 0; a < 3; ++a)
+            inside = inside && o[a] >= ng && o[a] + 4 <= S[a] - ng;
+          if (!inside) ghost->push_back({b, pack(x, y, z)});
+        }
+      }
+  if (inner)
+    for (int x = ng; x < S[0] - ng; x += 4)
+      for (int y = ng; y < S[1] - ng; y += 4)
+        for (int z = ng; z < S[2] - ng; z += 4) inner->push_back({b, pack(x, y, z)});
+}
+
+long list_count(int nblk, const int* S, int ng, int which) {
+  long n = 0;
+  for (int b = 0; b < nblk; ++b) {
+    const int* s = S + 3 * b;
+    if (which == 0) {
+      n += (long)cdiv4(s[0]) * cdiv4(s[1]) * cdiv4(s[2]);
+    } else if (which == 1) {
+      n += (long)cdiv4(s[0] - 2 * ng) * cdiv4(s[1] - 2 * ng) * cdiv4(s[2] - 2 * ng);
+    } else {
+      std::vector<BrickItem> g;
+      block_bricks(b, s, ng, nullptr, nullptr, &g);
+      n += (long)g.size();
+    }
+  }
+  return n;
+}
+
+// Deal blocks to chunks, largest first, each to the chunk with the fewest bricks so far; the
+// list is then written chunk by chunk. Returns the chunk bounds in cb.
+std::vector<BrickItem> chunked(const std::vector<std::vector<BrickItem>>& per_block, int* cb) {
+  std::vector<int> order(per_block.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return per_block[a].size() > per_block[b].size();
+  });
+  std::vector<long> load(BATCH_CHUNKS, 0);
+  std::vector<std::vector<int>> members(BATCH_CHUNKS);
+  for (int b : order) {
+    int best = 0;
+    for (int c = 1; c < BATCH_CHUNKS; ++c)
+      if (load[c] < load[best]) best = c;
+    load[best] += (long)per_block[b].size();
+    members[best].push_back(b);
+  }
+  std::vector<BrickItem> out;
+  cb[0] = 0;
+  for (int c = 0; c < BATCH_CHUNKS; ++c) {
+    std::sort(members[c].begin(), members[c].end());
+    for (int b : members[c]) out.insert(out.end(), per_block[b].begin(), per_block[b].end());
+    cb[c + 1] = (int)out.size();
+  }
+  return out;
+}
+
+int flat_grid(const int* cb) {
+  int mx = 0;
+  for (int c = 0; c < BATCH_CHUNKS; ++c) mx = std::max(mx, cb[c + 1] - cb[c]);
+  return BATCH_CHUNKS * mx;
+}
+
+int check_blocks(int nblk, const long long* off, const int* S, const double* h, int ng,
+                 long long pool_doubles) {
+  if (nblk < 1) return fail(BSSN_E_ARG, "a batch needs at least one block");
+  if (!off || !S || !h) return fail(BSSN_E_ARG, "null block table");
+  if (ng < 0) return fail(BSSN_E_ARG, "ng must be >= 0");
+  std::vector<std::pair<long long, long long>> spans;
+  for (int b = 0; b < nblk; ++b) {
+    const int* s = S + 3 * b;
+    for (int a = 0; a < 3; ++a) {
+      if (s[a] > 1024) return fail(BSSN_E_ARG, "block extents are limited to 1024 per axis");
+      if (!(h[3 * b + a] > 0.0) || !std::isfinite(h[3 * b + a]))
+        return fail(BSSN_E_ARG, "block spacings must be > 0");
+    }
+    TRY(check_shape(s[0], s[1], s[2]));
+    TRY(check_ng(s[0], s[1], s[2], ng));
+    const long long len = (long long)BSSN_NF * s[0] * s[1] * s[2];
+    if (off[b] < 0 || off[b] + len > pool_doubles)
+      return fail(BSSN_E_ARG, "block " + std::to_string(b) + " lies outside the pool");
+    spans.push_back({off[b], off[b] + len});
+  }
+  std::sort(spans.begin(), spans.end());
+  for (size_t i = 1; i < spans.size(); ++i)
+    if (spans[i].first < spans[i - 1].second) return fail(BSSN_E_ARG, "blocks overlap in the pool");
+  return BSSN_OK;
+}
+
+// S16 of a batch: its spacing entries are replaced per block, so only the others are checked
+int check_s16_batch(const double* S16) {
+  if (!S16) return fail(BSSN_E_ARG, "null scalar pack S16");
+  double s[16];
+  for (int i = 0; i < 16; ++i) s[i] = S16[i];
+  s[3] = s[13] = s[14] = s[15] = 1.0;
+  return check_s16(s);
+}
+
+int check_batch(const bssn_batch* b) {
+  if (!b || !b->blk) return fail(BSSN_E_ARG, "null or uninitialised batch");
+  return BSSN_OK;
+}
+
+BatchArgs batch_args(const bssn_batch* b, bool interior, const double* S16, double dt, double t,
+                     double sigma) {
+  BatchArgs a{};
+  a.blk = b->blk;
+  a.list = interior ? b->inner : b->full;
+  a.ctr = b->ctr;
+  a.persistent = b->persistent;
+  a.interior = interior ? 1 : 0;
+  a.base = make_scal(S16, dt, t);
+  a.cahd_c = S16[1];
+  a.dt = dt;
+  a.sigma = sigma;
+  a.tdev = nullptr;
+  a.toff = 0.0;
+  a.ssl_sig = S16[5];
+  return a;
+}
+
+int run_batch(const bssn_batch* b, const StageArgs& s, const BatchArgs& a, int ord, bool ko,
+              int mode, hipStream_t st) {
+  int grid = a.interior ? b->grid_inner : b->grid_full;
+  if (b->persistent) {
+    TRY(check_hip(launch_batch_reset(b->ctr, st), "batch queue reset"));
+    grid = b->persistent;
+  }
+  if (grid <= 0) return BSSN_OK;
+  return check_hip(launch_fused_batch(s, a, grid, ord, ko, mode, st), "batched stage launch");
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t bssn_batch_plan_bytes(int nblk, const int* S, int ng) {
+  if (nblk < 1 || !S || ng < 0) return 0;
+  for (int b = 0; b < 3 * nblk; ++b)
+    if (S[b] - 2 * ng < 1) return 0;
+  size_t n = align_up(sizeof(BlockDesc) * (size_t)nblk);
+  for (int w = 0; w < 3; ++w) n += align_up(sizeof(BrickItem) * (size_t)list_count(nblk, S, ng, w));
+  return n + align_up(sizeof(int) * BATCH_CHUNKS);
+}
+
+int bssn_batch_create(bssn_batch** out, void* plan, size_t plan_bytes, int nblk,
+                      const long long* off, const int* S, const double* h, int ng,
+                      long long pool_doubles, void* stream) {
+  if (!out || !plan) return fail(BSSN_E_ARG, "null pointer");
+  *out = nullptr;
+  TRY(check_blocks(nblk, off, S, h, ng, pool_doubles));
+  const size_t need = bssn_batch_plan_bytes(nblk, S, ng);
+  if (plan_bytes < need)
+    return fail(BSSN_E_ARG, "plan buffer too small: need " + std::to_string(need) + " bytes");
+  std::vector<BlockDesc> desc((size_t)nblk);
+  std::vector<std::vector<BrickItem>> pf((size_t)nblk), pi((size_t)nblk), pg((size_t)nblk);
+  long long interior = 0, padded = 0;
+  for (int b = 0; b < nblk; ++b) {
+    const int* s = S + 3 * b;
+    BlockDesc& d = desc[(size_t)b];
+    d.off = (long)off[b];
+    d.ng = ng;
+    for (int a = 0; a < 3; ++a) {
+      d.S[a] = s[a];
+      d.h[a] = h[3 * b + a];
+    }
+    block_bricks(b, s, ng, &pf[(size_t)b], &pi[(size_t)b], &pg[(size_t)b]);
+    interior += (long long)(s[0] - 2 * ng) * (s[1] - 2 * ng) * (s[2] - 2 * ng);
+    padded += (long long)s[0] * s[1] * s[2];
+  }
+  HostLists L;
+  L.full = chunked(pf, L.cb_full);
+  L.inner = chunked(pi, L.cb_inner);
+  L.ghost = chunked(pg, L.cb_ghost);
+
+  // plan image: block table | full | interior | ghost bricks | counters
+  std::vector<unsigned char> img(need, 0);
+  size_t pos = 0;
+  auto put = [&](const void* src, size_t bytes) {
+    std::memcpy(img.data() + pos, src, bytes);
+    const size_t at = pos;
+    pos += align_up(bytes);
+    return at;
+  };
+  const size_t o_blk = put(desc.data(), sizeof(BlockDesc) * desc.size());
+  const size_t o_full = put(L.full.data(), sizeof(BrickItem) * L.full.size());
+  const size_t o_inner = put(L.inner.data(), sizeof(BrickItem) * L.inner.size());
+  const size_t o_ghost = put(L.ghost.data(), sizeof(BrickItem) * L.ghost.size());
+  const size_t o_ctr = pos;
+  hipStream_t st = (hipStream_t)stream;
+  TRY(check_hip(hipMemcpyAsync(plan, img.data(), need, hipMemcpyHostToDevice, st), "plan upload"));
+  TRY(check_hip(hipStreamSynchronize(st), "plan upload sync"));
+
+  bssn_batch* bt = new bssn_batch();
+  unsigned char* base = (unsigned char*)plan;
+  bt->nblk = nblk;
+  bt->ng = ng;
+  bt->pool_doubles = pool_doubles;
+  bt->interior_pts = interior;
+  bt->padded_pts = padded;
+  bt->blk = (BlockDesc*)(base + o_blk);
+  bt->full.items = (const BrickItem*)(base + o_full);
+  bt->inner.items = (const BrickItem*)(base + o_inner);
+  bt->ghost.items = (const BrickItem*)(base + o_ghost);
+  for (int c = 0; c <= BATCH_CHUNKS; ++c) {
+    bt->full.cb[c] = L.cb_full[c];
+    bt->inner.cb[c] = L.cb_inner[c];
+    bt->ghost.cb[c] = L.cb_ghost[c];
+  }
+  bt->grid_full = flat_grid(L.cb_full);
+  bt->grid_inner = flat_grid(L.cb_inner);
+  bt->grid_ghost = flat_grid(L.cb_ghost);
+  bt->ctr = (int*)(base + o_ctr);
+  *out = bt;
+  return BSSN_OK;
+}
+
+void bssn_batch_destroy(bssn_batch* b) { delete b; }
+
+int bssn_batch_set_launch(bssn_batch* b, int persistent_waves) {
+  TRY(check_batch(b));
+  if (persistent_waves < 0 || persistent_waves % BATCH_CHUNKS != 0 || persistent_waves > (1 << 20))
+    return fail(BSSN_E_ARG, "persistent_waves must be 0 or a multiple of 8 (at most 2^20)");
+  b->persistent = persistent_waves;
+  return BSSN_OK;
+}
+
+long long bssn_batch_points(const bssn_batch* b, int interior) {
+  if (!b) return -1;
+  return interior ? b->interior_pts : b->padded_pts;
+}
+
+int bssn_batch_rhs_f64(const bssn_batch* b, const double* F, const double* S16, double* OUT,
+                       int fd_order, double ko_sigma, void* stream) {
+  TRY(check_batch(b));
+  if (!F || !OUT) return fail(BSSN_E_ARG, "null pool pointer");
+  if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4 or 6");
+  if (!(ko_sigma >= 0.0)) return fail(BSSN_E_ARG, "ko_sigma must be >= 0");
+  TRY(check_s16_batch(S16));
+  StageArgs s{};
+  s.Yin = F;
+  s.Y = OUT;
+  s.kind = KIND_RHS;
+  const BatchArgs a = batch_args(b, false, S16, S16[2], S16[6], ko_sigma);
+  return run_batch(b, s, a, fd_order, ko_sigma > 0.0, CLAMP, (hipStream_t)stream);
+}
+
+size_t bssn_batch_step_workspace_bytes(const bssn_batch* b, int integrator) {
+  if (!b) return 0;
+  const size_t pool = (size_t)b->pool_doubles * sizeof(double);
+  return integrator == 0 ? 3 * pool : 2 * pool;
+}
+
+int bssn_batch_periodic_sync_f64(const bssn_batch* b, double* U, void* stream) {
+  TRY(check_batch(b));
+  if (!U) return fail(BSSN_E_ARG, "null pool pointer");
+  if (b->ng == 0) return BSSN_OK;
+  return check_hip(launch_batch_periodic(U, b->blk, b->ghost, b->grid_ghost, (hipStream_t)stream),
+                   "batched periodic sync");
+}
+
+int bssn_batch_halo_f64(double* pool, const bssn_halo_copy* map, long long n, void* stream) {
+  if (!pool) return fail(BSSN_E_ARG, "null pool pointer");
+  if (n < 0 || (n > 0 && !map)) return fail(BSSN_E_ARG, "bad halo map");
+  return check_hip(launch_halo_copy(pool, map, (long)n, (hipStream_t)stream), "halo copy");
+}
+
+int bssn_batch_rk4_step_f64(const bssn_batch* b, double* U, double* work, const double* S16,
+                            int fd_order, double ko_sigma, int bc, const bssn_halo_copy* halo,
+                            long long nhalo, double t, double dt, double chi_floor,
+                            double alpha_floor, double* tdev, void* stream) {
+  TRY(check_batch(b));
+  if (!U || !work) return fail(BSSN_E_ARG, "null pointer");
+  if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4 or 6");
+  if (!(ko_sigma >= 0.0)) return fail(BSSN_E_ARG, "ko_sigma must be >= 0");
+  if (!(dt != 0.0) || !std::isfinite(dt) || !std::isfinite(t))
+    return fail(BSSN_E_ARG, "dt must be finite and nonzero, t finite");
+  if (bc < 0 || bc > 2) return fail(BSSN_E_ARG, "bc must be 0 (edge), 1 (periodic) or 2 (halo)");
+  if (bc == 2 && (nhalo < 0 || (nhalo > 0 && !halo)))
+    return fail(BSSN_E_ARG, "bc = 2 needs a halo map");
+  TRY(check_s16_batch(S16));
+  hipStream_t st = (hipStream_t)stream;
+  const size_t NS = (size_t)b->pool_doubles;
+  double* Ya = work;
+  double* Yb = work + NS;
+  double* KS = work + 2 * NS;
+  const bool interior = bc != 0;
+  const int mode = bc == 1 ? WRAP : CLAMP;
+  const bool ko = ko_sigma > 0.0;
+  const double hdt = 0.5 * dt;
+  StageArgs s{};
+  s.U = U;
+  s.K1 = KS;
+  s.Uout = U;
+  s.dt = dt;
+  s.chi_floor = chi_floor;
+  s.alpha_floor = alpha_floor;
+  auto stage = [&](double toff, int kind, double c0, const double* yin, double* y) {
+    BatchArgs a = batch_args(b, interior, S16, dt, t + toff, ko_sigma);
+    if (tdev) {
+      a.tdev = tdev;
+      a.toff = toff;
+    }
+    s.Yin = yin;
+    s.Y = y;
+    s.kind = kind;
+    s.c0 = c0;
+    TRY(run_batch(b, s, a, fd_order, ko, mode, st));
+    if (bc == 2 && y) TRY(bssn_batch_halo_f64(y, halo, nhalo, stream));
+    return BSSN_OK;
+  };
+  TRY(stage(0.0, KIND_RK1, hdt, U, Ya));
+  TRY(stage(hdt, KIND_RK2, hdt, Ya, Yb));
+  TRY(stage(hdt, KIND_RK3, dt, Yb, Ya));
+  TRY(stage(dt, KIND_RK4, dt / 6.0, Ya, nullptr));
+  if (bc == 1) TRY(bssn_batch_periodic_sync_f64(b, U, stream));
+  if (bc == 2) TRY(bssn_batch_halo_f64(U, halo, nhalo, stream));
+  if (tdev) TRY(check_hip(launch_time_advance(tdev, dt, st), "time advance"));
   return BSSN_OK;
 }
 

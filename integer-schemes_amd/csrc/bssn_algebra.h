@@ -23,7 +23,8 @@
 //   double d2(int i, int j)    second derivative, i <= j (mixed = d1_j(d1_i))
 //   double ko()                Kreiss-Oliger sum over the three axes (KO builds only)
 //   double center(int f)       field f at the point, any time
-// Out must provide: void put(int f, double rhs)   -- called exactly once per field.
+// Out must provide: void put(int f, double rhs)   -- called exactly once per field,
+//                   void metric_done()             -- after the last gt_ij and At_ij put.
 #pragma once
 
 #include "bssn_params.h"
@@ -340,6 +341,7 @@ __device__ __forceinline__ void bssn_rhs_point(Src& s, const Scal& P, Out& o) {
       o.put(AT0 + q, rA[q] + lie - (2.0 / 3.0) * A[q] * divb + al * (K * A[q] - 2.0 * AkA[q]));
       o.put(GT0 + q, rg[q] + lieg - (2.0 / 3.0) * g[q] * divb - 2.0 * al * A[q]);
     }
+    o.metric_done();
   }
 
   // ---------------------------------------------------------------- K

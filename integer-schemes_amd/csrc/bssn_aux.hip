@@ -36,7 +36,7 @@ __global__ __launch_bounds__(64) void k_deriv_bundle(const double* __restrict__ 
   if (!brick_origin<G>(d, &ox, &oy, &oz)) return;
   const int lane = (int)threadIdx.x;
   const bool valid = lane_valid<G>(d, ox, oy, oz, lane);
-  WaveSrc<ORD, false, CLAMP> s(lds, F, w, d, ox, oy, oz, lane);
+  WaveSrc<ORD, false, CLAMP, StateOrder> s(lds, F, w, d, ox, oy, oz, lane);
 #pragma unroll 1
   for (int f = 0; f < BSSN_NF; ++f) {
     s.stage(f);
@@ -91,6 +91,7 @@ struct PlainOut {
   double* OUT;
   long n, p;
   __device__ __forceinline__ void put(int f, double v) { OUT[(long)f * n + p] = v; }
+  __device__ __forceinline__ void metric_done() {}
 };
 
 __global__ __launch_bounds__(256) void k_algebra(const double* __restrict__ F,
@@ -120,7 +121,7 @@ __global__ __launch_bounds__(64) void k_ko_add(const double* __restrict__ F,
   if (!brick_origin<G>(d, &ox, &oy, &oz)) return;
   const int lane = (int)threadIdx.x;
   const bool valid = lane_valid<G>(d, ox, oy, oz, lane);
-  WaveSrc<ORD, true, MODE> s(lds, F, w, d, ox, oy, oz, lane);
+  WaveSrc<ORD, true, MODE, StateOrder> s(lds, F, w, d, ox, oy, oz, lane);
 #pragma unroll 1
   for (int f = 0; f < BSSN_NF; ++f) {
     s.stage(f);

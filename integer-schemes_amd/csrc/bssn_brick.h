@@ -70,8 +70,11 @@ struct Geo {
   static constexpr int B = 4;                             // brick edge
   static constexpr int NT = 64;                           // lanes (one wave)
   static constexpr int N = B + 2 * H;                     // box edge
-  static constexpr int SX = N * N, SY = N;                // LDS strides (z contiguous)
-  static constexpr int VOL = N * N * N;
+  // z rows are padded to a pitch of 12 doubles: for the 16 lanes of one LDS access phase
+  // (4 y-rows x 4 z of the brick) the addresses ty*12 + tz fall on 16 distinct bank pairs.
+  static constexpr int PZ = 12;
+  static constexpr int SY = PZ, SX = N * PZ;              // LDS strides (z contiguous)
+  static constexpr int VOL = N * N * PZ;
   static constexpr int NCORE = B * B * B;
   static constexpr int NFACE = 6 * B * B * H;
   static constexpr int NEDGE = 12 * B * R * R;
@@ -79,50 +82,78 @@ struct Geo {
   static constexpr int PER = (NCELL + NT - 1) / NT;       // staged cells per lane
 };
 
-// Box coordinates of staged cell s (core, then faces, then edges); s >= NCELL repeats cell 0.
+// The staged cells, ordered row by row along z so that the lanes of one load instruction
+// cover contiguous runs of a grid row: the 16 core rows (z over the whole box), the x- and
+// y-face rows (z over the core and, within reach R of the core, the z edge bars) and the xy edge
+// rows (z over the core). Entry = packed box coordinates x | y << 5 | z << 10; slots past the
+// cell count repeat the first core cell.
 template <class G>
-__device__ __forceinline__ void cell_of_slot(int s, int* c) {
-  constexpr int B = G::B, H = G::H, R = G::R;
-  if (s >= G::NCELL) s = 0;
-  if (s < G::NCORE) {
-    c[0] = H + s / (B * B);
-    c[1] = H + (s / B) % B;
-    c[2] = H + s % B;
-    return;
+struct CellTable {
+  short c[G::PER * G::NT];
+  constexpr CellTable() : c() {
+    constexpr int B = G::B, H = G::H, R = G::R, N = G::N;
+    int n = 0;
+    auto row = [&](int x, int y, int z0, int z1) {
+      for (int z = z0; z < z1; ++z) c[n++] = (short)(x | (y << 5) | (z << 10));
+    };
+    for (int x = H; x < H + B; ++x)
+      for (int y = H; y < H + B; ++y) row(x, y, 0, N);
+    for (int ax = 0; ax < 2; ++ax)
+      for (int side = 0; side < 2; ++side)
+        for (int d = 1; d <= H; ++d)
+          for (int u = H; u < H + B; ++u) {
+            const int w = side == 0 ? H - d : H + B - 1 + d;
+            const int z0 = d <= R ? H - R : H, z1 = d <= R ? H + B + R : H + B;
+            if (ax == 0) row(w, u, z0, z1);
+            else row(u, w, z0, z1);
+          }
+    for (int sx = 0; sx < 2; ++sx)
+      for (int sy = 0; sy < 2; ++sy)
+        for (int dx = 1; dx <= R; ++dx)
+          for (int dy = 1; dy <= R; ++dy)
+            row(sx == 0 ? H - dx : H + B - 1 + dx, sy == 0 ? H - dy : H + B - 1 + dy, H, H + B);
+    while (n < G::PER * G::NT) c[n++] = (short)(H | (H << 5) | (H << 10));
   }
-  s -= G::NCORE;
-  if (s < G::NFACE) {
-    const int a = s / (2 * B * B * H);
-    int r = s % (2 * B * B * H);
-    const int side = r / (B * B * H);
-    r %= B * B * H;
-    const int depth = r / (B * B);
-    const int q = r % (B * B);
-    const int u = H + q / B, v = H + q % B;
-    const int w = side == 0 ? H - 1 - depth : H + B + depth;
-    // a = axis normal to the face; (u, v) run over the other two axes in order
-    if (a == 0) { c[0] = w; c[1] = u; c[2] = v; }
-    else if (a == 1) { c[0] = u; c[1] = w; c[2] = v; }
-    else { c[0] = u; c[1] = v; c[2] = w; }
-    return;
+};
+
+template <class G>
+__constant__ CellTable<G> kCells = CellTable<G>();
+
+// Field staging orders: the fused RHS stages in the order of bssn_rhs_point's phases; the
+// per-field loops (bundle, KO) in state order.
+struct RhsOrder {
+  // gt0..5 | chi | Gt0..2 | alpha | beta0..2 | At0..5 | K | B0..2
+  __host__ __device__ static constexpr int at(int i) {
+    return i < 6 ? GT0 + i
+         : i == 6 ? CHI
+         : i < 10 ? GTI0 + (i - 7)
+         : i == 10 ? ALPHA
+         : i < 14 ? BETA0 + (i - 11)
+         : i < 20 ? AT0 + (i - 14)
+         : i == 20 ? KTR
+         : BB0 + (i - 21);
   }
-  s -= G::NFACE;
-  // 12 edge bars: axis pair p (01, 02, 12) x 4 sign combinations; the third axis is core
-  const int e = s / (B * R * R);
-  int r = s % (B * R * R);
-  const int p = e / 4, sg = e % 4;
-  const int t = H + r / (R * R);
-  r %= R * R;
-  const int di = r / R, dj = r % R;
-  const int wi = (sg & 1) ? H + B + di : H - 1 - di;
-  const int wj = (sg & 2) ? H + B + dj : H - 1 - dj;
-  if (p == 0) { c[0] = wi; c[1] = wj; c[2] = t; }
-  else if (p == 1) { c[0] = wi; c[1] = t; c[2] = wj; }
-  else { c[0] = t; c[1] = wi; c[2] = wj; }
-}
+  __host__ __device__ static constexpr int pos(int f) {
+    return f >= GT0 && f <= GT5 ? f - GT0
+         : f == CHI ? 6
+         : f >= GTI0 && f <= GTI0 + 2 ? 7 + f - GTI0
+         : f == ALPHA ? 10
+         : f >= BETA0 && f <= BETA2 ? 11 + f - BETA0
+         : f >= AT0 && f <= AT5 ? 14 + f - AT0
+         : f == KTR ? 20
+         : 21 + f - BB0;
+  }
+};
+struct StateOrder {
+  __host__ __device__ static constexpr int at(int i) { return i; }
+  __host__ __device__ static constexpr int pos(int f) { return f; }
+};
 
 // Source of fields and derivatives for bssn_rhs_point: one wave-private LDS box per field.
-template <int ORD, bool KO, int MODE>
+// Staging is pipelined one field ahead: stage(f) writes field f (loaded during the previous
+// stage) into the box, then issues the loads of the next field in ORDER, which complete while
+// field f's derivatives and algebra run.
+template <int ORD, bool KO, int MODE, class ORDER = RhsOrder>
 struct WaveSrc {
   using G = Geo<ORD, KO>;
   using C = Coef<ORD>;
@@ -132,6 +163,7 @@ struct WaveSrc {
   long P;
   int off[G::PER];      // grid offsets of this lane's staged cells
   int cel[G::PER];      // box indices of this lane's staged cells
+  double pf[G::PER];    // the next field's cells, in flight
   long gc;              // flat index of the lane's point (clamped into the grid)
   int c;                // box index of the lane's point
 
@@ -139,7 +171,7 @@ struct WaveSrc {
                                      const Dims& d, int ox, int oy, int oz, int lane)
       : lds(lds_), F(F_), sc(sc_), P(d.P) {
     const int tx = lane / (G::B * G::B), ty = (lane / G::B) % G::B, tz = lane % G::B;
-    c = ((tx + G::H) * G::N + (ty + G::H)) * G::N + (tz + G::H);
+    c = (tx + G::H) * G::SX + (ty + G::H) * G::SY + (tz + G::H);
     int p[3] = {ox + tx, oy + ty, oz + tz};
 #pragma unroll
     for (int a = 0; a < 3; ++a) p[a] = p[a] > d.S[a] - 1 ? d.S[a] - 1 : p[a];
@@ -147,9 +179,9 @@ struct WaveSrc {
     const int o[3] = {ox, oy, oz};
 #pragma unroll
     for (int k = 0; k < G::PER; ++k) {
-      int cc[3];
-      cell_of_slot<G>(lane + k * G::NT, cc);
-      cel[k] = (cc[0] * G::N + cc[1]) * G::N + cc[2];
+      const int e = kCells<G>.c[lane + k * G::NT];
+      const int cc[3] = {e & 31, (e >> 5) & 31, (e >> 10) & 31};
+      cel[k] = cc[0] * G::SX + cc[1] * G::SY + cc[2];
       int g[3];
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
@@ -166,18 +198,25 @@ struct WaveSrc {
       }
       off[k] = (g[0] * d.S[1] + g[1]) * d.S[2] + g[2];
     }
+    fetch(ORDER::at(0));
+  }
+
+  __device__ __forceinline__ void fetch(int f) {
+    const double* u = F + (long)f * P;
+#pragma unroll
+    for (int k = 0; k < G::PER; ++k) pf[k] = u[off[k]];
+    // keep the loads here, ahead of the work they overlap
+    __builtin_amdgcn_sched_barrier(0);
   }
 
   __device__ __forceinline__ void stage(int f) {
-    const double* u = F + (long)f * P;
-    double v[G::PER];
 #pragma unroll
-    for (int k = 0; k < G::PER; ++k) v[k] = u[off[k]];
-#pragma unroll
-    for (int k = 0; k < G::PER; ++k) lds[cel[k]] = v[k];
+    for (int k = 0; k < G::PER; ++k) lds[cel[k]] = pf[k];
     // The wave's LDS accesses complete in issue order; this only stops the compiler from
     // moving this field's reads above the writes.
     __builtin_amdgcn_wave_barrier();
+    const int i = ORDER::pos(f) + 1;
+    if (i < BSSN_NF) fetch(ORDER::at(i));
   }
 
   __device__ __forceinline__ double center(int f) const { return F[(long)f * P + gc]; }

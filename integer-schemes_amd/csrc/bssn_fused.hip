@@ -20,99 +20,9 @@
 #include "bssn_brick.h"
 #include "bssn_kernels.h"
 #include "bssn_params.h"
+#include "bssn_stage.h"
 
 namespace bssn {
-
-__device__ __forceinline__ void enforce_point(double* g, double* a) {
-  // det(gt) = 1 and tr(At) = 0 (3D/src/bssn3d/evolve.py:93-119)
-  const double det = g[0] * (g[3] * g[5] - g[4] * g[4]) - g[1] * (g[1] * g[5] - g[4] * g[2]) +
-                     g[2] * (g[1] * g[4] - g[3] * g[2]);
-  const double scale = pow(det, -1.0 / 3.0);
-#pragma unroll
-  for (int i = 0; i < 6; ++i) g[i] = g[i] * scale;
-  const double iuxx = g[3] * g[5] - g[4] * g[4];
-  const double iuxy = g[2] * g[4] - g[1] * g[5];
-  const double iuxz = g[1] * g[4] - g[2] * g[3];
-  const double iuyy = g[0] * g[5] - g[2] * g[2];
-  const double iuyz = g[1] * g[2] - g[0] * g[4];
-  const double iuzz = g[0] * g[3] - g[1] * g[1];
-  const double trA = iuxx * a[0] + iuyy * a[3] + iuzz * a[5] +
-                     2.0 * (iuxy * a[1] + iuxz * a[2] + iuyz * a[4]);
-  const double third = trA / 3.0;
-#pragma unroll
-  for (int i = 0; i < 6; ++i) a[i] = a[i] - g[i] * third;
-}
-
-struct StageOut {
-  const StageArgs& A;
-  long P, gp;
-  bool valid;
-  double gnew[6], anew[6];
-
-  __device__ __forceinline__ StageOut(const StageArgs& a, long P_, long gp_, bool v)
-      : A(a), P(P_), gp(gp_), valid(v) {}
-
-  __device__ __forceinline__ void final_value(int f, double u) {
-    if (f >= GT0 && f <= GT5) {
-      gnew[f - GT0] = u;
-    } else if (f >= AT0 && f <= AT5) {
-      anew[f - AT0] = u;
-    } else {
-      if (f == CHI) u = u < A.chi_floor ? A.chi_floor : u;
-      if (f == ALPHA) u = u < A.alpha_floor ? A.alpha_floor : u;
-      A.Uout[(long)f * P + gp] = u;
-    }
-  }
-
-  __device__ __forceinline__ void put(int f, double k) {
-    if (!valid) return;
-    const long i = (long)f * P + gp;
-    switch (A.kind) {
-      case KIND_RHS:
-        A.Y[i] = k;
-        break;
-      case KIND_RK1:                                  // y + (dt/2) k1 ; S = k1
-        A.Y[i] = A.U[i] + A.c0 * k;
-        A.K1[i] = k;
-        break;
-      case KIND_RK2:                                  // y + (dt/2) k2 ; S = S + 2 k2
-        A.Y[i] = A.U[i] + A.c0 * k;
-        A.K1[i] = A.K1[i] + 2.0 * k;
-        break;
-      case KIND_RK3:                                  // y + dt k3 ; S = S + 2 k3
-        A.Y[i] = A.U[i] + A.c0 * k;
-        A.K1[i] = A.K1[i] + 2.0 * k;
-        break;
-      case KIND_RK4:                                  // y + (dt/6) (S + k4)
-        final_value(f, A.U[i] + A.c0 * (A.K1[i] + k));
-        break;
-      case KIND_MS1: {                                // y + dt (a20 k0 + a21 k1) ; store k1
-        A.Y[i] = A.U[i] + A.dt * (A.c0 * A.FP[i] + A.c1 * k);
-        A.K1[i] = k;
-        break;
-      }
-      case KIND_MS2: {                                // y + dt (a30 k0 + a31 k1 + a32 k2)
-        const double k0 = A.FP[i], k1 = A.K1[i];
-        A.Y[i] = A.U[i] + A.dt * (A.c0 * k0 + A.c1 * k1 + A.c2 * k);
-        A.FP[i] = MS_B0 * k0 + MS_B1 * k1 + MS_B2 * k;
-        break;
-      }
-      case KIND_MS3:                                  // y + dt (b0 k0 + b1 k1 + b2 k2 + b3 k3)
-        final_value(f, A.U[i] + A.dt * (A.FP[i] + MS_B3 * k));
-        break;
-    }
-  }
-
-  __device__ __forceinline__ void finish() {
-    if (!valid || (A.kind != KIND_RK4 && A.kind != KIND_MS3)) return;
-    enforce_point(gnew, anew);
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-      A.Uout[(long)(GT0 + q) * P + gp] = gnew[q];
-      A.Uout[(long)(AT0 + q) * P + gp] = anew[q];
-    }
-  }
-};
 
 // Waves per SIMD the register allocator targets (build knob; 1 = the whole 512-entry
 // VGPR+AGPR file per lane).
@@ -131,7 +41,6 @@ void k_fused(StageArgs args, Dims d, Scales w, Scal sc) {
   WaveSrc<ORD, KO, MODE> src(lds, args.Yin, w, d, ox, oy, oz, lane);
   StageOut out(args, d.P, src.gc, lane_valid<G>(d, ox, oy, oz, lane));
   bssn_rhs_point<KO>(src, sc, out);
-  out.finish();
 }
 
 template <int ORD, bool KO, int MODE>

@@ -38,6 +38,52 @@ struct StageArgs {
 hipError_t launch_fused(const StageArgs& a, const Dims& d, const Scales& w, const Scal& sc, int ord,
                         bool ko, int mode, hipStream_t st);
 
+// ----------------------------------------------------------------------------- block batches
+// A batch holds many blocks of different extents and spacings in one pool: block b is a
+// (24, S0, S1, S2) state starting at pool offset off (doubles), laid out like a single grid.
+struct BlockDesc {
+  long off;       // pool offset of the block's field 0, in doubles
+  int S[3];       // padded extents
+  int ng;         // ghost width
+  double h[3];    // spacings
+};
+
+// One 4x4x4 brick of one block: the block index and the brick origin in the block's padded
+// coordinates (10 bits per axis, x | y << 10 | z << 20).
+struct BrickItem {
+  int blk;
+  int org;
+};
+
+// Brick lists are stored chunk-major, one chunk per XCD (blocks are dealt to chunks so that
+// the chunks hold nearly equal brick counts and every block's bricks stay in one chunk).
+constexpr int BATCH_CHUNKS = 8;
+
+struct BrickList {
+  const BrickItem* items;
+  int cb[BATCH_CHUNKS + 1];   // chunk c = items[cb[c] .. cb[c+1])
+};
+
+struct BatchArgs {
+  const BlockDesc* blk;
+  BrickList list;
+  int* ctr;          // BATCH_CHUNKS queue counters (zeroed before each persistent launch)
+  int persistent;    // 0: one workgroup per brick; else waves pull bricks from the queues
+  int interior;      // the list covers block interiors (else whole padded blocks)
+  Scal base;         // shared scalars; cahd is set per block from its spacing
+  double cahd_c, dt, sigma;
+  const double* tdev;   // if set, the stage time is *tdev + toff and ssl_e is formed on the device
+  double toff, ssl_sig;
+};
+
+hipError_t launch_fused_batch(const StageArgs& a, const BatchArgs& b, int grid, int ord, bool ko,
+                              int mode, hipStream_t st);
+hipError_t launch_batch_reset(int* ctr, hipStream_t st);
+hipError_t launch_batch_periodic(double* pool, const BlockDesc* blk, const BrickList& ghost,
+                                 int grid, hipStream_t st);
+hipError_t launch_halo_copy(double* pool, const void* map, long n, hipStream_t st);
+hipError_t launch_time_advance(double* tdev, double dt, hipStream_t st);
+
 // two-pass (materialised derivative bundle) path
 hipError_t launch_deriv_bundle(const double* F, double* D, const Dims& d, const Scales& w, int ord,
                                hipStream_t st);

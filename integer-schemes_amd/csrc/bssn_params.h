@@ -58,6 +58,35 @@ struct Dims {
 };
 
 #if defined(__HIPCC__)
+// Scales of the integer stencils: the denominators of the centred and Kreiss-Oliger
+// coefficients (common/src/mcs_common/derivatives.py:33-49) times the spacings. Computed on
+// the host for one grid and on the device per block of a batch; contraction is off so that
+// both round identically (division and multiplication are correctly rounded on both).
+__host__ __device__ inline Scales make_scales(const double h3[3], int ord, double sigma) {
+#pragma clang fp contract(off)
+  Scales s;
+  const double den1 = ord == 4 ? 12.0 : 60.0;    // first derivative
+  const double den2 = ord == 4 ? 12.0 : 180.0;   // second derivative
+  const double denk = ord == 4 ? 64.0 : 256.0;   // KO of order ord + 2
+  for (int a = 0; a < 3; ++a) {
+    s.s1[a] = 1.0 / (den1 * h3[a]);
+    s.s2[a] = 1.0 / (den2 * (h3[a] * h3[a]));
+    s.sko[a] = sigma / (denk * h3[a]);
+  }
+  s.sm[0] = 1.0 / (den1 * den1 * (h3[0] * h3[1]));
+  s.sm[1] = 1.0 / (den1 * den1 * (h3[0] * h3[2]));
+  s.sm[2] = 1.0 / (den1 * den1 * (h3[1] * h3[2]));
+  return s;
+}
+
+// CAHD factor C dx_i^2 / (dt (10 dx_i^2 + 1)) (vendor/bssneqs_SSL_HD_dxsq.cpp:857), same
+// rounding on host and device.
+__host__ __device__ inline double cahd_factor(double C, double dx, double dt) {
+#pragma clang fp contract(off)
+  const double dx2 = dx * dx;
+  return C * dx2 / (dt * (10.0 * dx2 + 1.0));
+}
+
 // Keep a value where it is computed: an empty volatile asm that claims to rewrite x. The
 // compiler must evaluate x before this point and may not sink its arithmetic past later
 // barriers, which otherwise keeps raw stencil taps live across field stages.

@@ -23,7 +23,8 @@ HEADER = ROOT / "include" / "bssn_hip.h"
 
 def declared_symbols():
     text = HEADER.read_text()
-    return sorted(set(re.findall(r"^(?:int|long|size_t|const char\*)\s+(bssn_\w+)\(", text, re.M)))
+    return sorted(set(re.findall(r"^(?:int|long|long long|size_t|void|const char\*)\s+(bssn_\w+)\(",
+                                 text, re.M)))
 
 
 def test_header_and_binding_agree():
