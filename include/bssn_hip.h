@@ -55,6 +55,16 @@ int bssn_rhs_f64(const double* F, const double* S16, double* OUT, int Sx, int Sy
 int bssn_deriv_bundle_f64(const double* F, double* D, int Sx, int Sy, int Sz, int fd_order,
                           const double* h3, void* stream);
 
+/* Precision form of pass 1: the same bundle from int8 digits on the INT8 matrix cores
+ * (Ozaki scheme I: per 28 x 28 plane tile an offset and a power-of-two scale, `slices` int8
+ * digits of 7 bits, banded integer stencils x12 / x144 contracted exactly in INT32 by
+ * v_mfma_i32_32x32x32_i8, recombined in FP64). fd_order 4 only; slices 1..7.
+ * Replaces the reference's precision-form derivative operator OzakiDerivative
+ * (2D/src/mcs2d/schemes/ozaki.py:257-562, 3D/src/mcs3d/schemes/ozaki.py:162-282; there an
+ * RNS/CRT form for the MCS solver) applied to the BSSN bundle (derivative_bundle.py:31-48). */
+int bssn_deriv_bundle_i8_f64(const double* F, double* D, int Sx, int Sy, int Sz, int fd_order,
+                             const double* h3, int slices, void* stream);
+
 /* Two-pass RHS, pass 2: the pointwise algebra over npts points of F (24 x npts) and
  * D (138 x npts). Replaces bssn_rhs_algebra (3D/src/bssn3d/_bssn_rhs_generated.py) and the
  * algebra-only kernel bssn_algebra (3D/src/bssn3d/cuda/gen_algebra_cuda.py:55-96). */

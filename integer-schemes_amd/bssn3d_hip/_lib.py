@@ -30,6 +30,8 @@ SIGNATURES = {
                               _c_double, _c_void_p]),
     "bssn_deriv_bundle_f64": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, _c_int, _c_int,
                                        _PD, _c_void_p]),
+    "bssn_deriv_bundle_i8_f64": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, _c_int, _c_int,
+                                          _PD, _c_int, _c_void_p]),
     "bssn_algebra_f64": (_c_int, [_c_void_p, _c_void_p, _PD, _c_void_p, _c_long, _c_void_p]),
     "bssn_ko_add_f64": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, _c_int, _c_int,
                                  _c_double, _PD, _c_int, _c_int, _c_void_p]),

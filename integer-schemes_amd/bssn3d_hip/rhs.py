@@ -13,6 +13,8 @@ Schemes:
   "fused"   one fused kernel per evaluation (derivatives in LDS/registers, no bundle in HBM)
   "twopass" the materialised path: a 138-array derivative bundle, then the pointwise algebra
             (the data flow of the reference's "verbatim" scheme)
+  "int8"    the two-pass path with the bundle taken from int8 digits on the INT8 matrix cores
+            (Ozaki scheme I, 7 digits by default; FD order 4); the algebra stays FP64
 """
 from __future__ import annotations
 
@@ -24,7 +26,7 @@ from . import _lib
 from .grid import Grid, ghost_width
 from .state import BSSNState, PhysicsParams, VAR_NAMES, pack_S, scalars
 
-SCHEMES = ("fused", "twopass")
+SCHEMES = ("fused", "twopass", "int8")
 
 
 def as_tensor(state) -> torch.Tensor:
@@ -73,6 +75,21 @@ def rhs_twopass_into(F: torch.Tensor, OUT: torch.Tensor, D: torch.Tensor, S16, o
                                        h3, 0, 0, st))
 
 
+def rhs_int8_into(F: torch.Tensor, OUT: torch.Tensor, D: torch.Tensor, S16, order: int = 4,
+                  ko_sigma: float = 0.0, slices: int = 7) -> None:
+    """OUT <- algebra(F, INT8-split bundle(F)) (+ FP64 KO), bundle materialised in D."""
+    _, Sx, Sy, Sz = F.shape
+    lib = _lib.load()
+    st = stream_handle(F.device)
+    h3 = _lib.dbl(S16[13:16])
+    _lib.check(lib.bssn_deriv_bundle_i8_f64(_ptr(F), _ptr(D), Sx, Sy, Sz, order, h3, int(slices),
+                                            st))
+    _lib.check(lib.bssn_algebra_f64(_ptr(F), _ptr(D), _lib.dbl(S16), _ptr(OUT), Sx * Sy * Sz, st))
+    if ko_sigma > 0.0:
+        _lib.check(lib.bssn_ko_add_f64(_ptr(F), _ptr(OUT), Sx, Sy, Sz, order, float(ko_sigma),
+                                       h3, 0, 0, st))
+
+
 def cuda_fused_algebra(F, dx, dy, dz, eta, lmbda, lambda_f, cahd_c, dt, ssl_h, ssl_sigma, t,
                        order: int = 6) -> Dict[str, torch.Tensor]:
     """The reference's fused-scheme callable (fused_rhs_cuda.py:62-72): a dict of 24 fields in,
@@ -97,9 +114,10 @@ def device_rhs_fused(state, params: PhysicsParams, dx, dy, dz, dt, t=0.0,
 class BSSNSolver:
     """Grid + params + FD order -> RHS evaluator (3D/src/bssn3d/rhs.py:63-139).
 
-    ``scheme`` is "fused" (default) or "twopass"; the reference's scheme names map as
+    ``scheme`` is "fused" (default), "twopass" or "int8"; the reference's scheme names map as
     cuda_fused/fused/fused_fp64/fused_tiled -> "fused" and verbatim/staged/pallas -> "twopass".
-    ``precision`` must be "fp64".
+    ``precision`` must be "fp64" (the "int8" scheme computes its derivatives from int8 digits
+    and everything else in fp64; FD order 4).
     """
 
     _ALIASES = {"cuda_fused": "fused", "fused_fp64": "fused", "fused_tiled": "fused",
@@ -117,6 +135,8 @@ class BSSNSolver:
             raise ValueError(f"unknown precision {precision!r} (this library computes in fp64)")
         if order not in (4, 6):
             raise ValueError(f"order {order} not supported on the device (4 or 6)")
+        if scheme == "int8" and order != 4:
+            raise ValueError("the int8 scheme supports FD order 4 only")
         ko_order = order + 2 if ko_order is None else ko_order
         if ko_order != order + 2:
             raise ValueError(f"KO order must be order + 2 = {order + 2} on the device")
@@ -148,7 +168,10 @@ class BSSNSolver:
             if self._D is None or self._D.device != F.device:
                 self._D = torch.empty((138,) + tuple(F.shape[1:]), dtype=torch.float64,
                                       device=F.device)
-            rhs_twopass_into(F, out, self._D, S, self.order)
+            if self.scheme == "int8":
+                rhs_int8_into(F, out, self._D, S, self.order)
+            else:
+                rhs_twopass_into(F, out, self._D, S, self.order)
         return out
 
     def rhs_dict(self, state, t: float = 0.0, dt: float | None = None) -> Dict[str, torch.Tensor]:

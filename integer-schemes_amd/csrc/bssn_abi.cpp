@@ -170,6 +170,21 @@ int bssn_deriv_bundle_f64(const double* F, double* D, int Sx, int Sy, int Sz, in
                    "deriv bundle launch");
 }
 
+int bssn_deriv_bundle_i8_f64(const double* F, double* D, int Sx, int Sy, int Sz, int fd_order,
+                             const double* h3, int slices, void* stream) {
+  if (!F || !D) return fail(BSSN_E_ARG, "null pointer");
+  if (fd_order != 4)
+    return fail(BSSN_E_ARG, "the INT8-split bundle supports fd_order 4 only (order-6 second "
+                            "derivative numerators exceed int8)");
+  if (slices < 1 || slices > 7) return fail(BSSN_E_ARG, "slices must be 1..7");
+  TRY(check_shape(Sx, Sy, Sz));
+  TRY(check_h(h3));
+  const Scales w = make_scales(h3, 4, 0.0);
+  return check_hip(launch_deriv_bundle_i8(F, D, full_dims(Sx, Sy, Sz, 0), w, slices,
+                                          (hipStream_t)stream),
+                   "INT8 bundle launch");
+}
+
 int bssn_algebra_f64(const double* F, const double* D, const double* S16, double* OUT, long npts,
                      void* stream) {
   if (!F || !D || !OUT) return fail(BSSN_E_ARG, "null pointer");

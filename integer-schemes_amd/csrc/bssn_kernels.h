@@ -89,6 +89,9 @@ hipError_t launch_deriv_bundle(const double* F, double* D, const Dims& d, const 
                                hipStream_t st);
 hipError_t launch_algebra(const double* F, const double* D, double* OUT, long npts,
                           const Scal& sc, hipStream_t st);
+// INT8-split (Ozaki scheme I) derivative bundle, FD order 4, nslice int8 digits
+hipError_t launch_deriv_bundle_i8(const double* F, double* D, const Dims& d, const Scales& w,
+                                  int nslice, hipStream_t st);
 hipError_t launch_ko_add(const double* F, double* OUT, const Dims& d, const Scales& w, int ord,
                          int mode, hipStream_t st);
 
