@@ -1,6 +1,7 @@
 """Benchmark: the fused BSSN RHS + RK4 step at 256^3 FP64 on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--no-cpu] [--no-twopass]
+                    [--no-int8] [--no-batch]
 
 A step is one full RK4 step (four fused stage kernels: RHS + Kreiss-Oliger + stage update,
 enforcement folded into the last; then the periodic ghost refill) of the 256^3-interior,
@@ -11,6 +12,15 @@ so no input is reused from cache. ``value`` is interior points per second of the
 arrays through HBM, then the algebra) on the same state, the roofline position of the fused
 stage kernel (timed live with HIP events on the launch stream), and the CPU oracle timed on
 this host's cores on a bounded 48^3 sample.
+
+Secondary forms (extra keys, same metric family):
+  int8_split  the RHS with its 138 derivatives from 7 int8 digits on the INT8 matrix cores
+              (Ozaki scheme I) and the FP64 algebra, on the single-puncture construction at
+              256^3 (h = 12/256), with its normwise error against the FP64 fused RHS;
+  batched     640 cubic blocks of interior 8..32 on three levels (h = 1/4, 1/8, 1/16) with
+              masses 0.909 / 0.0909 4M apart: the RHS of all blocks in one launch and the RK4
+              step of all blocks (each periodic on itself), launch shape autotuned, eager and
+              replayed from a captured graph; Mpts/s over interior points.
 
 Rank 0 prints one JSON line.
 """
@@ -74,6 +84,104 @@ def cpu_baseline(seconds: float = 12.0):
                       f"FD4/KO6, oracle/bssn_oracle.c, {el:.1f} s"}
 
 
+def _events():
+    import torch
+    return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+
+def bench_int8(N: int, reps: int) -> dict:
+    """INT8-split RHS (bundle from 7 int8 digits + FP64 algebra) vs the fused FP64 RHS."""
+    import numpy as np
+    import torch
+    from bssn3d_hip import _lib
+    from bssn3d_hip import initial_data as idata
+    from bssn3d_hip.rhs import rhs_int8_into, rhs_into, stream_handle
+    from bssn3d_hip.state import PhysicsParams, scalars
+
+    dev = torch.device("cuda:0")
+    grid, U, _ = idata.config_single_puncture(seed=0, N=N, device=dev)
+    S16 = scalars(PhysicsParams(), grid.dx, grid.dy, grid.dz, 0.25 * grid.dx, 0.0)
+    Sx, Sy, Sz = grid.shape
+    out8 = torch.empty_like(U)
+    ref = torch.empty_like(U)
+    D = torch.empty((138, Sx, Sy, Sz), dtype=torch.float64, device=dev)
+    rhs_into(U, ref, S16, 4)
+    rhs_int8_into(U, out8, D, S16, 4)
+    torch.cuda.synchronize()
+    a, b = _events()
+    a.record()
+    for _ in range(reps):
+        rhs_int8_into(U, out8, D, S16, 4)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / reps
+    h3 = _lib.dbl([grid.dx, grid.dy, grid.dz])
+    lib = _lib.load()
+    a.record()
+    for _ in range(reps):
+        _lib.check(lib.bssn_deriv_bundle_i8_f64(U.data_ptr(), D.data_ptr(), Sx, Sy, Sz, 4, h3, 7,
+                                                stream_handle(dev)))
+    b.record()
+    torch.cuda.synchronize()
+    ms_bundle = a.elapsed_time(b) / reps
+    err = []
+    for f in range(24):
+        den = max(float(ref[f].abs().max()), 1e-12)
+        err.append(float((out8[f] - ref[f]).abs().max()) / den)
+    npts = N ** 3
+    del D, out8, ref, U
+    return {"workload": f"single_puncture_{N}^3_h{12.0 / N:g}", "slices": 7,
+            "rhs_ms": round(ms, 4), "rhs_mpts_s": round(npts / (ms * 1e-3) / 1e6, 2),
+            "bundle_ms": round(ms_bundle, 4),
+            "bundle_hbm_frac": round(138 * 8 * Sx * Sy * Sz / (ms_bundle * 1e-3) / PEAK_HBM, 4),
+            "max_normwise_err_vs_fp64_rhs": float("%.3e" % max(err)),
+            "median_normwise_err_vs_fp64_rhs": float("%.3e" % float(np.median(err)))}
+
+
+def bench_batched(reps: int) -> dict:
+    """640 uneven blocks: batched RHS and RK4 step (blocks periodic on themselves)."""
+    import torch
+    from bssn3d_hip.batch import BlockBatch, config_blocks, fill_punctures, timed
+    from bssn3d_hip.state import PhysicsParams
+
+    dev = torch.device("cuda:0")
+    blocks = config_blocks(seed=0)
+    batch = BlockBatch(blocks, ng=3, device=dev)
+    U = batch.empty_pool()
+    fill_punctures(batch, U, seed=0)
+    p = PhysicsParams()
+    dt = 0.25 * 0.0625
+    tune = batch.autotune(U, dt, p, candidates=[0, 1024, 2048, 4096], reps=2)
+    OUT = batch.empty_pool()
+    batch.rhs_into(U, OUT, p, dt, 0.0, 4, 0.0)
+    torch.cuda.synchronize()
+    ms_rhs = timed(lambda: batch.rhs_into(U, OUT, p, dt, 0.0, 4, 0.0), reps, dev)
+    del OUT
+    t = [0.0]
+
+    def step():
+        batch.step_(U, t[0], dt, p, 4, 0.1, "periodic")
+        t[0] += dt
+    step()
+    torch.cuda.synchronize()
+    ms_step = timed(step, reps, dev)
+    tdev = torch.full((1,), t[0], dtype=torch.float64, device=dev)
+    g = batch.capture_steps(U, tdev, dt, nsteps=1, params=p)
+    g.replay()
+    torch.cuda.synchronize()
+    ms_graph = timed(g.replay, reps, dev)
+    assert bool(torch.isfinite(U).all()), "non-finite batched state"
+    pts = batch.interior_points
+    return {"workload": "640_blocks_8-32_levels_1/4_1/8_1/16_masses_0.909_0.0909_sep4",
+            "blocks": len(blocks), "interior_points": pts, "padded_points": batch.padded_points,
+            "launch_persistent_waves": batch.persistent_waves,
+            "autotune_step_ms": {str(k): round(v, 4) for k, v in tune.items()},
+            "rhs_ms": round(ms_rhs, 4), "rhs_mpts_s": round(pts / (ms_rhs * 1e-3) / 1e6, 2),
+            "step_ms": round(ms_step, 4), "step_mpts_s": round(pts / (ms_step * 1e-3) / 1e6, 2),
+            "graph_step_ms": round(ms_graph, 4),
+            "graph_step_mpts_s": round(pts / (ms_graph * 1e-3) / 1e6, 2)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +190,8 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-twopass", action="store_true")
+    ap.add_argument("--no-int8", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="also run one plain device copy of this many GiB (profiler calibration)")
@@ -226,6 +336,12 @@ def main() -> int:
         res["fused_vs_twopass_rhs"] = round(ms_twopass / ms_rhs, 3)
     del out, U
     torch.cuda.empty_cache()
+    if not args.no_int8:
+        res["int8_split"] = bench_int8(N, reps)
+        torch.cuda.empty_cache()
+    if not args.no_batch:
+        res["batched"] = bench_batched(reps)
+        torch.cuda.empty_cache()
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     else:

@@ -21,8 +21,9 @@ CSRC = PKG / "csrc"
 OUT = PKG / "bssn3d_hip" / "libbssn_hip.so"
 ARCH = os.environ.get("BSSN_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["bssn_fused.hip", "bssn_batch.hip", "bssn_aux.hip", "bssn_i8.hip",
-           "bssn_abi.cpp"]
+# largest compile units first (the fused stage kernels are instantiated per kind)
+SOURCES = ["bssn_fused4.hip", "bssn_fused6.hip", "bssn_batch4.hip", "bssn_batch6.hip",
+           "bssn_aux.hip", "bssn_batch.hip", "bssn_i8.hip", "bssn_abi.cpp"]
 
 # Packed-FP32 instructions are disabled for every HIP compile (they are not used by this
 # FP64 library; the flag keeps hipcc from forming them from any float code).

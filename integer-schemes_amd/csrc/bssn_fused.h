@@ -12,6 +12,8 @@
 //   RK4 stages  : BSSNEvolution.step (3D/src/bssn3d/evolve.py:125-132) with rhs() =
 //                 periodic sync + RHS + KO (evolve.py:86-90) and enforce (evolve.py:93-119)
 //   MSRK stages : BSSNEvolution._msrk2_step (evolve.py:139-153)
+#pragma once
+
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -30,7 +32,7 @@ namespace bssn {
 #define BSSN_FUSED_WAVES 2
 #endif
 
-template <int ORD, bool KO, int MODE>
+template <int ORD, bool KO, int MODE, int KIND>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BSSN_FUSED_WAVES, BSSN_FUSED_WAVES)))
 void k_fused(StageArgs args, Dims d, Scales w, Scal sc) {
   using G = Geo<ORD, KO>;
@@ -39,31 +41,30 @@ void k_fused(StageArgs args, Dims d, Scales w, Scal sc) {
   if (!brick_origin<G>(d, &ox, &oy, &oz)) return;
   const int lane = (int)threadIdx.x;
   WaveSrc<ORD, KO, MODE> src(lds, args.Yin, w, d, ox, oy, oz, lane);
-  StageOut out(args, d.P, src.gc, lane_valid<G>(d, ox, oy, oz, lane));
+  StageOut<KIND> out(args, d.P, src.gc, lane_valid<G>(d, ox, oy, oz, lane));
   bssn_rhs_point<KO>(src, sc, out);
 }
 
-template <int ORD, bool KO, int MODE>
+template <int ORD, bool KO, int MODE, int KIND>
 static hipError_t launch_fused_t(const StageArgs& a, const Dims& d, const Scales& w,
                                  const Scal& sc, hipStream_t st) {
   using G = Geo<ORD, KO>;
-  hipLaunchKernelGGL((k_fused<ORD, KO, MODE>), dim3(brick_blocks<G>(d)), dim3(G::NT), 0, st, a,
-                     d, w, sc);
+  hipLaunchKernelGGL((k_fused<ORD, KO, MODE, KIND>), dim3(brick_blocks<G>(d)), dim3(G::NT), 0,
+                     st, a, d, w, sc);
   return hipGetLastError();
 }
 
-hipError_t launch_fused(const StageArgs& a, const Dims& d, const Scales& w, const Scal& sc,
-                        int ord, bool ko, int mode, hipStream_t st) {
-  if (ord == 4) {
-    if (ko) return mode == WRAP ? launch_fused_t<4, true, WRAP>(a, d, w, sc, st)
-                                : launch_fused_t<4, true, CLAMP>(a, d, w, sc, st);
-    return mode == WRAP ? launch_fused_t<4, false, WRAP>(a, d, w, sc, st)
-                        : launch_fused_t<4, false, CLAMP>(a, d, w, sc, st);
-  }
-  if (ko) return mode == WRAP ? launch_fused_t<6, true, WRAP>(a, d, w, sc, st)
-                              : launch_fused_t<6, true, CLAMP>(a, d, w, sc, st);
-  return mode == WRAP ? launch_fused_t<6, false, WRAP>(a, d, w, sc, st)
-                      : launch_fused_t<6, false, CLAMP>(a, d, w, sc, st);
+// every (KO, MODE, kind) instance of one FD order
+template <int ORD>
+static hipError_t launch_fused_ord(const StageArgs& a, const Dims& d, const Scales& w,
+                                   const Scal& sc, bool ko, int mode, hipStream_t st) {
+  return with_kind<true>(a.kind, [&](auto k) -> hipError_t {
+    constexpr int K = decltype(k)::value;
+    if (ko) return mode == WRAP ? launch_fused_t<ORD, true, WRAP, K>(a, d, w, sc, st)
+                                : launch_fused_t<ORD, true, CLAMP, K>(a, d, w, sc, st);
+    return mode == WRAP ? launch_fused_t<ORD, false, WRAP, K>(a, d, w, sc, st)
+                        : launch_fused_t<ORD, false, CLAMP, K>(a, d, w, sc, st);
+  });
 }
 
 }  // namespace bssn

@@ -37,6 +37,10 @@ struct StageArgs {
 
 hipError_t launch_fused(const StageArgs& a, const Dims& d, const Scales& w, const Scal& sc, int ord,
                         bool ko, int mode, hipStream_t st);
+hipError_t launch_fused_o4(const StageArgs& a, const Dims& d, const Scales& w, const Scal& sc,
+                           bool ko, int mode, hipStream_t st);
+hipError_t launch_fused_o6(const StageArgs& a, const Dims& d, const Scales& w, const Scal& sc,
+                           bool ko, int mode, hipStream_t st);
 
 // ----------------------------------------------------------------------------- block batches
 // A batch holds many blocks of different extents and spacings in one pool: block b is a
@@ -78,6 +82,10 @@ struct BatchArgs {
 
 hipError_t launch_fused_batch(const StageArgs& a, const BatchArgs& b, int grid, int ord, bool ko,
                               int mode, hipStream_t st);
+hipError_t launch_batch_o4(const StageArgs& a, const BatchArgs& b, int grid, bool ko, int mode,
+                           hipStream_t st);
+hipError_t launch_batch_o6(const StageArgs& a, const BatchArgs& b, int grid, bool ko, int mode,
+                           hipStream_t st);
 hipError_t launch_batch_reset(int* ctr, hipStream_t st);
 hipError_t launch_batch_periodic(double* pool, const BlockDesc* blk, const BrickList& ghost,
                                  int grid, hipStream_t st);

@@ -9,6 +9,8 @@
 
 #include <math.h>
 
+#include <type_traits>
+
 #include "bssn_kernels.h"
 #include "bssn_params.h"
 
@@ -34,6 +36,10 @@ __device__ __forceinline__ void enforce_point(double* g, double* a) {
   for (int i = 0; i < 6; ++i) a[i] = a[i] - g[i] * third;
 }
 
+// The write-back kind is a template parameter: each launch runs the code of its own kind only
+// (a runtime switch over all kinds made the stage kernel half again as large).
+// RK2 and RK3 share their code (c0 differs); KIND_RK3 launches StageOut<KIND_RK2>.
+template <int KIND>
 struct StageOut {
   const StageArgs& A;
   long P, gp;
@@ -58,52 +64,64 @@ struct StageOut {
   __device__ __forceinline__ void put(int f, double k) {
     if (!valid) return;
     const long i = (long)f * P + gp;
-    switch (A.kind) {
-      case KIND_RHS:
-        A.Y[i] = k;
-        break;
-      case KIND_RK1:                                  // y + (dt/2) k1 ; S = k1
-        A.Y[i] = A.U[i] + A.c0 * k;
-        A.K1[i] = k;
-        break;
-      case KIND_RK2:                                  // y + (dt/2) k2 ; S = S + 2 k2
-        A.Y[i] = A.U[i] + A.c0 * k;
-        A.K1[i] = A.K1[i] + 2.0 * k;
-        break;
-      case KIND_RK3:                                  // y + dt k3 ; S = S + 2 k3
-        A.Y[i] = A.U[i] + A.c0 * k;
-        A.K1[i] = A.K1[i] + 2.0 * k;
-        break;
-      case KIND_RK4:                                  // y + (dt/6) (S + k4)
-        final_value(f, A.U[i] + A.c0 * (A.K1[i] + k));
-        break;
-      case KIND_MS1: {                                // y + dt (a20 k0 + a21 k1) ; store k1
-        A.Y[i] = A.U[i] + A.dt * (A.c0 * A.FP[i] + A.c1 * k);
-        A.K1[i] = k;
-        break;
-      }
-      case KIND_MS2: {                                // y + dt (a30 k0 + a31 k1 + a32 k2)
-        const double k0 = A.FP[i], k1 = A.K1[i];
-        A.Y[i] = A.U[i] + A.dt * (A.c0 * k0 + A.c1 * k1 + A.c2 * k);
-        A.FP[i] = MS_B0 * k0 + MS_B1 * k1 + MS_B2 * k;
-        break;
-      }
-      case KIND_MS3:                                  // y + dt (b0 k0 + b1 k1 + b2 k2 + b3 k3)
-        final_value(f, A.U[i] + A.dt * (A.FP[i] + MS_B3 * k));
-        break;
+    if constexpr (KIND == KIND_RHS) {
+      A.Y[i] = k;
+    } else if constexpr (KIND == KIND_RK1) {            // y + (dt/2) k1 ; S = k1
+      A.Y[i] = A.U[i] + A.c0 * k;
+      A.K1[i] = k;
+    } else if constexpr (KIND == KIND_RK2) {            // y + c0 k ; S = S + 2 k  (RK2, RK3)
+      A.Y[i] = A.U[i] + A.c0 * k;
+      A.K1[i] = A.K1[i] + 2.0 * k;
+    } else if constexpr (KIND == KIND_RK4) {            // y + (dt/6) (S + k4)
+      final_value(f, A.U[i] + A.c0 * (A.K1[i] + k));
+    } else if constexpr (KIND == KIND_MS1) {            // y + dt (a20 k0 + a21 k1) ; store k1
+      A.Y[i] = A.U[i] + A.dt * (A.c0 * A.FP[i] + A.c1 * k);
+      A.K1[i] = k;
+    } else if constexpr (KIND == KIND_MS2) {            // y + dt (a30 k0 + a31 k1 + a32 k2)
+      const double k0 = A.FP[i], k1 = A.K1[i];
+      A.Y[i] = A.U[i] + A.dt * (A.c0 * k0 + A.c1 * k1 + A.c2 * k);
+      A.FP[i] = MS_B0 * k0 + MS_B1 * k1 + MS_B2 * k;
+    } else {                                            // MS3: y + dt (b0 k0 + ... + b3 k3)
+      static_assert(KIND == KIND_MS3, "unknown stage kind");
+      final_value(f, A.U[i] + A.dt * (A.FP[i] + MS_B3 * k));
     }
   }
 
   // all gt_ij and At_ij of a final stage are known: enforce det = 1, tr = 0 and store them
   __device__ __forceinline__ void metric_done() {
-    if (!valid || (A.kind != KIND_RK4 && A.kind != KIND_MS3)) return;
-    enforce_point(gnew, anew);
+    if constexpr (KIND == KIND_RK4 || KIND == KIND_MS3) {
+      if (!valid) return;
+      enforce_point(gnew, anew);
 #pragma unroll
-    for (int q = 0; q < 6; ++q) {
-      A.Uout[(long)(GT0 + q) * P + gp] = gnew[q];
-      A.Uout[(long)(AT0 + q) * P + gp] = anew[q];
+      for (int q = 0; q < 6; ++q) {
+        A.Uout[(long)(GT0 + q) * P + gp] = gnew[q];
+        A.Uout[(long)(AT0 + q) * P + gp] = anew[q];
+      }
     }
   }
 };
+
+// Call fn(std::integral_constant<int, K>) with K the template kind of the runtime stage kind
+// (MSRK kinds only if MSRK is set).
+template <bool MSRK, class Fn>
+__host__ inline hipError_t with_kind(int kind, Fn&& fn) {
+  switch (kind) {
+    case KIND_RHS: return fn(std::integral_constant<int, KIND_RHS>());
+    case KIND_RK1: return fn(std::integral_constant<int, KIND_RK1>());
+    case KIND_RK2:
+    case KIND_RK3: return fn(std::integral_constant<int, KIND_RK2>());
+    case KIND_RK4: return fn(std::integral_constant<int, KIND_RK4>());
+    default: break;
+  }
+  if constexpr (MSRK) {
+    switch (kind) {
+      case KIND_MS1: return fn(std::integral_constant<int, KIND_MS1>());
+      case KIND_MS2: return fn(std::integral_constant<int, KIND_MS2>());
+      case KIND_MS3: return fn(std::integral_constant<int, KIND_MS3>());
+      default: break;
+    }
+  }
+  return hipErrorInvalidValue;
+}
 
 }  // namespace bssn
