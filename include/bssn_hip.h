@@ -97,6 +97,16 @@ int bssn_rk4_step_f64(double* U, double* work, int Sx, int Sy, int Sz, const dou
                       int fd_order, double ko_sigma, int ng, int periodic, double t, double dt,
                       double chi_floor, double alpha_floor, void* stream);
 
+/* bssn_rk4_step_f64 with the step time in device memory: each stage reads *tdev (plus its
+ * offset) and forms the SSL factor on the device, and *tdev is advanced by dt at the end, so a
+ * HIP graph that captured K steps advances the state and the time by K steps per replay.
+ * The analogue of the reference's compiled stepping loop with traced t0
+ * (BSSNEvolution.evolve, 3D/src/bssn3d/evolve.py:155-194). */
+int bssn_rk4_step_devtime_f64(double* U, double* work, int Sx, int Sy, int Sz,
+                              const double* S16, int fd_order, double ko_sigma, int ng,
+                              int periodic, double* tdev, double dt, double chi_floor,
+                              double alpha_floor, void* stream);
+
 /* One MSRK rk4_2_2 step of U in place. fprev holds f(y_{n-1}) on entry and is clobbered;
  * fnext receives f(y_n) (swap the two between steps). Replaces BSSNEvolution._msrk2_step
  * (3D/src/bssn3d/evolve.py:139-153). */

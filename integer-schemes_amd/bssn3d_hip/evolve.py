@@ -83,15 +83,45 @@ class BSSNEvolution:
         return F
 
     # --- steps (in place on a tensor) ---------------------------------------------------------
-    def step_(self, U: torch.Tensor, t: float, dt: float) -> torch.Tensor:
-        """One RK4 step of U in place."""
+    def step_(self, U: torch.Tensor, t: float, dt: float,
+              tdev: torch.Tensor | None = None) -> torch.Tensor:
+        """One RK4 step of U in place. With ``tdev`` (one float64 on U's device) the step
+        takes its time from there (``t`` is ignored) and advances it by dt."""
         _, Sx, Sy, Sz = U.shape
         work = self._workspace(U, 0)
-        _lib.check(_lib.load().bssn_rk4_step_f64(
-            U.data_ptr(), work.data_ptr(), Sx, Sy, Sz, _lib.dbl(self._S16(t, dt)), self.order,
-            self.ko_sigma, self.ng, int(self.periodic), float(t), float(dt),
+        lib = _lib.load()
+        if tdev is None:
+            _lib.check(lib.bssn_rk4_step_f64(
+                U.data_ptr(), work.data_ptr(), Sx, Sy, Sz, _lib.dbl(self._S16(t, dt)),
+                self.order, self.ko_sigma, self.ng, int(self.periodic), float(t), float(dt),
+                self.params.chi_floor, self.params.alpha_floor, stream_handle(U.device)))
+            return U
+        if tdev.dtype != torch.float64 or tdev.numel() != 1 or tdev.device != U.device:
+            raise ValueError("tdev must be one float64 value on the state's device")
+        _lib.check(lib.bssn_rk4_step_devtime_f64(
+            U.data_ptr(), work.data_ptr(), Sx, Sy, Sz, _lib.dbl(self._S16(0.0, dt)),
+            self.order, self.ko_sigma, self.ng, int(self.periodic), tdev.data_ptr(), float(dt),
             self.params.chi_floor, self.params.alpha_floor, stream_handle(U.device)))
         return U
+
+    def capture_steps(self, U: torch.Tensor, tdev: torch.Tensor, dt: float,
+                      nsteps: int) -> torch.cuda.CUDAGraph:
+        """Record ``nsteps`` RK4 steps of U (time in tdev) into a HIP graph; each replay
+        advances U and tdev by nsteps steps."""
+        self._workspace(U, 0)
+        warm = U.clone()                      # first launches happen outside the capture
+        self.step_(warm, 0.0, dt)
+        torch.cuda.synchronize(U.device)
+        del warm
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream(device=U.device)
+        s.wait_stream(torch.cuda.current_stream(U.device))
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                for _ in range(nsteps):
+                    self.step_(U, 0.0, dt, tdev=tdev)
+        torch.cuda.current_stream(U.device).wait_stream(s)
+        return g
 
     def msrk_step_(self, U: torch.Tensor, fprev: torch.Tensor, fnext: torch.Tensor, t: float,
                    dt: float) -> torch.Tensor:
@@ -109,12 +139,24 @@ class BSSNEvolution:
         U = as_tensor(state).clone()
         return BSSNState(self.step_(U, t, dt))
 
-    def evolve(self, state, dt: float, nsteps: int, t0: float = 0.0) -> BSSNState:
-        """nsteps steps from t0 (evolve.py:155-194); the input is not modified."""
+    def evolve(self, state, dt: float, nsteps: int, t0: float = 0.0,
+               graph_chunk: int = 0) -> BSSNState:
+        """nsteps steps from t0 (evolve.py:155-194); the input is not modified.
+        graph_chunk > 0 (RK4): capture that many steps once into a HIP graph and replay it,
+        with the time advanced on the device -- the analogue of the reference's single
+        compiled fori_loop; the SSL factor is then formed by the device's exp."""
         U = as_tensor(state).clone()
         if nsteps <= 0:
             return BSSNState(U)
         if self.integrator == "rk4":
+            if graph_chunk > 0 and nsteps >= graph_chunk:
+                tdev = torch.full((1,), float(t0), dtype=torch.float64, device=U.device)
+                g = self.capture_steps(U, tdev, dt, graph_chunk)
+                for _ in range(nsteps // graph_chunk):
+                    g.replay()
+                for _ in range(nsteps % graph_chunk):
+                    self.step_(U, 0.0, dt, tdev=tdev)
+                return BSSNState(U)
             t = t0
             for _ in range(nsteps):
                 self.step_(U, t, dt)

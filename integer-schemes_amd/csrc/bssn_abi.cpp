@@ -112,6 +112,56 @@ int run_stage(const StageArgs& a, const Dims& d, const Scales& w, const Scal& sc
   return check_hip(launch_fused(a, d, w, sc, ord, ko, mode, st), "fused stage launch");
 }
 
+// RK4 step; with tdev set, every stage reads its time from device memory (*tdev + offset)
+int rk4_step(double* U, double* work, int Sx, int Sy, int Sz, const double* S16, int fd_order,
+             double ko_sigma, int ng, int periodic, double t, const double* tdev, double dt,
+             double chi_floor, double alpha_floor, void* stream) {
+  if (!U || !work) return fail(BSSN_E_ARG, "null pointer");
+  if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4 or 6");
+  if (!(ko_sigma >= 0.0)) return fail(BSSN_E_ARG, "ko_sigma must be >= 0");
+  if (!(dt != 0.0) || !std::isfinite(dt) || !std::isfinite(t))
+    return fail(BSSN_E_ARG, "dt must be finite and nonzero, t finite");
+  TRY(check_shape(Sx, Sy, Sz));
+  TRY(check_s16(S16));
+  if (periodic) TRY(check_ng(Sx, Sy, Sz, ng));
+  hipStream_t st = (hipStream_t)stream;
+  const long P = (long)Sx * Sy * Sz;
+  const size_t NS = (size_t)BSSN_NF * P;
+  double* Ya = work;
+  double* Yb = work + NS;
+  double* KS = work + 2 * NS;
+  const double h3[3] = {S16[13], S16[14], S16[15]};
+  const Scales w = make_scales(h3, fd_order, ko_sigma);
+  const Dims d = periodic ? interior_dims(Sx, Sy, Sz, ng) : full_dims(Sx, Sy, Sz, ng);
+  const int mode = periodic ? WRAP : CLAMP;
+  const bool ko = ko_sigma > 0.0;
+  const double hdt = 0.5 * dt;
+  StageArgs a{};
+  a.U = U;
+  a.K1 = KS;
+  a.Uout = U;
+  a.dt = dt;
+  a.chi_floor = chi_floor;
+  a.alpha_floor = alpha_floor;
+  a.tdev = tdev;
+  a.ssl_sig = S16[5];
+  // k1 = f(y, t);          Y = y + (dt/2) k1
+  a.Yin = U; a.Y = Ya; a.kind = KIND_RK1; a.c0 = hdt; a.toff = 0.0;
+  TRY(run_stage(a, d, w, make_scal(S16, dt, t), fd_order, ko, mode, st));
+  // k2 = f(Y, t + dt/2);   Y = y + (dt/2) k2
+  a.Yin = Ya; a.Y = Yb; a.kind = KIND_RK2; a.c0 = hdt; a.toff = hdt;
+  TRY(run_stage(a, d, w, make_scal(S16, dt, t + hdt), fd_order, ko, mode, st));
+  // k3 = f(Y, t + dt/2);   Y = y + dt k3
+  a.Yin = Yb; a.Y = Ya; a.kind = KIND_RK3; a.c0 = dt; a.toff = hdt;
+  TRY(run_stage(a, d, w, make_scal(S16, dt, t + hdt), fd_order, ko, mode, st));
+  // k4 = f(Y, t + dt);     y + (dt/6)(k1 + 2k2 + 2k3 + k4), then enforce
+  a.Yin = Ya; a.Y = nullptr; a.kind = KIND_RK4; a.c0 = dt / 6.0; a.toff = dt;
+  TRY(run_stage(a, d, w, make_scal(S16, dt, t + dt), fd_order, ko, mode, st));
+  if (periodic && ng > 0)
+    return check_hip(launch_periodic_sync(U, full_dims(Sx, Sy, Sz, ng), st), "periodic sync");
+  return BSSN_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -234,48 +284,18 @@ size_t bssn_step_workspace_bytes(int integrator, int Sx, int Sy, int Sz) {
 int bssn_rk4_step_f64(double* U, double* work, int Sx, int Sy, int Sz, const double* S16,
                       int fd_order, double ko_sigma, int ng, int periodic, double t, double dt,
                       double chi_floor, double alpha_floor, void* stream) {
-  if (!U || !work) return fail(BSSN_E_ARG, "null pointer");
-  if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4 or 6");
-  if (!(ko_sigma >= 0.0)) return fail(BSSN_E_ARG, "ko_sigma must be >= 0");
-  if (!(dt != 0.0) || !std::isfinite(dt) || !std::isfinite(t))
-    return fail(BSSN_E_ARG, "dt must be finite and nonzero, t finite");
-  TRY(check_shape(Sx, Sy, Sz));
-  TRY(check_s16(S16));
-  if (periodic) TRY(check_ng(Sx, Sy, Sz, ng));
-  hipStream_t st = (hipStream_t)stream;
-  const long P = (long)Sx * Sy * Sz;
-  const size_t NS = (size_t)BSSN_NF * P;
-  double* Ya = work;
-  double* Yb = work + NS;
-  double* KS = work + 2 * NS;
-  const double h3[3] = {S16[13], S16[14], S16[15]};
-  const Scales w = make_scales(h3, fd_order, ko_sigma);
-  const Dims d = periodic ? interior_dims(Sx, Sy, Sz, ng) : full_dims(Sx, Sy, Sz, ng);
-  const int mode = periodic ? WRAP : CLAMP;
-  const bool ko = ko_sigma > 0.0;
-  const double hdt = 0.5 * dt;
-  StageArgs a{};
-  a.U = U;
-  a.K1 = KS;
-  a.Uout = U;
-  a.dt = dt;
-  a.chi_floor = chi_floor;
-  a.alpha_floor = alpha_floor;
-  // k1 = f(y, t);          Y = y + (dt/2) k1
-  a.Yin = U; a.Y = Ya; a.kind = KIND_RK1; a.c0 = hdt;
-  TRY(run_stage(a, d, w, make_scal(S16, dt, t), fd_order, ko, mode, st));
-  // k2 = f(Y, t + dt/2);   Y = y + (dt/2) k2
-  a.Yin = Ya; a.Y = Yb; a.kind = KIND_RK2; a.c0 = hdt;
-  TRY(run_stage(a, d, w, make_scal(S16, dt, t + hdt), fd_order, ko, mode, st));
-  // k3 = f(Y, t + dt/2);   Y = y + dt k3
-  a.Yin = Yb; a.Y = Ya; a.kind = KIND_RK3; a.c0 = dt;
-  TRY(run_stage(a, d, w, make_scal(S16, dt, t + hdt), fd_order, ko, mode, st));
-  // k4 = f(Y, t + dt);     y + (dt/6)(k1 + 2k2 + 2k3 + k4), then enforce
-  a.Yin = Ya; a.Y = nullptr; a.kind = KIND_RK4; a.c0 = dt / 6.0;
-  TRY(run_stage(a, d, w, make_scal(S16, dt, t + dt), fd_order, ko, mode, st));
-  if (periodic && ng > 0)
-    return check_hip(launch_periodic_sync(U, full_dims(Sx, Sy, Sz, ng), st), "periodic sync");
-  return BSSN_OK;
+  return rk4_step(U, work, Sx, Sy, Sz, S16, fd_order, ko_sigma, ng, periodic, t, nullptr, dt,
+                  chi_floor, alpha_floor, stream);
+}
+
+int bssn_rk4_step_devtime_f64(double* U, double* work, int Sx, int Sy, int Sz,
+                              const double* S16, int fd_order, double ko_sigma, int ng,
+                              int periodic, double* tdev, double dt, double chi_floor,
+                              double alpha_floor, void* stream) {
+  if (!tdev) return fail(BSSN_E_ARG, "null device time");
+  TRY(rk4_step(U, work, Sx, Sy, Sz, S16, fd_order, ko_sigma, ng, periodic, 0.0, tdev, dt,
+               chi_floor, alpha_floor, stream));
+  return check_hip(launch_time_advance(tdev, dt, (hipStream_t)stream), "time advance");
 }
 
 int bssn_msrk_step_f64(double* U, double* fprev, double* fnext, double* work, int Sx, int Sy,
@@ -520,9 +540,6 @@ BatchArgs batch_args(const bssn_batch* b, bool interior, const double* S16, doub
   a.cahd_c = S16[1];
   a.dt = dt;
   a.sigma = sigma;
-  a.tdev = nullptr;
-  a.toff = 0.0;
-  a.ssl_sig = S16[5];
   return a;
 }
 
@@ -702,12 +719,11 @@ int bssn_batch_rk4_step_f64(const bssn_batch* b, double* U, double* work, const 
   s.dt = dt;
   s.chi_floor = chi_floor;
   s.alpha_floor = alpha_floor;
+  s.tdev = tdev;
+  s.ssl_sig = S16[5];
   auto stage = [&](double toff, int kind, double c0, const double* yin, double* y) {
-    BatchArgs a = batch_args(b, interior, S16, dt, t + toff, ko_sigma);
-    if (tdev) {
-      a.tdev = tdev;
-      a.toff = toff;
-    }
+    const BatchArgs a = batch_args(b, interior, S16, dt, t + toff, ko_sigma);
+    s.toff = toff;
     s.Yin = yin;
     s.Y = y;
     s.kind = kind;

@@ -37,15 +37,6 @@ __device__ __forceinline__ const double* shifted(const double* p, long off) {
 }
 __device__ __forceinline__ double* shifted(double* p, long off) { return p ? p + off : p; }
 
-// A wave-uniform double computed by vector instructions (there is no scalar FP64 division),
-// moved to scalar registers so that it does not occupy a vector register for the whole kernel.
-__device__ __forceinline__ double uniform(double x) {
-  const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
-  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
-  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
-  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
-}
-
 // next brick of this wave: -1 when every chunk it may serve is drained
 __device__ __forceinline__ int next_item(const BatchArgs& b, int* chunk, int* round) {
   if (!b.persistent) {
@@ -93,12 +84,8 @@ __device__ __forceinline__ void batch_brick(double* lds, const StageArgs& args,
     w.sm[a] = uniform(w.sm[a]);
     w.sko[a] = uniform(w.sko[a]);
   }
-  Scal sc = b.base;
+  Scal sc = stage_scal(b.base, args.tdev, args.toff, args.ssl_sig);
   sc.cahd = uniform(cahd_factor(b.cahd_c, blk.h[0], b.dt));
-  if (b.tdev) {
-    const double t = *b.tdev + b.toff;
-    sc.ssl_e = uniform(exp(-0.5 * t * t / (b.ssl_sig * b.ssl_sig)));
-  }
   StageArgs a = args;
   a.Yin = shifted(args.Yin, blk.off);
   a.U = shifted(args.U, blk.off);

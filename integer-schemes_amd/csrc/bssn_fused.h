@@ -42,7 +42,7 @@ void k_fused(StageArgs args, Dims d, Scales w, Scal sc) {
   const int lane = (int)threadIdx.x;
   WaveSrc<ORD, KO, MODE> src(lds, args.Yin, w, d, ox, oy, oz, lane);
   StageOut<KIND> out(args, d.P, src.gc, lane_valid<G>(d, ox, oy, oz, lane));
-  bssn_rhs_point<KO>(src, sc, out);
+  bssn_rhs_point<KO>(src, stage_scal(sc, args.tdev, args.toff, args.ssl_sig), out);
 }
 
 template <int ORD, bool KO, int MODE, int KIND>

@@ -33,6 +33,8 @@ struct StageArgs {
   double c0, c1, c2;   // per-kind coefficients
   double dt;
   double chi_floor, alpha_floor;
+  const double* tdev;  // if set, the stage time is *tdev + toff and ssl_e is formed on the device
+  double toff, ssl_sig;
 };
 
 hipError_t launch_fused(const StageArgs& a, const Dims& d, const Scales& w, const Scal& sc, int ord,
@@ -76,8 +78,6 @@ struct BatchArgs {
   int interior;      // the list covers block interiors (else whole padded blocks)
   Scal base;         // shared scalars; cahd is set per block from its spacing
   double cahd_c, dt, sigma;
-  const double* tdev;   // if set, the stage time is *tdev + toff and ssl_e is formed on the device
-  double toff, ssl_sig;
 };
 
 hipError_t launch_fused_batch(const StageArgs& a, const BatchArgs& b, int grid, int ord, bool ko,

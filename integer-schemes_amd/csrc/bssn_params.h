@@ -58,6 +58,8 @@ struct Dims {
 };
 
 #if defined(__HIPCC__)
+#include <math.h>
+
 // Scales of the integer stencils: the denominators of the centred and Kreiss-Oliger
 // coefficients (common/src/mcs_common/derivatives.py:33-49) times the spacings. Computed on
 // the host for one grid and on the device per block of a batch; contraction is off so that
@@ -85,6 +87,26 @@ __host__ __device__ inline double cahd_factor(double C, double dx, double dt) {
 #pragma clang fp contract(off)
   const double dx2 = dx * dx;
   return C * dx2 / (dt * (10.0 * dx2 + 1.0));
+}
+
+// A wave-uniform double computed by vector instructions (there is no scalar FP64 arithmetic),
+// moved to scalar registers so that it does not occupy a vector register for the whole kernel.
+__device__ __forceinline__ double uniform(double x) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// The SSL factor exp(-t^2 / (2 sig^2)) at a stage time held in device memory (a captured step
+// replayed with its time advanced on the device), else the host's value in sc.
+__device__ __forceinline__ Scal stage_scal(Scal sc, const double* tdev, double toff,
+                                           double sig) {
+  if (tdev) {
+    const double t = *tdev + toff;
+    sc.ssl_e = uniform(exp(-0.5 * t * t / (sig * sig)));
+  }
+  return sc;
 }
 
 // Keep a value where it is computed: an empty volatile asm that claims to rewrite x. The

@@ -162,6 +162,46 @@ def test_evolution_matches_oracle(integrator, nsteps):
     assert err.max() < TOL, err
 
 
+def test_graph_evolve_matches_oracle():
+    """7 RK4 steps as 2 replays of a captured 3-step graph + 1 device-time step, against the
+    oracle (the SSL factor comes from the device's exp: same 1e-12 bar)."""
+    say("graph evolve 7 steps (chunk 3), periodic, puncture 20^3")
+    grid, st, _ = idata.config_single_puncture(seed=6, N=20)
+    F = st.numpy()
+    dt = 0.25 * grid.dx
+    p = PhysicsParams()
+    evo = BSSNEvolution(grid, p, order=4, ko_sigma=0.1, bc="periodic")
+    got = to_host(evo.evolve(to_dev(F), dt, 7, t0=0.3, graph_chunk=3).data)
+    S16 = scalars(p, grid.dx, grid.dy, grid.dz, dt, 0.3)
+    ref = O.periodic_sync(O.evolve(F, S16, dt, 7, t0=0.3, integrator="rk4", order=4,
+                                   ko_sigma=0.1, ng=3), 3)
+    err = normwise(got, ref)
+    assert err.max() < TOL, err
+
+
+def test_graph_replays_with_eager_work_between():
+    """A captured 2-step graph replayed 3 times, eager steps of a copy between the replays;
+    every replay must agree with the eager result."""
+    say("graph replay x3 vs eager, puncture 16^3")
+    grid, st, _ = idata.config_single_puncture(seed=7, N=16, device=DEV)
+    dt = 0.25 * grid.dx
+    evo = BSSNEvolution(grid, PhysicsParams(), order=4, ko_sigma=0.1, bc="periodic")
+    U = st.clone()
+    V = st.clone()
+    tdev = torch.full((1,), 0.1, dtype=torch.float64, device=DEV)
+    g = evo.capture_steps(U, tdev, dt, 2)
+    t = 0.1
+    for r in range(3):
+        g.replay()
+        torch.cuda.synchronize()
+        for _ in range(2):
+            evo.step_(V, t, dt)
+            t += dt
+        err = normwise(to_host(U), to_host(V))
+        assert err.max() < TOL, (r, err.max())
+    assert abs(float(tdev.item()) - t) < 1e-14
+
+
 def test_step_edge_padded():
     say("rk4 step with bc='none' (edge-padded stencils, all padded points advanced)")
     grid, st, _ = idata.config_single_puncture(seed=5, N=12)
