@@ -190,9 +190,10 @@ int bssn_batch_halo_f64(double* pool, const bssn_halo_copy* map, long long n, vo
  * RHS of all blocks, interior update, ghost sync; 2D/src/mcs2d/amr/evolve.py:745-771).
  * bc = 0: blocks edge-padded (bssn_rk4_step_f64 with periodic = 0, per block);
  * bc = 1: blocks periodic on their own interiors (bssn_rk4_step_f64 with periodic = 1);
- * bc = 2: ghosts come from other blocks: interiors advance with stencils reading the ghosts,
- *         and the halo map (nhalo device entries) refills the ghosts of every stage input and
- *         of U at the end.
+ * bc = 2: ghosts are data: interiors advance with stencils reading the ghosts. The stage
+ *         inputs start with U's ghosts (held: Dirichlet data from the last sync), then the halo
+ *         map (nhalo device entries, may be 0) refills ghosts of every stage input and of U at
+ *         the end (the reference's same-level neighbour sync).
  * tdev: if not null, the step time is read from device memory (*tdev, t ignored) and
  * advanced by dt at the end, so a captured step can be replayed step after step. */
 int bssn_batch_rk4_step_f64(const bssn_batch* b, double* U, double* work, const double* S16,

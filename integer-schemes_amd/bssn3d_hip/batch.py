@@ -13,9 +13,11 @@ replayed with its time in device memory.
 Ghost handling per step (``bc``):
   "edge"     each block edge-padded (= the single-grid step with bc != "periodic")
   "periodic" each block periodic on its own interior (= the single-grid periodic step)
-  "halo"     ghosts filled from other blocks by a halo copy map (HaloMap) after every stage,
-             the reference's same-level neighbour sync (amr/evolve.py:719-735); prolongation
-             from a coarser parent is not part of this form.
+  "halo"     ghosts are data: the stage inputs start with U's ghosts (held, Dirichlet data from
+             the last sync), and an optional halo copy map (HaloMap) refills them from other
+             blocks after every stage, the reference's same-level neighbour sync
+             (amr/evolve.py:719-735); prolongation from a coarser parent is not part of this
+             form.
 """
 from __future__ import annotations
 
@@ -173,8 +175,6 @@ class BlockBatch(BlockLayout):
         if bc not in BC:
             raise ValueError(f"bc must be one of {sorted(BC)}")
         self._check_pool(U)
-        if bc == "halo" and halo is None:
-            raise ValueError('bc="halo" needs a HaloMap')
         hptr, hn = (halo.data.data_ptr(), halo.n) if halo is not None and bc == "halo" else (None, 0)
         tptr = None
         if tdev is not None:

@@ -732,6 +732,9 @@ int bssn_batch_rk4_step_f64(const bssn_batch* b, double* U, double* work, const 
     if (bc == 2 && y) TRY(bssn_batch_halo_f64(y, halo, nhalo, stream));
     return BSSN_OK;
   };
+  if (bc == 2 && b->ng > 0)   // stage inputs start with U's ghosts
+    TRY(check_hip(launch_batch_hold(U, Ya, Yb, b->blk, b->ghost, b->grid_ghost, st),
+                  "ghost hold"));
   TRY(stage(0.0, KIND_RK1, hdt, U, Ya));
   TRY(stage(hdt, KIND_RK2, hdt, Ya, Yb));
   TRY(stage(hdt, KIND_RK3, dt, Yb, Ya));

@@ -253,7 +253,7 @@ __global__ __launch_bounds__(64) void k_constraints(const double* __restrict__ F
   int ox, oy, oz;
   if (!brick_origin<G>(d, &ox, &oy, &oz)) return;
   const int lane = (int)threadIdx.x;
-  WaveSrc<ORD, false, CLAMP> s(lds, F, w, d, ox, oy, oz, lane);
+  WaveSrc<ORD, false, CLAMP, ConsOrder> s(lds, F, w, d, ox, oy, oz, lane);
   ConsOut o{HM, d.P, s.gc, lane_valid<G>(d, ox, oy, oz, lane)};
   bssn_constraints_point(s, o);
 }

@@ -28,9 +28,6 @@
 
 namespace bssn {
 
-#ifndef BSSN_FUSED_WAVES
-#define BSSN_FUSED_WAVES 2
-#endif
 
 __device__ __forceinline__ const double* shifted(const double* p, long off) {
   return p ? p + off : p;
@@ -99,7 +96,8 @@ __device__ __forceinline__ void batch_brick(double* lds, const StageArgs& args,
 }
 
 template <int ORD, bool KO, int MODE, bool PERSIST, int KIND>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BSSN_FUSED_WAVES, BSSN_FUSED_WAVES)))
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(fused_waves<KIND>(), fused_waves<KIND>())))
 void k_fused_batch(StageArgs args, BatchArgs b) {
   using G = Geo<ORD, KO>;
   __shared__ double lds[G::VOL];

@@ -68,6 +68,47 @@ hipError_t launch_batch_periodic(double* pool, const BlockDesc* blk, const Brick
   return hipGetLastError();
 }
 
+// Ghosts held for a step: every ghost point of every block is copied from U into the two stage
+// buffers (all 24 fields), so stages whose ghosts are not refilled by a halo map read the
+// ghosts of the step's base state (Dirichlet data from the last sync).
+__global__ __launch_bounds__(64) void k_batch_hold(const double* __restrict__ U,
+                                                   double* __restrict__ Ya,
+                                                   double* __restrict__ Yb,
+                                                   const BlockDesc* __restrict__ blks,
+                                                   BrickList ghost) {
+  const int w = (int)blockIdx.x;
+  const int ch = w & (BATCH_CHUNKS - 1);
+  const int i = ghost.cb[ch] + (w >> 3);
+  if (i >= ghost.cb[ch + 1]) return;
+  const BrickItem item = ghost.items[i];
+  const BlockDesc blk = blks[item.blk];
+  const int lane = (int)threadIdx.x;
+  const int c[3] = {(item.org & 1023) + lane / 16, ((item.org >> 10) & 1023) + (lane / 4) % 4,
+                    ((item.org >> 20) & 1023) + lane % 4};
+  bool ghost_pt = false;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (c[a] >= blk.S[a]) return;
+    if (c[a] < blk.ng || c[a] >= blk.S[a] - blk.ng) ghost_pt = true;
+  }
+  if (!ghost_pt) return;
+  const long P = (long)blk.S[0] * blk.S[1] * blk.S[2];
+  const long p = blk.off + ((long)c[0] * blk.S[1] + c[1]) * blk.S[2] + c[2];
+#pragma unroll 4
+  for (int f = 0; f < BSSN_NF; ++f) {
+    const double v = U[p + (long)f * P];
+    Ya[p + (long)f * P] = v;
+    Yb[p + (long)f * P] = v;
+  }
+}
+
+hipError_t launch_batch_hold(const double* U, double* Ya, double* Yb, const BlockDesc* blk,
+                             const BrickList& ghost, int grid, hipStream_t st) {
+  if (grid <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_batch_hold, dim3(grid), dim3(64), 0, st, U, Ya, Yb, blk, ghost);
+  return hipGetLastError();
+}
+
 // Ghost exchange between blocks: entry e copies all 24 fields of one point,
 // pool[dst + f * dst_stride] = pool[src + f * src_stride]. Sources are interior points, which
 // no entry writes, so the order of the copies does not matter.

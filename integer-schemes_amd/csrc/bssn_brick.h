@@ -123,6 +123,7 @@ __constant__ CellTable<G> kCells = CellTable<G>();
 // per-field loops (bundle, KO) in state order.
 struct RhsOrder {
   // gt0..5 | chi | Gt0..2 | alpha | beta0..2 | At0..5 | K | B0..2
+  static constexpr int N = BSSN_NF;
   __host__ __device__ static constexpr int at(int i) {
     return i < 6 ? GT0 + i
          : i == 6 ? CHI
@@ -145,14 +146,33 @@ struct RhsOrder {
   }
 };
 struct StateOrder {
+  static constexpr int N = BSSN_NF;
   __host__ __device__ static constexpr int at(int i) { return i; }
   __host__ __device__ static constexpr int pos(int f) { return f; }
+};
+// the constraints: gt0..5 | chi | Gt0..2 | At0..5 | K (pos = N for the fields they do not use)
+struct ConsOrder {
+  static constexpr int N = 17;
+  __host__ __device__ static constexpr int at(int i) {
+    return i < 6 ? GT0 + i : (i == 6 ? CHI : (i < 10 ? GTI0 + (i - 7) : (i < 16 ? AT0 + (i - 10)
+                                                                                  : KTR)));
+  }
+  __host__ __device__ static constexpr int pos(int f) {
+    return f >= GT0 && f <= GT5 ? f - GT0
+         : f == CHI ? 6
+         : f >= GTI0 && f <= GTI0 + 2 ? 7 + f - GTI0
+         : f >= AT0 && f <= AT5 ? 10 + f - AT0
+         : f == KTR ? 16
+         : N;
+  }
 };
 
 // Source of fields and derivatives for bssn_rhs_point: one wave-private LDS box per field.
 // Staging is pipelined one field ahead: stage(f) writes field f (loaded during the previous
 // stage) into the box, then issues the loads of the next field in ORDER, which complete while
-// field f's derivatives and algebra run.
+// field f's derivatives and algebra run. Contract: a kernel stages its fields exactly in the
+// sequence of its ORDER (RhsOrder for bssn_rhs_point, ConsOrder for the constraints,
+// StateOrder for per-field loops); a different sequence would stage the wrong field.
 template <int ORD, bool KO, int MODE, class ORDER = RhsOrder>
 struct WaveSrc {
   using G = Geo<ORD, KO>;
@@ -216,7 +236,7 @@ struct WaveSrc {
     // moving this field's reads above the writes.
     __builtin_amdgcn_wave_barrier();
     const int i = ORDER::pos(f) + 1;
-    if (i < BSSN_NF) fetch(ORDER::at(i));
+    if (i < ORDER::N) fetch(ORDER::at(i));
   }
 
   __device__ __forceinline__ double center(int f) const { return F[(long)f * P + gc]; }

@@ -26,14 +26,9 @@
 
 namespace bssn {
 
-// Waves per SIMD the register allocator targets (build knob; 1 = the whole 512-entry
-// VGPR+AGPR file per lane).
-#ifndef BSSN_FUSED_WAVES
-#define BSSN_FUSED_WAVES 2
-#endif
-
 template <int ORD, bool KO, int MODE, int KIND>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BSSN_FUSED_WAVES, BSSN_FUSED_WAVES)))
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(fused_waves<KIND>(), fused_waves<KIND>())))
 void k_fused(StageArgs args, Dims d, Scales w, Scal sc) {
   using G = Geo<ORD, KO>;
   __shared__ double lds[G::VOL];

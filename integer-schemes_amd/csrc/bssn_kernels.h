@@ -90,6 +90,8 @@ hipError_t launch_batch_reset(int* ctr, hipStream_t st);
 hipError_t launch_batch_periodic(double* pool, const BlockDesc* blk, const BrickList& ghost,
                                  int grid, hipStream_t st);
 hipError_t launch_halo_copy(double* pool, const void* map, long n, hipStream_t st);
+hipError_t launch_batch_hold(const double* U, double* Ya, double* Yb, const BlockDesc* blk,
+                             const BrickList& ghost, int grid, hipStream_t st);
 hipError_t launch_time_advance(double* tdev, double dt, hipStream_t st);
 
 // two-pass (materialised derivative bundle) path

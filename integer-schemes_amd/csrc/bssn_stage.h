@@ -36,6 +36,21 @@ __device__ __forceinline__ void enforce_point(double* g, double* a) {
   for (int i = 0; i < 6; ++i) a[i] = a[i] - g[i] * third;
 }
 
+// Waves per SIMD the register allocator targets: 1 gives a wave the whole 512-entry
+// VGPR+AGPR file (no spills, less latency hiding), 2 halves it. Measured at 256^3 (FD4/KO6):
+// the bare RHS is faster at 2, the RK stages (KO + stage update) at 1.
+#ifndef BSSN_RHS_WAVES
+#define BSSN_RHS_WAVES 2
+#endif
+#ifndef BSSN_STAGE_WAVES
+#define BSSN_STAGE_WAVES 1
+#endif
+
+template <int KIND>
+constexpr int fused_waves() {
+  return KIND == KIND_RHS ? BSSN_RHS_WAVES : BSSN_STAGE_WAVES;
+}
+
 // The write-back kind is a template parameter: each launch runs the code of its own kind only
 // (a runtime switch over all kinds made the stage kernel half again as large).
 // RK2 and RK3 share their code (c0 differs); KIND_RK3 launches StageOut<KIND_RK2>.

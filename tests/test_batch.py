@@ -37,7 +37,7 @@ def small_blocks(seed=0, n=9, ng=3):
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(n):
-        ext = tuple(int(v) for v in rng.choice([3, 5, 8, 12], size=3))
+        ext = tuple(int(v) for v in rng.choice([ng, 5, 8, 12], size=3))
         h = float(rng.choice([0.25, 0.125, 0.0625]))
         lo = tuple(float(v) for v in rng.uniform(-2, 2, size=3))
         out.append(Block(ext, (h, h, h), lo))
@@ -98,7 +98,7 @@ def test_batch_periodic_sync_equals_single_grid():
     batch.periodic_sync_(V)
     for b, blk in enumerate(batch.blocks):
         ref = batch.view(U, b).clone()
-        BSSNEvolution(blk.grid(3), bc="periodic").periodic_sync_(ref)
+        BSSNEvolution(blk.grid(3), order=4, bc="periodic").periodic_sync_(ref)
         torch.cuda.synchronize()
         assert torch.equal(batch.view(V, b), ref)
 
@@ -135,6 +135,54 @@ def test_tiled_box_with_halo_equals_single_grid():
         want = Ug[:, t[0]:t[0] + e[0] + 2 * ng, t[1]:t[1] + e[1] + 2 * ng,
                   t[2]:t[2] + e[2] + 2 * ng]
         assert torch.equal(batch.view(U, b), want), f"block {b}"
+
+
+def test_held_ghosts_step_equals_composed_rk4():
+    """bc="halo" without a map: stage inputs carry U's ghosts. Against RK4 composed in torch
+    from the single-grid stage RHS (bssn_stage_rhs_f64, ghosts read as data) and enforce;
+    1e-12 normwise (the fused update rounds with FMAs); U's ghosts must stay bit-identical."""
+    say("held ghosts vs composed RK4")
+    from bssn3d_hip import _lib
+    from bssn3d_hip.rhs import stream_handle
+    batch, U = make_batch(small_blocks(6, n=5), seed=5)
+    p = PhysicsParams()
+    dt, t = 0.01, 0.2
+    U0 = U.clone()
+    batch.step_(U, t, dt, p, order=4, ko_sigma=0.1, bc="halo")
+    lib = _lib.load()
+    ng = 3
+    for b, blk in enumerate(batch.blocks):
+        F = batch.view(U0, b).clone()
+        _, Sx, Sy, Sz = F.shape
+        inner = (slice(None),) + (slice(ng, -ng),) * 3
+
+        def f(Y, tt):
+            out = torch.empty_like(Y)
+            _lib.check(lib.bssn_stage_rhs_f64(Y.data_ptr(), out.data_ptr(), Sx, Sy, Sz,
+                                              _lib.dbl(scalars(p, *blk.h, dt, tt)), 4, 0.1, ng, 0,
+                                              stream_handle(Y.device)))
+            return out
+
+        def upd(c, k):
+            Y = F.clone()
+            Y[inner] = F[inner] + c * k[inner]
+            return Y
+        k1 = f(F, t)
+        k2 = f(upd(0.5 * dt, k1), t + 0.5 * dt)
+        k3 = f(upd(0.5 * dt, k2), t + 0.5 * dt)
+        k4 = f(upd(dt, k3), t + dt)
+        new = upd(dt / 6.0, k1 + 2.0 * k2 + 2.0 * k3 + k4)
+        enf = new.clone()
+        _lib.check(lib.bssn_enforce_f64(enf.data_ptr(), Sx * Sy * Sz, p.chi_floor, p.alpha_floor,
+                                        stream_handle(enf.device)))
+        new[inner] = enf[inner]
+        got = batch.view(U, b)
+        torch.cuda.synchronize()
+        e = normwise(got[inner].cpu().numpy(), new[inner].cpu().numpy())
+        assert np.all(e <= 1e-12), (b, e.max())
+        mask = torch.ones_like(got, dtype=torch.bool)
+        mask[inner] = False
+        assert torch.equal(got[mask], batch.view(U0, b)[mask]), f"ghosts of block {b} changed"
 
 
 def test_captured_steps_replay():
