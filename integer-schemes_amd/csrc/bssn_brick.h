@@ -170,10 +170,11 @@ struct ConsOrder {
 // Source of fields and derivatives for bssn_rhs_point: one wave-private LDS box per field.
 // Staging is pipelined one field ahead: stage(f) writes field f (loaded during the previous
 // stage) into the box, then issues the loads of the next field in ORDER, which complete while
-// field f's derivatives and algebra run. Contract: a kernel stages its fields exactly in the
+// field f's derivatives and algebra run (DEPTH fields ahead: DEPTH > 1 needs the stage calls
+// unrolled, so that each slot index is a constant). Contract: a kernel stages its fields exactly in the
 // sequence of its ORDER (RhsOrder for bssn_rhs_point, ConsOrder for the constraints,
 // StateOrder for per-field loops); a different sequence would stage the wrong field.
-template <int ORD, bool KO, int MODE, class ORDER = RhsOrder>
+template <int ORD, bool KO, int MODE, class ORDER = RhsOrder, int DEPTH = 1>
 struct WaveSrc {
   using G = Geo<ORD, KO>;
   using C = Coef<ORD>;
@@ -183,7 +184,7 @@ struct WaveSrc {
   long P;
   int off[G::PER];      // grid offsets of this lane's staged cells
   int cel[G::PER];      // box indices of this lane's staged cells
-  double pf[G::PER];    // the next field's cells, in flight
+  double pf[DEPTH][G::PER];   // the next DEPTH fields' cells, in flight (slot = position % DEPTH)
   long gc;              // flat index of the lane's point (clamped into the grid)
   int c;                // box index of the lane's point
 
@@ -218,25 +219,27 @@ struct WaveSrc {
       }
       off[k] = (g[0] * d.S[1] + g[1]) * d.S[2] + g[2];
     }
-    fetch(ORDER::at(0));
+#pragma unroll
+    for (int i = 0; i < DEPTH; ++i)
+      if (i < ORDER::N) fetch(i, ORDER::at(i));
   }
 
-  __device__ __forceinline__ void fetch(int f) {
+  __device__ __forceinline__ void fetch(int slot, int f) {
     const double* u = F + (long)f * P;
 #pragma unroll
-    for (int k = 0; k < G::PER; ++k) pf[k] = u[off[k]];
+    for (int k = 0; k < G::PER; ++k) pf[slot][k] = u[off[k]];
     // keep the loads here, ahead of the work they overlap
     __builtin_amdgcn_sched_barrier(0);
   }
 
   __device__ __forceinline__ void stage(int f) {
+    const int at = ORDER::pos(f), slot = DEPTH == 1 ? 0 : at % DEPTH;
 #pragma unroll
-    for (int k = 0; k < G::PER; ++k) lds[cel[k]] = pf[k];
+    for (int k = 0; k < G::PER; ++k) lds[cel[k]] = pf[slot][k];
     // The wave's LDS accesses complete in issue order; this only stops the compiler from
     // moving this field's reads above the writes.
     __builtin_amdgcn_wave_barrier();
-    const int i = ORDER::pos(f) + 1;
-    if (i < ORDER::N) fetch(ORDER::at(i));
+    if (at + DEPTH < ORDER::N) fetch(slot, ORDER::at(at + DEPTH));
   }
 
   __device__ __forceinline__ double center(int f) const { return F[(long)f * P + gc]; }

@@ -35,7 +35,7 @@ void k_fused(StageArgs args, Dims d, Scales w, Scal sc) {
   int ox, oy, oz;
   if (!brick_origin<G>(d, &ox, &oy, &oz)) return;
   const int lane = (int)threadIdx.x;
-  WaveSrc<ORD, KO, MODE> src(lds, args.Yin, w, d, ox, oy, oz, lane);
+  WaveSrc<ORD, KO, MODE, RhsOrder, fused_depth<KIND>()> src(lds, args.Yin, w, d, ox, oy, oz, lane);
   StageOut<KIND> out(args, d.P, src.gc, lane_valid<G>(d, ox, oy, oz, lane));
   bssn_rhs_point<KO>(src, stage_scal(sc, args.tdev, args.toff, args.ssl_sig), out);
 }

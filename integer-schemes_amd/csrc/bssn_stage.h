@@ -51,6 +51,16 @@ constexpr int fused_waves() {
   return KIND == KIND_RHS ? BSSN_RHS_WAVES : BSSN_STAGE_WAVES;
 }
 
+// Fields in flight while one is computed: deeper at 1 wave per SIMD, where no other wave hides
+// the load latency.
+#ifndef BSSN_STAGE_DEPTH
+#define BSSN_STAGE_DEPTH 1
+#endif
+template <int KIND>
+constexpr int fused_depth() {
+  return fused_waves<KIND>() == 1 ? BSSN_STAGE_DEPTH : 1;
+}
+
 // The write-back kind is a template parameter: each launch runs the code of its own kind only
 // (a runtime switch over all kinds made the stage kernel half again as large).
 // RK2 and RK3 share their code (c0 differs); KIND_RK3 launches StageOut<KIND_RK2>.
@@ -61,8 +71,23 @@ struct StageOut {
   bool valid;
   double gnew[6], anew[6];
 
+  // The write-back operands of all 24 fields are loaded at the start, so their latency is
+  // hidden behind the whole evaluation instead of stalling each put.
+  static constexpr bool NU = KIND != KIND_RHS;
+  static constexpr bool NK = KIND == KIND_RK2 || KIND == KIND_RK4 || KIND == KIND_MS2;
+  static constexpr bool NF = KIND == KIND_MS1 || KIND == KIND_MS2 || KIND == KIND_MS3;
+  double pu[NU ? BSSN_NF : 1], pk[NK ? BSSN_NF : 1], pp[NF ? BSSN_NF : 1];
+
   __device__ __forceinline__ StageOut(const StageArgs& a, long P_, long gp_, bool v)
-      : A(a), P(P_), gp(gp_), valid(v) {}
+      : A(a), P(P_), gp(gp_), valid(v) {
+#pragma unroll
+    for (int f = 0; f < BSSN_NF; ++f) {
+      const long i = (long)f * P + gp;
+      if constexpr (NU) pu[f] = A.U[i];
+      if constexpr (NK) pk[f] = A.K1[i];
+      if constexpr (NF) pp[f] = A.FP[i];
+    }
+  }
 
   __device__ __forceinline__ void final_value(int f, double u) {
     if (f >= GT0 && f <= GT5) {
@@ -82,23 +107,23 @@ struct StageOut {
     if constexpr (KIND == KIND_RHS) {
       A.Y[i] = k;
     } else if constexpr (KIND == KIND_RK1) {            // y + (dt/2) k1 ; S = k1
-      A.Y[i] = A.U[i] + A.c0 * k;
+      A.Y[i] = pu[f] + A.c0 * k;
       A.K1[i] = k;
     } else if constexpr (KIND == KIND_RK2) {            // y + c0 k ; S = S + 2 k  (RK2, RK3)
-      A.Y[i] = A.U[i] + A.c0 * k;
-      A.K1[i] = A.K1[i] + 2.0 * k;
+      A.Y[i] = pu[f] + A.c0 * k;
+      A.K1[i] = pk[f] + 2.0 * k;
     } else if constexpr (KIND == KIND_RK4) {            // y + (dt/6) (S + k4)
-      final_value(f, A.U[i] + A.c0 * (A.K1[i] + k));
+      final_value(f, pu[f] + A.c0 * (pk[f] + k));
     } else if constexpr (KIND == KIND_MS1) {            // y + dt (a20 k0 + a21 k1) ; store k1
-      A.Y[i] = A.U[i] + A.dt * (A.c0 * A.FP[i] + A.c1 * k);
+      A.Y[i] = pu[f] + A.dt * (A.c0 * pp[f] + A.c1 * k);
       A.K1[i] = k;
     } else if constexpr (KIND == KIND_MS2) {            // y + dt (a30 k0 + a31 k1 + a32 k2)
-      const double k0 = A.FP[i], k1 = A.K1[i];
-      A.Y[i] = A.U[i] + A.dt * (A.c0 * k0 + A.c1 * k1 + A.c2 * k);
+      const double k0 = pp[f], k1 = pk[f];
+      A.Y[i] = pu[f] + A.dt * (A.c0 * k0 + A.c1 * k1 + A.c2 * k);
       A.FP[i] = MS_B0 * k0 + MS_B1 * k1 + MS_B2 * k;
     } else {                                            // MS3: y + dt (b0 k0 + ... + b3 k3)
       static_assert(KIND == KIND_MS3, "unknown stage kind");
-      final_value(f, A.U[i] + A.dt * (A.FP[i] + MS_B3 * k));
+      final_value(f, pu[f] + A.dt * (pp[f] + MS_B3 * k));
     }
   }
 
