@@ -167,82 +167,26 @@ struct ConsOrder {
   }
 };
 
-// Source of fields and derivatives for bssn_rhs_point: one wave-private LDS box per field.
-// Staging is pipelined one field ahead: stage(f) writes field f (loaded during the previous
-// stage) into the box, then issues the loads of the next field in ORDER, which complete while
-// field f's derivatives and algebra run (DEPTH fields ahead: DEPTH > 1 needs the stage calls
-// unrolled, so that each slot index is a constant). Contract: a kernel stages its fields exactly in the
-// sequence of its ORDER (RhsOrder for bssn_rhs_point, ConsOrder for the constraints,
-// StateOrder for per-field loops); a different sequence would stage the wrong field.
-template <int ORD, bool KO, int MODE, class ORDER = RhsOrder, int DEPTH = 1>
-struct WaveSrc {
-  using G = Geo<ORD, KO>;
-  using C = Coef<ORD>;
-  double* lds;          // VOL doubles, this wave's box
-  const double* F;      // stencil source state
+// flat index of the lane's point, clamped into the grid (write-back index of invalid lanes)
+template <class G>
+__device__ __forceinline__ long lane_point(const Dims& d, int ox, int oy, int oz, int lane) {
+  const int tx = lane / (G::B * G::B), ty = (lane / G::B) % G::B, tz = lane % G::B;
+  int p[3] = {ox + tx, oy + ty, oz + tz};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) p[a] = p[a] > d.S[a] - 1 ? d.S[a] - 1 : p[a];
+  return ((long)p[0] * d.S[1] + p[1]) * d.S[2] + p[2];
+}
+
+// Stencils on a staged box: the derivative interface of bssn_rhs_point over the LDS box `lds`
+// (row-major, strides SX, SY, 1) at the lane's box index c.
+template <class G, class C>
+struct BoxStencil {
+  double* lds;          // the current field's box
   const Scales& sc;
-  long P;
-  int off[G::PER];      // grid offsets of this lane's staged cells
-  int cel[G::PER];      // box indices of this lane's staged cells
-  double pf[DEPTH][G::PER];   // the next DEPTH fields' cells, in flight (slot = position % DEPTH)
-  long gc;              // flat index of the lane's point (clamped into the grid)
   int c;                // box index of the lane's point
 
-  __device__ __forceinline__ WaveSrc(double* lds_, const double* F_, const Scales& sc_,
-                                     const Dims& d, int ox, int oy, int oz, int lane)
-      : lds(lds_), F(F_), sc(sc_), P(d.P) {
-    const int tx = lane / (G::B * G::B), ty = (lane / G::B) % G::B, tz = lane % G::B;
-    c = (tx + G::H) * G::SX + (ty + G::H) * G::SY + (tz + G::H);
-    int p[3] = {ox + tx, oy + ty, oz + tz};
-#pragma unroll
-    for (int a = 0; a < 3; ++a) p[a] = p[a] > d.S[a] - 1 ? d.S[a] - 1 : p[a];
-    gc = ((long)p[0] * d.S[1] + p[1]) * d.S[2] + p[2];
-    const int o[3] = {ox, oy, oz};
-#pragma unroll
-    for (int k = 0; k < G::PER; ++k) {
-      const int e = kCells<G>.c[lane + k * G::NT];
-      const int cc[3] = {e & 31, (e >> 5) & 31, (e >> 10) & 31};
-      cel[k] = cc[0] * G::SX + cc[1] * G::SY + cc[2];
-      int g[3];
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        int v = o[a] + cc[a] - G::H;
-        if (MODE == CLAMP) {
-          v = v < 0 ? 0 : (v > d.S[a] - 1 ? d.S[a] - 1 : v);
-        } else {
-          const int N = d.S[a] - 2 * d.ng;
-          int w = (v - d.ng) % N;   // once per lane and slot, not per field
-          w = w < 0 ? w + N : w;
-          v = d.ng + w;
-        }
-        g[a] = v;
-      }
-      off[k] = (g[0] * d.S[1] + g[1]) * d.S[2] + g[2];
-    }
-#pragma unroll
-    for (int i = 0; i < DEPTH; ++i)
-      if (i < ORDER::N) fetch(i, ORDER::at(i));
-  }
+  __device__ __forceinline__ BoxStencil(double* lds_, const Scales& sc_) : lds(lds_), sc(sc_) {}
 
-  __device__ __forceinline__ void fetch(int slot, int f) {
-    const double* u = F + (long)f * P;
-#pragma unroll
-    for (int k = 0; k < G::PER; ++k) pf[slot][k] = u[off[k]];
-    // keep the loads here, ahead of the work they overlap
-    __builtin_amdgcn_sched_barrier(0);
-  }
-
-  __device__ __forceinline__ void stage(int f) {
-    const int at = ORDER::pos(f), slot = DEPTH == 1 ? 0 : at % DEPTH;
-#pragma unroll
-    for (int k = 0; k < G::PER; ++k) lds[cel[k]] = pf[slot][k];
-    // The wave's LDS accesses complete in issue order; this only stops the compiler from
-    // moving this field's reads above the writes.
-    __builtin_amdgcn_wave_barrier();
-    if (at + DEPTH < ORDER::N) fetch(slot, ORDER::at(at + DEPTH));
-  }
-
-  __device__ __forceinline__ double center(int f) const { return F[(long)f * P + gc]; }
   __device__ __forceinline__ double val() const { return pin(lds[c]); }
 
   __device__ __forceinline__ static constexpr int stride(int a) {
@@ -287,6 +231,79 @@ struct WaveSrc {
     }
     return pin(acc);
   }
+};
+
+// halo position -> grid index along one axis (clamped, or wrapped over the interior)
+template <int MODE>
+__device__ __forceinline__ int map_axis(int v, int S, int ng) {
+  if (MODE == CLAMP) return v < 0 ? 0 : (v > S - 1 ? S - 1 : v);
+  const int N = S - 2 * ng;
+  int w = (v - ng) % N;
+  w = w < 0 ? w + N : w;
+  return ng + w;
+}
+
+// Source of fields and derivatives for bssn_rhs_point: one wave-private LDS box per field.
+// Staging is pipelined one field ahead: stage(f) writes field f (loaded during the previous
+// stage) into the box, then issues the loads of the next field in ORDER, which complete while
+// field f's derivatives and algebra run (DEPTH fields ahead: DEPTH > 1 needs the stage calls
+// unrolled, so that each slot index is a constant). Contract: a kernel stages its fields exactly in the
+// sequence of its ORDER (RhsOrder for bssn_rhs_point, ConsOrder for the constraints,
+// StateOrder for per-field loops); a different sequence would stage the wrong field.
+template <int ORD, bool KO, int MODE, class ORDER = RhsOrder, int DEPTH = 1>
+struct WaveSrc : BoxStencil<Geo<ORD, KO>, Coef<ORD>> {
+  using G = Geo<ORD, KO>;
+  using C = Coef<ORD>;
+  using BoxStencil<G, C>::lds;
+  using BoxStencil<G, C>::c;
+  const double* F;      // stencil source state
+  long P;
+  int off[G::PER];      // grid offsets of this lane's staged cells
+  int cel[G::PER];      // box indices of this lane's staged cells
+  double pf[DEPTH][G::PER];   // the next DEPTH fields' cells, in flight (slot = position % DEPTH)
+  long gc;              // flat index of the lane's point (clamped into the grid)
+
+  __device__ __forceinline__ WaveSrc(double* lds_, const double* F_, const Scales& sc_,
+                                     const Dims& d, int ox, int oy, int oz, int lane)
+      : BoxStencil<G, C>(lds_, sc_), F(F_), P(d.P) {
+    const int tx = lane / (G::B * G::B), ty = (lane / G::B) % G::B, tz = lane % G::B;
+    c = (tx + G::H) * G::SX + (ty + G::H) * G::SY + (tz + G::H);
+    gc = lane_point<G>(d, ox, oy, oz, lane);
+    const int o[3] = {ox, oy, oz};
+#pragma unroll
+    for (int k = 0; k < G::PER; ++k) {
+      const int e = kCells<G>.c[lane + k * G::NT];
+      const int cc[3] = {e & 31, (e >> 5) & 31, (e >> 10) & 31};
+      cel[k] = cc[0] * G::SX + cc[1] * G::SY + cc[2];
+      int g[3];   // mapped once per lane and slot, not per field
+#pragma unroll
+      for (int a = 0; a < 3; ++a) g[a] = map_axis<MODE>(o[a] + cc[a] - G::H, d.S[a], d.ng);
+      off[k] = (g[0] * d.S[1] + g[1]) * d.S[2] + g[2];
+    }
+#pragma unroll
+    for (int i = 0; i < DEPTH; ++i)
+      if (i < ORDER::N) fetch(i, ORDER::at(i));
+  }
+
+  __device__ __forceinline__ void fetch(int slot, int f) {
+    const double* u = F + (long)f * P;
+#pragma unroll
+    for (int k = 0; k < G::PER; ++k) pf[slot][k] = u[off[k]];
+    // keep the loads here, ahead of the work they overlap
+    __builtin_amdgcn_sched_barrier(0);
+  }
+
+  __device__ __forceinline__ void stage(int f) {
+    const int at = ORDER::pos(f), slot = DEPTH == 1 ? 0 : at % DEPTH;
+#pragma unroll
+    for (int k = 0; k < G::PER; ++k) lds[cel[k]] = pf[slot][k];
+    // The wave's LDS accesses complete in issue order; this only stops the compiler from
+    // moving this field's reads above the writes.
+    __builtin_amdgcn_wave_barrier();
+    if (at + DEPTH < ORDER::N) fetch(slot, ORDER::at(at + DEPTH));
+  }
+
+  __device__ __forceinline__ double center(int f) const { return F[(long)f * P + gc]; }
 };
 
 // Workgroup -> brick, XCD-aware: the dispatcher deals workgroups round-robin over the 8 XCDs,

@@ -38,9 +38,10 @@ __device__ __forceinline__ void enforce_point(double* g, double* a) {
 
 // Waves per SIMD the register allocator targets: 1 gives a wave the whole 512-entry
 // VGPR+AGPR file (no spills, less latency hiding), 2 halves it. Measured at 256^3 (FD4/KO6):
-// the bare RHS is faster at 2, the RK stages (KO + stage update) at 1.
+// the RK stages (KO + stage update, write-back operands preloaded) are faster at 1
+// (43.0 vs 51.7 ms per step), the bare RHS slightly (7.09 vs 7.27 ms).
 #ifndef BSSN_RHS_WAVES
-#define BSSN_RHS_WAVES 2
+#define BSSN_RHS_WAVES 1
 #endif
 #ifndef BSSN_STAGE_WAVES
 #define BSSN_STAGE_WAVES 1
