@@ -85,6 +85,19 @@ def cpu_baseline(seconds: float = 12.0):
                       f"FD4/KO6, oracle/bssn_oracle.c, {el:.1f} s"}
 
 
+def measured_traffic(workload: str):
+    """HBM bytes per stage launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes
+    (each scaled by the same counter on a 1 GiB elementwise copy), or None for another
+    workload."""
+    path = ROOT / "profiles" / "rk4_stage_256_fd4_traffic.json"
+    if not path.exists():
+        return None
+    t = json.loads(path.read_text())
+    if t.get("workload") != workload:
+        return None
+    return t["read_bytes_per_launch"] + t["write_bytes_per_launch"]
+
+
 def _events():
     import torch
     return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -304,7 +317,7 @@ def main() -> int:
     else:
         roof = {"bound": "mfma", "achieved": round(flops_launch / t_launch / 1e12, 3),
                 "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": round(fp64_frac, 4)}
-    roof["traffic"] = None
+    roof["traffic"] = measured_traffic(f"bssn_rk4_step_{N}^3_fp64_bl_binary")
     roof["kernel"] = "k_fused<4,KO,WRAP> (RK4 stage)"
     roof["fp64_frac"] = round(fp64_frac, 4)
     roof["hbm_frac"] = round(hbm_frac, 4)
