@@ -16,7 +16,9 @@ this host's cores on a bounded 48^3 sample.
 Secondary forms (extra keys, same metric family):
   int8_split  the RHS with its 138 derivatives from 7 int8 digits on the INT8 matrix cores
               (Ozaki scheme I) and the FP64 algebra, on the single-puncture construction at
-              256^3 (h = 12/256), with its normwise error against the FP64 fused RHS;
+              256^3 (h = 12/256), with its normwise error against the FP64 fused RHS, and the
+              RK4 step composed from it (per stage: sync, bundle, algebra, KO; full-array
+              updates);
   batched     640 cubic blocks of interior 8..32 on three levels (h = 1/4, 1/8, 1/16) with
               masses 0.909 / 0.0909 4M apart: the RHS of all blocks in one launch and the RK4
               step of all blocks (interiors advanced, ghosts held from the initial data),
@@ -143,9 +145,29 @@ def bench_int8(N: int, reps: int) -> dict:
         den = max(float(ref[f].abs().max()), 1e-12)
         err.append(float((out8[f] - ref[f]).abs().max()) / den)
     npts = N ** 3
-    del D, out8, ref, U
+    del D, out8, ref
+    torch.cuda.empty_cache()
+    # the RK4 step in this form: per stage sync + INT8 bundle + FP64 algebra + KO, full-array
+    # RK4 updates, enforce, sync (the reference's composition)
+    from bssn3d_hip.evolve import BSSNEvolution
+    evo = BSSNEvolution(grid, PhysicsParams(), order=4, ko_sigma=0.1, bc="periodic",
+                        scheme="int8")
+    dt = 0.25 * grid.dx
+    evo.step_(U, 0.0, dt)
+    torch.cuda.synchronize()
+    nstep = max(2, reps // 3)
+    a.record()
+    for i in range(nstep):
+        evo.step_(U, (i + 1) * dt, dt)
+    b.record()
+    torch.cuda.synchronize()
+    ms_step = a.elapsed_time(b) / nstep
+    assert bool(torch.isfinite(U).all()), "non-finite INT8-split state"
+    del evo, U
     return {"workload": f"single_puncture_{N}^3_h{12.0 / N:g}", "slices": 7,
             "rhs_ms": round(ms, 4), "rhs_mpts_s": round(npts / (ms * 1e-3) / 1e6, 2),
+            "step_ms": round(ms_step, 4), "step_mpts_s": round(npts / (ms_step * 1e-3) / 1e6, 2),
+            "steps_timed": nstep,
             "bundle_ms": round(ms_bundle, 4),
             "bundle_hbm_frac": round(138 * 8 * Sx * Sy * Sz / (ms_bundle * 1e-3) / PEAK_HBM, 4),
             "max_normwise_err_vs_fp64_rhs": float("%.3e" % max(err)),

@@ -6,11 +6,16 @@ the reference's constructor, rhs/enforce/step/evolve methods and semantics:
   * step(state, t, dt) = classic RK4 with stage times t, t+dt/2, t+dt/2, t+dt, the CAHD
                           factor on the full dt at every stage, then enforce
   * integrator "rk4_2_2": one RK4 start step, then the 2-step / 3-stage MSRK recurrence
-Each RK stage is ONE fused kernel (RHS + KO + stage update; enforcement folded into the last
-stage). With bc="periodic" stencils read periodically wrapped data, the interior is advanced
-and the ghosts are refilled by a sync at the end of the step, so the returned state equals the
-reference's after the sync its next step would perform. Other bc values advance every padded
-point with edge-padded stencils (the reference leaves such ghosts unsynced too).
+With scheme "fused" each RK stage is ONE fused kernel (RHS + KO + stage update; enforcement
+folded into the last stage). With bc="periodic" stencils read periodically wrapped data, the
+interior is advanced and the ghosts are refilled by a sync at the end of the step, so the
+returned state equals the reference's after the sync its next step would perform. Other bc
+values advance every padded point with edge-padded stencils (the reference leaves such ghosts
+unsynced too).
+With scheme "twopass" or "int8" (the precision form: derivatives from int8 digits) the step is
+the reference's own composition (evolve.py:86-90, 125-132): per stage a periodic sync, the
+materialised 138-derivative bundle, the algebra and KO over the full padded arrays, then the
+RK4 combinations over the full arrays, enforce and (periodic) a final sync.
 """
 from __future__ import annotations
 
@@ -18,7 +23,7 @@ import torch
 
 from . import _lib
 from .grid import Grid
-from .rhs import BSSNSolver, as_tensor, stream_handle
+from .rhs import BSSNSolver, as_tensor, rhs_int8_into, rhs_twopass_into, stream_handle
 from .state import BSSNState, PhysicsParams
 
 INTEGRATORS = ("rk4", "rk4_2_2")
@@ -39,7 +44,10 @@ class BSSNEvolution:
         if integrator not in INTEGRATORS:
             raise ValueError(f"integrator {integrator!r}: only 'rk4' or 'rk4_2_2' are supported")
         self.integrator = integrator
+        if self.solver.scheme != "fused" and integrator != "rk4":
+            raise ValueError(f"the {self.solver.scheme} scheme steps with rk4 only")
         self._work = None
+        self._D = None
 
     @property
     def periodic(self) -> bool:
@@ -87,6 +95,10 @@ class BSSNEvolution:
               tdev: torch.Tensor | None = None) -> torch.Tensor:
         """One RK4 step of U in place. With ``tdev`` (one float64 on U's device) the step
         takes its time from there (``t`` is ignored) and advances it by dt."""
+        if self.solver.scheme != "fused":
+            if tdev is not None:
+                raise ValueError("device-resident time needs the fused scheme")
+            return self._composed_step_(U, t, dt)
         _, Sx, Sy, Sz = U.shape
         work = self._workspace(U, 0)
         lib = _lib.load()
@@ -102,6 +114,50 @@ class BSSNEvolution:
             U.data_ptr(), work.data_ptr(), Sx, Sy, Sz, _lib.dbl(self._S16(0.0, dt)),
             self.order, self.ko_sigma, self.ng, int(self.periodic), tdev.data_ptr(), float(dt),
             self.params.chi_floor, self.params.alpha_floor, stream_handle(U.device)))
+        return U
+
+    # --- the composed step of the two-pass schemes ----------------------------------------
+    def _bundle(self, F: torch.Tensor) -> torch.Tensor:
+        if self._D is None or self._D.shape[1:] != F.shape[1:] or self._D.device != F.device:
+            self._D = torch.empty((138,) + tuple(F.shape[1:]), dtype=torch.float64,
+                                  device=F.device)
+        return self._D
+
+    def _stage_rhs_twopass(self, Y: torch.Tensor, out: torch.Tensor, t: float, dt: float):
+        """f(Y, t): periodic sync of Y's ghosts, then bundle + algebra + KO at every point."""
+        if self.periodic:
+            self.periodic_sync_(Y)
+        S = self._S16(t, dt)
+        D = self._bundle(Y)
+        if self.solver.scheme == "int8":
+            rhs_int8_into(Y, out, D, S, self.order, self.ko_sigma)
+        else:
+            rhs_twopass_into(Y, out, D, S, self.order, self.ko_sigma)
+        return out
+
+    def _composed_step_(self, U: torch.Tensor, t: float, dt: float) -> torch.Tensor:
+        k = torch.empty_like(U)
+        acc = torch.empty_like(U)
+        Y = torch.empty_like(U)
+        h = 0.5 * dt
+        self._stage_rhs_twopass(U, k, t, dt)                     # k1 (syncs U's ghosts)
+        acc.copy_(k)
+        torch.add(U, k, alpha=h, out=Y)
+        self._stage_rhs_twopass(Y, k, t + h, dt)                 # k2
+        acc.add_(k, alpha=2.0)
+        torch.add(U, k, alpha=h, out=Y)
+        self._stage_rhs_twopass(Y, k, t + h, dt)                 # k3
+        acc.add_(k, alpha=2.0)
+        torch.add(U, k, alpha=dt, out=Y)
+        self._stage_rhs_twopass(Y, k, t + dt, dt)                # k4
+        acc.add_(k)
+        U.add_(acc, alpha=dt / 6.0)
+        _, Sx, Sy, Sz = U.shape
+        _lib.check(_lib.load().bssn_enforce_f64(U.data_ptr(), Sx * Sy * Sz,
+                                                self.params.chi_floor, self.params.alpha_floor,
+                                                stream_handle(U.device)))
+        if self.periodic:
+            self.periodic_sync_(U)
         return U
 
     def capture_steps(self, U: torch.Tensor, tdev: torch.Tensor, dt: float,
