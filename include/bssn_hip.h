@@ -107,6 +107,23 @@ int bssn_rk4_step_devtime_f64(double* U, double* work, int Sx, int Sy, int Sz,
                               int periodic, double* tdev, double dt, double chi_floor,
                               double alpha_floor, void* stream);
 
+/* Radiative (Sommerfeld) outer boundary, for puncture runs on a finite domain: the right-hand
+ * side of the ng-deep ghost zones, d_t u = -(x d_x u + y d_y u + z d_z u) / r with the normal
+ * derivative one-sided and the tangential ones centred (edge padded), z face over y over x at
+ * edges. xmin3 = coordinates of the first interior point (x_i = xmin + (i - ng) h).
+ * bssn_sommerfeld_rhs_f64 writes OUT at the ghost points only. bssn_rk4_step_sommerfeld_f64 is
+ * bssn_rk4_step_f64 with the interiors advanced by the fused stages (stencils reading the ghost
+ * values) and the ghosts by this condition, enforce at every point.
+ * The reference has no outer boundary for BSSN (its steps are periodic or leave edge-padded
+ * ghosts, 3D/src/bssn3d/evolve.py:65-73,125-132); this follows its MCS solver's
+ * bc_sommerfeld (3D/src/mcs3d/main.py:230-250). */
+int bssn_sommerfeld_rhs_f64(const double* F, double* OUT, int Sx, int Sy, int Sz, int fd_order,
+                            const double* h3, int ng, const double* xmin3, void* stream);
+int bssn_rk4_step_sommerfeld_f64(double* U, double* work, int Sx, int Sy, int Sz,
+                                 const double* S16, int fd_order, double ko_sigma, int ng,
+                                 const double* xmin3, double t, double dt, double chi_floor,
+                                 double alpha_floor, void* stream);
+
 /* One MSRK rk4_2_2 step of U in place. fprev holds f(y_{n-1}) on entry and is clobbered;
  * fnext receives f(y_n) (swap the two between steps). Replaces BSSNEvolution._msrk2_step
  * (3D/src/bssn3d/evolve.py:139-153). */

@@ -44,6 +44,11 @@ SIGNATURES = {
     "bssn_rk4_step_devtime_f64": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, _c_int, _PD,
                                            _c_int, _c_double, _c_int, _c_int, _c_void_p,
                                            _c_double, _c_double, _c_double, _c_void_p]),
+    "bssn_sommerfeld_rhs_f64": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, _c_int, _c_int,
+                                         _PD, _c_int, _PD, _c_void_p]),
+    "bssn_rk4_step_sommerfeld_f64": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, _c_int, _PD,
+                                              _c_int, _c_double, _c_int, _PD, _c_double,
+                                              _c_double, _c_double, _c_double, _c_void_p]),
     "bssn_msrk_step_f64": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_void_p, _c_int, _c_int,
                                     _c_int, _PD, _c_int, _c_double, _c_int, _c_int, _c_double,
                                     _c_double, _c_double, _c_double, _c_void_p]),

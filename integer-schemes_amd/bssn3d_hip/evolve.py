@@ -11,7 +11,9 @@ folded into the last stage). With bc="periodic" stencils read periodically wrapp
 interior is advanced and the ghosts are refilled by a sync at the end of the step, so the
 returned state equals the reference's after the sync its next step would perform. Other bc
 values advance every padded point with edge-padded stencils (the reference leaves such ghosts
-unsynced too).
+unsynced too). bc="sommerfeld" gives the ghost zones the radiative outgoing-wave condition
+(bssn_rk4_step_sommerfeld_f64; the reference has none for BSSN and this follows its MCS
+solver's, 3D/src/mcs3d/main.py:230-250) while the interiors advance as above.
 With scheme "twopass" or "int8" (the precision form: derivatives from int8 digits) the step is
 the reference's own composition (evolve.py:86-90, 125-132): per stage a periodic sync, the
 materialised 138-derivative bundle, the algebra and KO over the full padded arrays, then the
@@ -46,12 +48,22 @@ class BSSNEvolution:
         self.integrator = integrator
         if self.solver.scheme != "fused" and integrator != "rk4":
             raise ValueError(f"the {self.solver.scheme} scheme steps with rk4 only")
+        if bc == "sommerfeld" and (integrator != "rk4" or self.solver.scheme != "fused"):
+            raise ValueError('bc="sommerfeld" steps with the fused scheme and rk4')
         self._work = None
         self._D = None
 
     @property
     def periodic(self) -> bool:
         return self.bc == "periodic"
+
+    @property
+    def radiative(self) -> bool:
+        return self.bc == "sommerfeld"
+
+    def _xmin3(self):
+        g = self.grid
+        return _lib.dbl([g.xmin, g.ymin, g.zmin])
 
     def _S16(self, t: float, dt: float):
         return self.solver.S16(t, dt)
@@ -72,9 +84,15 @@ class BSSNEvolution:
         out = torch.zeros_like(F)
         _, Sx, Sy, Sz = F.shape
         dt = self.solver.dt if dt is None else dt
-        _lib.check(_lib.load().bssn_stage_rhs_f64(
-            F.data_ptr(), out.data_ptr(), Sx, Sy, Sz, _lib.dbl(self._S16(t, dt)), self.order,
+        lib = _lib.load()
+        S16 = self._S16(t, dt)
+        _lib.check(lib.bssn_stage_rhs_f64(
+            F.data_ptr(), out.data_ptr(), Sx, Sy, Sz, _lib.dbl(S16), self.order,
             self.ko_sigma, self.ng, int(self.periodic), stream_handle(F.device)))
+        if self.radiative:
+            _lib.check(lib.bssn_sommerfeld_rhs_f64(
+                F.data_ptr(), out.data_ptr(), Sx, Sy, Sz, self.order, _lib.dbl(S16[13:16]),
+                self.ng, self._xmin3(), stream_handle(F.device)))
         return BSSNState(out)
 
     def enforce(self, state) -> BSSNState:
@@ -102,6 +120,14 @@ class BSSNEvolution:
         _, Sx, Sy, Sz = U.shape
         work = self._workspace(U, 0)
         lib = _lib.load()
+        if self.radiative:
+            if tdev is not None:
+                raise ValueError("device-resident time is for the periodic and edge steps")
+            _lib.check(lib.bssn_rk4_step_sommerfeld_f64(
+                U.data_ptr(), work.data_ptr(), Sx, Sy, Sz, _lib.dbl(self._S16(t, dt)),
+                self.order, self.ko_sigma, self.ng, self._xmin3(), float(t), float(dt),
+                self.params.chi_floor, self.params.alpha_floor, stream_handle(U.device)))
+            return U
         if tdev is None:
             _lib.check(lib.bssn_rk4_step_f64(
                 U.data_ptr(), work.data_ptr(), Sx, Sy, Sz, _lib.dbl(self._S16(t, dt)),

@@ -298,6 +298,88 @@ int bssn_rk4_step_devtime_f64(double* U, double* work, int Sx, int Sy, int Sz,
   return check_hip(launch_time_advance(tdev, dt, (hipStream_t)stream), "time advance");
 }
 
+int bssn_sommerfeld_rhs_f64(const double* F, double* OUT, int Sx, int Sy, int Sz, int fd_order,
+                            const double* h3, int ng, const double* xmin3, void* stream) {
+  if (!F || !OUT || !xmin3) return fail(BSSN_E_ARG, "null pointer");
+  if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4 or 6");
+  TRY(check_shape(Sx, Sy, Sz));
+  TRY(check_h(h3));
+  TRY(check_ng(Sx, Sy, Sz, ng));
+  for (int q = 0; q < 3; ++q)
+    if (!std::isfinite(xmin3[q])) return fail(BSSN_E_ARG, "non-finite grid origin");
+  BcGeom g;
+  for (int q = 0; q < 3; ++q) {
+    g.xmin[q] = xmin3[q];
+    g.h[q] = h3[q];
+  }
+  StageArgs a{};
+  a.Yin = F;
+  a.Y = OUT;
+  a.kind = KIND_RHS;
+  return check_hip(launch_sommerfeld(a, full_dims(Sx, Sy, Sz, ng), make_scales(h3, fd_order, 0.0),
+                                     g, fd_order, (hipStream_t)stream),
+                   "radiative boundary launch");
+}
+
+int bssn_rk4_step_sommerfeld_f64(double* U, double* work, int Sx, int Sy, int Sz,
+                                 const double* S16, int fd_order, double ko_sigma, int ng,
+                                 const double* xmin3, double t, double dt, double chi_floor,
+                                 double alpha_floor, void* stream) {
+  if (!U || !work || !xmin3) return fail(BSSN_E_ARG, "null pointer");
+  if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4 or 6");
+  if (!(ko_sigma >= 0.0)) return fail(BSSN_E_ARG, "ko_sigma must be >= 0");
+  if (!(dt != 0.0) || !std::isfinite(dt) || !std::isfinite(t))
+    return fail(BSSN_E_ARG, "dt must be finite and nonzero, t finite");
+  TRY(check_shape(Sx, Sy, Sz));
+  TRY(check_s16(S16));
+  TRY(check_ng(Sx, Sy, Sz, ng));
+  const int reach = ko_sigma > 0.0 ? fd_order / 2 + 1 : fd_order / 2;
+  if (ng < reach) return fail(BSSN_E_ARG, "ng must cover the reach of the interior stencils");
+  for (int q = 0; q < 3; ++q)
+    if (!std::isfinite(xmin3[q])) return fail(BSSN_E_ARG, "non-finite grid origin");
+  hipStream_t st = (hipStream_t)stream;
+  const long P = (long)Sx * Sy * Sz;
+  const size_t NS = (size_t)BSSN_NF * P;
+  double* Ya = work;
+  double* Yb = work + NS;
+  double* KS = work + 2 * NS;
+  const double h3[3] = {S16[13], S16[14], S16[15]};
+  const Scales w = make_scales(h3, fd_order, ko_sigma);
+  const Scales w1 = make_scales(h3, fd_order, 0.0);
+  const Dims din = interior_dims(Sx, Sy, Sz, ng);
+  const Dims dall = full_dims(Sx, Sy, Sz, ng);
+  BcGeom g;
+  for (int q = 0; q < 3; ++q) {
+    g.xmin[q] = xmin3[q];
+    g.h[q] = h3[q];
+  }
+  const bool ko = ko_sigma > 0.0;
+  const double hdt = 0.5 * dt;
+  StageArgs a{};
+  a.U = U;
+  a.K1 = KS;
+  a.Uout = U;
+  a.dt = dt;
+  a.chi_floor = chi_floor;
+  a.alpha_floor = alpha_floor;
+  a.ssl_sig = S16[5];
+  // interiors by the fused kernel (clamped stencils read the ghost values as data), ghosts by
+  // the radiative condition; both read the stage input and write disjoint points
+  auto stage = [&](int kind, double c0, double toff, const double* yin, double* y) {
+    a.Yin = yin;
+    a.Y = y;
+    a.kind = kind;
+    a.c0 = c0;
+    a.toff = toff;
+    TRY(run_stage(a, din, w, make_scal(S16, dt, t + toff), fd_order, ko, CLAMP, st));
+    return check_hip(launch_sommerfeld(a, dall, w1, g, fd_order, st), "radiative boundary");
+  };
+  TRY(stage(KIND_RK1, hdt, 0.0, U, Ya));
+  TRY(stage(KIND_RK2, hdt, hdt, Ya, Yb));
+  TRY(stage(KIND_RK3, dt, hdt, Yb, Ya));
+  return stage(KIND_RK4, dt / 6.0, dt, Ya, nullptr);
+}
+
 int bssn_msrk_step_f64(double* U, double* fprev, double* fnext, double* work, int Sx, int Sy,
                        int Sz, const double* S16, int fd_order, double ko_sigma, int ng,
                        int periodic, double t, double dt, double chi_floor, double alpha_floor,

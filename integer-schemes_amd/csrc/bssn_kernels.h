@@ -94,6 +94,14 @@ hipError_t launch_batch_hold(const double* U, double* Ya, double* Yb, const Bloc
                              const BrickList& ghost, int grid, hipStream_t st);
 hipError_t launch_time_advance(double* tdev, double dt, hipStream_t st);
 
+// radiative outer boundary: grid origin (coordinate of the first interior point) and spacings
+struct BcGeom {
+  double xmin[3];
+  double h[3];
+};
+hipError_t launch_sommerfeld(const StageArgs& a, const Dims& d, const Scales& w, const BcGeom& g,
+                             int ord, hipStream_t st);
+
 // two-pass (materialised derivative bundle) path
 hipError_t launch_deriv_bundle(const double* F, double* D, const Dims& d, const Scales& w, int ord,
                                hipStream_t st);
