@@ -91,8 +91,12 @@ size_t bssn_step_workspace_bytes(int integrator, int Sx, int Sy, int Sz);
  * last. periodic = 1: every RHS sees periodically wrapped data (the reference's sync before each
  * RHS), interior points are advanced and the ghosts of U are refilled by a periodic sync at the
  * end. periodic = 0: all padded points are advanced with edge-padded stencils and ghosts are not
- * synced (the reference's bc != "periodic"). S16's dt and t entries are replaced by dt and the
- * stage times. Replaces BSSNEvolution.step (3D/src/bssn3d/evolve.py:125-132). */
+ * synced (the reference's bc != "periodic"). periodic = 2 ("carried" ghosts): each stage input
+ * is synced, every padded point is advanced with edge-padded stencils from it, U keeps its own
+ * ghost values as the base of the update and is not synced at the end -- the reference's
+ * periodic step value for value, ghosts included (evolve.py:86-90,125-132); its interior equals
+ * that of periodic = 1 bit for bit. S16's dt and t entries are replaced by dt and the stage
+ * times. Replaces BSSNEvolution.step (3D/src/bssn3d/evolve.py:125-132). */
 int bssn_rk4_step_f64(double* U, double* work, int Sx, int Sy, int Sz, const double* S16,
                       int fd_order, double ko_sigma, int ng, int periodic, double t, double dt,
                       double chi_floor, double alpha_floor, void* stream);
@@ -125,7 +129,8 @@ int bssn_rk4_step_sommerfeld_f64(double* U, double* work, int Sx, int Sy, int Sz
                                  double alpha_floor, void* stream);
 
 /* One MSRK rk4_2_2 step of U in place. fprev holds f(y_{n-1}) on entry and is clobbered;
- * fnext receives f(y_n) (swap the two between steps). Replaces BSSNEvolution._msrk2_step
+ * fnext receives f(y_n) (swap the two between steps). periodic as for bssn_rk4_step_f64 (with
+ * 2, fprev and fnext hold the ghost rates too). Replaces BSSNEvolution._msrk2_step
  * (3D/src/bssn3d/evolve.py:139-153). */
 int bssn_msrk_step_f64(double* U, double* fprev, double* fnext, double* work, int Sx, int Sy,
                        int Sz, const double* S16, int fd_order, double ko_sigma, int ng,

@@ -18,6 +18,11 @@ With scheme "twopass" or "int8" (the precision form: derivatives from int8 digit
 the reference's own composition (evolve.py:86-90, 125-132): per stage a periodic sync, the
 materialised 138-derivative bundle, the algebra and KO over the full padded arrays, then the
 RK4 combinations over the full arrays, enforce and (periodic) a final sync.
+ghosts="carried" (bc="periodic") keeps the reference's ghost values exactly: each stage input is
+synced, every padded point is advanced with edge-padded stencils from it, the base state keeps
+its own ghosts and nothing is synced at the end. The interior is the same as with the default
+ghosts="synced" (bit for bit, fused scheme); the ghosts, which the reference's constraint norms
+and long-run samples read, are the reference's.
 """
 from __future__ import annotations
 
@@ -29,12 +34,14 @@ from .rhs import BSSNSolver, as_tensor, rhs_int8_into, rhs_twopass_into, stream_
 from .state import BSSNState, PhysicsParams
 
 INTEGRATORS = ("rk4", "rk4_2_2")
+GHOSTS = ("synced", "carried")
 
 
 class BSSNEvolution:
     def __init__(self, grid: Grid, params: PhysicsParams | None = None, order: int = 6,
                  ko_sigma: float = 0.0, bc: str = "periodic", precision: str = "fp64",
-                 scheme: str = "fused", ko_order: int | None = None, integrator: str = "rk4"):
+                 scheme: str = "fused", ko_order: int | None = None, integrator: str = "rk4",
+                 ghosts: str = "synced"):
         self.grid = grid
         self.params = params or PhysicsParams()
         self.solver = BSSNSolver(grid, self.params, order, precision=precision, scheme=scheme,
@@ -50,12 +57,24 @@ class BSSNEvolution:
             raise ValueError(f"the {self.solver.scheme} scheme steps with rk4 only")
         if bc == "sommerfeld" and (integrator != "rk4" or self.solver.scheme != "fused"):
             raise ValueError('bc="sommerfeld" steps with the fused scheme and rk4')
+        if ghosts not in GHOSTS:
+            raise ValueError(f"ghosts {ghosts!r}: use one of {GHOSTS}")
+        self.ghosts = ghosts
         self._work = None
         self._D = None
 
     @property
     def periodic(self) -> bool:
         return self.bc == "periodic"
+
+    @property
+    def carried(self) -> bool:
+        return self.periodic and self.ghosts == "carried"
+
+    @property
+    def _ghost_mode(self) -> int:
+        """The step entry points' ghost argument: 0 edge, 1 wrapped + final sync, 2 carried."""
+        return 2 if self.carried else int(self.periodic)
 
     @property
     def radiative(self) -> bool:
@@ -79,16 +98,21 @@ class BSSNEvolution:
     # --- pieces -----------------------------------------------------------------------------
     def rhs(self, state, t: float = 0.0, dt: float | None = None) -> BSSNState:
         """d/dt state as the step sees it: sync (periodic) + RHS + KO (evolve.py:86-90).
-        Periodic: interior values only (ghost entries of the result are zero)."""
+        Periodic: interior values only (ghost entries of the result are zero); with carried
+        ghosts, the rates at every padded point of the synced state (edge-padded stencils)."""
         F = as_tensor(state)
         out = torch.zeros_like(F)
         _, Sx, Sy, Sz = F.shape
         dt = self.solver.dt if dt is None else dt
         lib = _lib.load()
         S16 = self._S16(t, dt)
+        wrap = self.periodic
+        if self.carried:
+            F = self.periodic_sync_(F.clone())
+            wrap = False
         _lib.check(lib.bssn_stage_rhs_f64(
             F.data_ptr(), out.data_ptr(), Sx, Sy, Sz, _lib.dbl(S16), self.order,
-            self.ko_sigma, self.ng, int(self.periodic), stream_handle(F.device)))
+            self.ko_sigma, self.ng, int(wrap), stream_handle(F.device)))
         if self.radiative:
             _lib.check(lib.bssn_sommerfeld_rhs_f64(
                 F.data_ptr(), out.data_ptr(), Sx, Sy, Sz, self.order, _lib.dbl(S16[13:16]),
@@ -131,14 +155,14 @@ class BSSNEvolution:
         if tdev is None:
             _lib.check(lib.bssn_rk4_step_f64(
                 U.data_ptr(), work.data_ptr(), Sx, Sy, Sz, _lib.dbl(self._S16(t, dt)),
-                self.order, self.ko_sigma, self.ng, int(self.periodic), float(t), float(dt),
+                self.order, self.ko_sigma, self.ng, self._ghost_mode, float(t), float(dt),
                 self.params.chi_floor, self.params.alpha_floor, stream_handle(U.device)))
             return U
         if tdev.dtype != torch.float64 or tdev.numel() != 1 or tdev.device != U.device:
             raise ValueError("tdev must be one float64 value on the state's device")
         _lib.check(lib.bssn_rk4_step_devtime_f64(
             U.data_ptr(), work.data_ptr(), Sx, Sy, Sz, _lib.dbl(self._S16(0.0, dt)),
-            self.order, self.ko_sigma, self.ng, int(self.periodic), tdev.data_ptr(), float(dt),
+            self.order, self.ko_sigma, self.ng, self._ghost_mode, tdev.data_ptr(), float(dt),
             self.params.chi_floor, self.params.alpha_floor, stream_handle(U.device)))
         return U
 
@@ -166,7 +190,11 @@ class BSSNEvolution:
         acc = torch.empty_like(U)
         Y = torch.empty_like(U)
         h = 0.5 * dt
-        self._stage_rhs_twopass(U, k, t, dt)                     # k1 (syncs U's ghosts)
+        if self.carried:                                         # k1 from a synced copy:
+            Y.copy_(U)                                           # U keeps its own ghosts
+            self._stage_rhs_twopass(Y, k, t, dt)
+        else:
+            self._stage_rhs_twopass(U, k, t, dt)                 # k1 (syncs U's ghosts)
         acc.copy_(k)
         torch.add(U, k, alpha=h, out=Y)
         self._stage_rhs_twopass(Y, k, t + h, dt)                 # k2
@@ -182,7 +210,7 @@ class BSSNEvolution:
         _lib.check(_lib.load().bssn_enforce_f64(U.data_ptr(), Sx * Sy * Sz,
                                                 self.params.chi_floor, self.params.alpha_floor,
                                                 stream_handle(U.device)))
-        if self.periodic:
+        if self.periodic and not self.carried:
             self.periodic_sync_(U)
         return U
 
@@ -212,7 +240,7 @@ class BSSNEvolution:
         work = self._workspace(U, 1)
         _lib.check(_lib.load().bssn_msrk_step_f64(
             U.data_ptr(), fprev.data_ptr(), fnext.data_ptr(), work.data_ptr(), Sx, Sy, Sz,
-            _lib.dbl(self._S16(t, dt)), self.order, self.ko_sigma, self.ng, int(self.periodic),
+            _lib.dbl(self._S16(t, dt)), self.order, self.ko_sigma, self.ng, self._ghost_mode,
             float(t), float(dt), self.params.chi_floor, self.params.alpha_floor,
             stream_handle(U.device)))
         return U

@@ -1,6 +1,7 @@
 """Initial data for the BSSN workloads (torch tensors, any device).
 
-* minkowski / gauge_wave: the reference's test data (3D/src/bssn3d/initial_data.py:28-137).
+* minkowski / gauge_wave / hamiltonian_bump / robust_stability: the reference's test data
+  (3D/src/bssn3d/initial_data.py:28-137).
 * punctures: Brill-Lindquist punctures with seeded sinusoid perturbations, the workload data
   of the benchmark configurations. The reference has no puncture data
   (initial_data.py:1: "NOT BBH ID"), so this construction is defined here:
@@ -72,6 +73,26 @@ def gauge_wave(grid: Grid, amplitude: float = 0.01, wavelength: float | None = N
         d[at_diag[a]] = At_wave if a == axis else At_trans
     d[GTILDE0 + axis] = Gt_wave
     return d
+
+
+def hamiltonian_bump(grid: Grid, amplitude: float = 0.01, device="cpu") -> torch.Tensor:
+    """Minkowski with chi + A cos(2 pi x) cos(2 pi y) cos(2 pi z) (initial_data.py:33-55): a
+    smooth, coherent Hamiltonian-constraint violation with no momentum violation, periodic on
+    [-1/2, 1/2]^3."""
+    X, Y, Z = _coords(grid, device)
+    twopi = 2.0 * math.pi
+    d = minkowski(grid, device)
+    d[CHI] += amplitude * torch.cos(twopi * X) * torch.cos(twopi * Y) * torch.cos(twopi * Z)
+    return d
+
+
+def robust_stability(grid: Grid, amp: float = 1.0e-10, seed: int = 0,
+                     device="cpu") -> torch.Tensor:
+    """Minkowski + uniform noise in [-amp, amp) on all 24 fields (initial_data.py:58-69).
+    The noise is drawn from numpy's default_rng(seed) (field-major, C order), not from JAX's
+    PRNG stream; the property that matters (bounded round-off-scale noise) is the same."""
+    noise = np.random.default_rng(seed).uniform(-amp, amp, size=(NUM_VARS,) + grid.shape)
+    return minkowski(grid, device) + torch.from_numpy(noise).to(device)
 
 
 @dataclass

@@ -112,6 +112,28 @@ int run_stage(const StageArgs& a, const Dims& d, const Scales& w, const Scal& sc
   return check_hip(launch_fused(a, d, w, sc, ord, ko, mode, st), "fused stage launch");
 }
 
+// Ghost handling of a step (the "periodic" argument of the step entry points):
+//   0 = edge: every padded point advanced with edge-padded stencils, no sync;
+//   1 = wrapped: interior advanced with periodically wrapped stencils, ghosts synced at the end;
+//   2 = carried: each stage input is synced, every padded point is advanced with edge-padded
+//       stencils from it, the base state keeps its own ghost values and nothing is synced at
+//       the end -- the reference's periodic step exactly (3D/src/bssn3d/evolve.py:86-90,
+//       125-132: rhs() syncs a copy, the RK combinations run over the whole padded arrays).
+// The interiors of 1 and 2 are identical (the same stencil values in the same order).
+enum GhostMode { GH_EDGE = 0, GH_WRAP = 1, GH_CARRY = 2 };
+
+// stage input for a carried-ghost step: Yin synced in place (a stage buffer), or for the first
+// stage a synced copy of the base state in `spare`
+int carried_input(const double*& yin, double* spare, size_t NS, const Dims& full,
+                  hipStream_t st) {
+  if (spare) {
+    TRY(check_hip(hipMemcpyAsync(spare, yin, NS * sizeof(double), hipMemcpyDeviceToDevice, st),
+                  "stage input copy"));
+    yin = spare;
+  }
+  return check_hip(launch_periodic_sync(const_cast<double*>(yin), full, st), "stage sync");
+}
+
 // RK4 step; with tdev set, every stage reads its time from device memory (*tdev + offset)
 int rk4_step(double* U, double* work, int Sx, int Sy, int Sz, const double* S16, int fd_order,
              double ko_sigma, int ng, int periodic, double t, const double* tdev, double dt,
@@ -119,6 +141,7 @@ int rk4_step(double* U, double* work, int Sx, int Sy, int Sz, const double* S16,
   if (!U || !work) return fail(BSSN_E_ARG, "null pointer");
   if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4 or 6");
   if (!(ko_sigma >= 0.0)) return fail(BSSN_E_ARG, "ko_sigma must be >= 0");
+  if (periodic < 0 || periodic > 2) return fail(BSSN_E_ARG, "periodic must be 0, 1 or 2");
   if (!(dt != 0.0) || !std::isfinite(dt) || !std::isfinite(t))
     return fail(BSSN_E_ARG, "dt must be finite and nonzero, t finite");
   TRY(check_shape(Sx, Sy, Sz));
@@ -132,8 +155,10 @@ int rk4_step(double* U, double* work, int Sx, int Sy, int Sz, const double* S16,
   double* KS = work + 2 * NS;
   const double h3[3] = {S16[13], S16[14], S16[15]};
   const Scales w = make_scales(h3, fd_order, ko_sigma);
-  const Dims d = periodic ? interior_dims(Sx, Sy, Sz, ng) : full_dims(Sx, Sy, Sz, ng);
-  const int mode = periodic ? WRAP : CLAMP;
+  const bool wrap = periodic == GH_WRAP, carry = periodic == GH_CARRY && ng > 0;
+  const Dims full = full_dims(Sx, Sy, Sz, ng);
+  const Dims d = wrap ? interior_dims(Sx, Sy, Sz, ng) : full;
+  const int mode = wrap ? WRAP : CLAMP;
   const bool ko = ko_sigma > 0.0;
   const double hdt = 0.5 * dt;
   StageArgs a{};
@@ -145,20 +170,21 @@ int rk4_step(double* U, double* work, int Sx, int Sy, int Sz, const double* S16,
   a.alpha_floor = alpha_floor;
   a.tdev = tdev;
   a.ssl_sig = S16[5];
-  // k1 = f(y, t);          Y = y + (dt/2) k1
-  a.Yin = U; a.Y = Ya; a.kind = KIND_RK1; a.c0 = hdt; a.toff = 0.0;
-  TRY(run_stage(a, d, w, make_scal(S16, dt, t), fd_order, ko, mode, st));
+  auto stage = [&](int kind, double c0, double toff, const double* yin, double* y, double* spare) {
+    if (carry) TRY(carried_input(yin, spare, NS, full, st));
+    a.Yin = yin; a.Y = y; a.kind = kind; a.c0 = c0; a.toff = toff;
+    return run_stage(a, d, w, make_scal(S16, dt, t + toff), fd_order, ko, mode, st);
+  };
+  // k1 = f(y, t);          Y = y + (dt/2) k1    (carried: the synced copy of y lives in Yb)
+  TRY(stage(KIND_RK1, hdt, 0.0, U, Ya, Yb));
   // k2 = f(Y, t + dt/2);   Y = y + (dt/2) k2
-  a.Yin = Ya; a.Y = Yb; a.kind = KIND_RK2; a.c0 = hdt; a.toff = hdt;
-  TRY(run_stage(a, d, w, make_scal(S16, dt, t + hdt), fd_order, ko, mode, st));
+  TRY(stage(KIND_RK2, hdt, hdt, Ya, Yb, nullptr));
   // k3 = f(Y, t + dt/2);   Y = y + dt k3
-  a.Yin = Yb; a.Y = Ya; a.kind = KIND_RK3; a.c0 = dt; a.toff = hdt;
-  TRY(run_stage(a, d, w, make_scal(S16, dt, t + hdt), fd_order, ko, mode, st));
+  TRY(stage(KIND_RK3, dt, hdt, Yb, Ya, nullptr));
   // k4 = f(Y, t + dt);     y + (dt/6)(k1 + 2k2 + 2k3 + k4), then enforce
-  a.Yin = Ya; a.Y = nullptr; a.kind = KIND_RK4; a.c0 = dt / 6.0; a.toff = dt;
-  TRY(run_stage(a, d, w, make_scal(S16, dt, t + dt), fd_order, ko, mode, st));
-  if (periodic && ng > 0)
-    return check_hip(launch_periodic_sync(U, full_dims(Sx, Sy, Sz, ng), st), "periodic sync");
+  TRY(stage(KIND_RK4, dt / 6.0, dt, Ya, nullptr, nullptr));
+  if (wrap && ng > 0)
+    return check_hip(launch_periodic_sync(U, full, st), "periodic sync");
   return BSSN_OK;
 }
 
@@ -388,6 +414,7 @@ int bssn_msrk_step_f64(double* U, double* fprev, double* fnext, double* work, in
   if (fprev == fnext) return fail(BSSN_E_ARG, "fprev and fnext must be distinct buffers");
   if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4 or 6");
   if (!(ko_sigma >= 0.0)) return fail(BSSN_E_ARG, "ko_sigma must be >= 0");
+  if (periodic < 0 || periodic > 2) return fail(BSSN_E_ARG, "periodic must be 0, 1 or 2");
   if (!(dt != 0.0) || !std::isfinite(dt) || !std::isfinite(t))
     return fail(BSSN_E_ARG, "dt must be finite and nonzero, t finite");
   TRY(check_shape(Sx, Sy, Sz));
@@ -400,8 +427,10 @@ int bssn_msrk_step_f64(double* U, double* fprev, double* fnext, double* work, in
   double* Yb = work + NS;
   const double h3[3] = {S16[13], S16[14], S16[15]};
   const Scales w = make_scales(h3, fd_order, ko_sigma);
-  const Dims d = periodic ? interior_dims(Sx, Sy, Sz, ng) : full_dims(Sx, Sy, Sz, ng);
-  const int mode = periodic ? WRAP : CLAMP;
+  const bool wrap = periodic == GH_WRAP, carry = periodic == GH_CARRY && ng > 0;
+  const Dims full = full_dims(Sx, Sy, Sz, ng);
+  const Dims d = wrap ? interior_dims(Sx, Sy, Sz, ng) : full;
+  const int mode = wrap ? WRAP : CLAMP;
   const bool ko = ko_sigma > 0.0;
   const double c2 = MS_A20 + MS_A21;
   const double c3 = MS_A30 + MS_A31 + MS_A32;
@@ -413,17 +442,21 @@ int bssn_msrk_step_f64(double* U, double* fprev, double* fnext, double* work, in
   a.dt = dt;
   a.chi_floor = chi_floor;
   a.alpha_floor = alpha_floor;
-  // k1 = f(y, t);            Y2 = y + dt (a20 k0 + a21 k1)
-  a.Yin = U; a.Y = Ya; a.kind = KIND_MS1; a.c0 = MS_A20; a.c1 = MS_A21;
-  TRY(run_stage(a, d, w, make_scal(S16, dt, t), fd_order, ko, mode, st));
+  auto stage = [&](int kind, double toff, const double* yin, double* y, double* spare) {
+    if (carry) TRY(carried_input(yin, spare, NS, full, st));
+    a.Yin = yin; a.Y = y; a.kind = kind;
+    return run_stage(a, d, w, make_scal(S16, dt, t + toff), fd_order, ko, mode, st);
+  };
+  // k1 = f(y, t);            Y2 = y + dt (a20 k0 + a21 k1)   (carried: synced y in Yb)
+  a.c0 = MS_A20; a.c1 = MS_A21;
+  TRY(stage(KIND_MS1, 0.0, U, Ya, Yb));
   // k2 = f(Y2, t + c2 dt);   Y3 = y + dt (a30 k0 + a31 k1 + a32 k2)
-  a.Yin = Ya; a.Y = Yb; a.kind = KIND_MS2; a.c0 = MS_A30; a.c1 = MS_A31; a.c2 = MS_A32;
-  TRY(run_stage(a, d, w, make_scal(S16, dt, t + c2 * dt), fd_order, ko, mode, st));
+  a.c0 = MS_A30; a.c1 = MS_A31; a.c2 = MS_A32;
+  TRY(stage(KIND_MS2, c2 * dt, Ya, Yb, nullptr));
   // k3 = f(Y3, t + c3 dt);   y + dt (b0 k0 + b1 k1 + b2 k2 + b3 k3), then enforce
-  a.Yin = Yb; a.Y = nullptr; a.kind = KIND_MS3;
-  TRY(run_stage(a, d, w, make_scal(S16, dt, t + c3 * dt), fd_order, ko, mode, st));
-  if (periodic && ng > 0)
-    return check_hip(launch_periodic_sync(U, full_dims(Sx, Sy, Sz, ng), st), "periodic sync");
+  TRY(stage(KIND_MS3, c3 * dt, Yb, nullptr, nullptr));
+  if (wrap && ng > 0)
+    return check_hip(launch_periodic_sync(U, full, st), "periodic sync");
   return BSSN_OK;
 }
 
