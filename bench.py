@@ -1,7 +1,7 @@
 """Benchmark: the fused BSSN RHS + RK4 step at 256^3 FP64 on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--no-cpu] [--no-twopass]
-                    [--no-int8] [--no-batch]
+                    [--no-int8] [--no-batch] [--no-longrun]
 
 A step is one full RK4 step (four fused stage kernels: RHS + Kreiss-Oliger + stage update,
 enforcement folded into the last; then the periodic ghost refill) of the 256^3-interior,
@@ -23,7 +23,11 @@ Secondary forms (extra keys, same metric family):
               masses 0.909 / 0.0909 4M apart: the RHS of all blocks in one launch and the RK4
               step of all blocks (interiors advanced, ghosts held from the initial data),
               launch shape autotuned, eager and replayed from a captured graph; Mpts/s over
-              interior points.
+              interior points;
+  longrun     the apples gate (fused and INT8-split schemes against the two-pass scheme on
+              the reference's three N=24 configurations) with its verdicts and wall time, and
+              a 128^3 long run with the reference's carried ghost values; the carried-ghost
+              RK4 step at 256^3 is reported beside the default one (carried_ghost_step_ms).
 
 Rank 0 prints one JSON line.
 """
@@ -229,6 +233,39 @@ def bench_batched(reps: int) -> dict:
             "graph_step_mpts_s": round(pts / (ms_graph * 1e-3) / 1e6, 2)}
 
 
+def bench_longrun() -> dict:
+    """The apples gate on the reference's suite (robust / gauge / hambump, N=24, 2 crossing
+    times, 8 samples each; fused against the two-pass scheme at FD order 6 and the INT8-split
+    scheme at FD order 4), and a long run at 128^3 (robust noise 1e-8, 1 crossing time = 512
+    RK4 steps, 4 samples) with the reference's carried ghosts."""
+    import torch
+    from bssn3d_hip import apples_gate as ag
+    from bssn3d_hip.longrun import run
+
+    out = {}
+    for name, scheme, order in (("gate_fused_vs_twopass_fd6", "fused", 6),
+                                ("gate_int8_vs_twopass_fd4", "int8", 4)):
+        res = []
+        w0 = time.perf_counter()
+        ok = ag.gate(scheme, order=order, verbose=False, results=res, device="cuda:0")
+        torch.cuda.synchronize()
+        out[name] = {"pass": bool(ok), "wall_s": round(time.perf_counter() - w0, 3),
+                     "worst_dev": float("%.3e" % max(max(r["dev"].values()) for r in res))}
+    N = 128
+    run("robust", N=N, crossings=0.05, samples=1, ko_sigma=0.1, amp=1e-8, device="cuda:0")
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    rows = run("robust", N=N, crossings=1.0, samples=4, ko_sigma=0.1, amp=1e-8, device="cuda:0")
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    steps = rows[-1]["steps"]
+    out["longrun_robust_128^3"] = {
+        "steps": steps, "samples": len(rows), "wall_s": round(wall, 3),
+        "steps_per_s": round(steps / wall, 2), "mpts_s": round(steps * N ** 3 / wall / 1e6, 2),
+        "final_H": float("%.4e" % rows[-1]["H"]), "finite": all(r["finite"] for r in rows)}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -239,6 +276,7 @@ def main() -> int:
     ap.add_argument("--no-twopass", action="store_true")
     ap.add_argument("--no-int8", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-longrun", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="also run one plain device copy of this many GiB (profiler calibration)")
@@ -299,6 +337,21 @@ def main() -> int:
     torch.cuda.synchronize()
     ms_sync = s0.elapsed_time(s1) / reps
     ms_stage = (ms_step - ms_sync) / 4.0
+
+    # the same step keeping the reference's carried ghost values (per stage: sync of the stage
+    # input, every padded point advanced), on a copy of the state
+    evo_c = BSSNEvolution(grid, p, order=4, ko_sigma=0.1, bc="periodic", integrator="rk4",
+                          ghosts="carried")
+    Uc = U.clone()
+    evo_c.step_(Uc, t, dt)
+    c0, c1 = ev(), ev()
+    c0.record()
+    for i in range(reps):
+        evo_c.step_(Uc, t + i * dt, dt)
+    c1.record()
+    torch.cuda.synchronize()
+    ms_carried = c0.elapsed_time(c1) / reps
+    del Uc, evo_c
 
     # ---------------------------------------------------------------- bare fused RHS
     S16 = scalars(p, grid.dx, grid.dy, grid.dz, dt, t)
@@ -375,6 +428,7 @@ def main() -> int:
         "rhs_ms": round(ms_rhs, 4),
         "rhs_mpts_s": round(npts / (ms_rhs * 1e-3) / 1e6, 2),
         "sync_ms": round(ms_sync, 4),
+        "carried_ghost_step_ms": round(ms_carried, 4),
         "roofline": roof,
     }
     if ms_twopass is not None:
@@ -388,6 +442,9 @@ def main() -> int:
         torch.cuda.empty_cache()
     if not args.no_batch:
         res["batched"] = bench_batched(reps)
+        torch.cuda.empty_cache()
+    if not args.no_longrun:
+        res["longrun"] = bench_longrun()
         torch.cuda.empty_cache()
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
