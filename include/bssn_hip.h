@@ -175,6 +175,33 @@ typedef struct {
   long long dst_stride, src_stride;
 } bssn_halo_copy;
 
+/* One point of a node-centred 2:1 prolongation from a parent block (the tensor product, x then
+ * y then z, of the 1D refinement fine[2k] = coarse[k], fine[2k+1] = sum_j w_j coarse[k + j],
+ * j = -2..3, w = (3, -25, 150, 150, -25, 3)/256, parent reads clamped to its padded block:
+ * 3D/src/multipatch/amr/kernels.py:53-108). dst: pool offset of the destination point (field
+ * 0); parent: pool offset of the parent block's first padded point (field 0); dst_stride:
+ * field stride of the destination block; fine: the index in the 2:1 refinement of the parent's
+ * padded block, per axis (0 <= fine < 2 pS); pS: the parent's padded extents (its field stride
+ * is pS0 pS1 pS2). */
+typedef struct {
+  long long dst, parent, dst_stride;
+  int fine[3];
+  int pS[3];
+} bssn_prolong_point;
+
+/* A ghost fill program for a block hierarchy: phases run in order; phase p applies prolongs
+ * [prolong_end[p-1], prolong_end[p]) and copies [copy_end[p-1], copy_end[p]) (ends cumulative,
+ * host arrays of nphase entries; the entry arrays are device memory). Within a phase the
+ * destinations must be disjoint and no entry may read a point the phase writes; a phase may
+ * read what earlier phases wrote (a level's prolongation reads its parent level's ghosts). */
+typedef struct {
+  int nphase;
+  const long long* prolong_end;
+  const long long* copy_end;
+  const bssn_prolong_point* prolongs;
+  const bssn_halo_copy* copies;
+} bssn_ghost_program;
+
 /* Plan size for nblk blocks of padded extents S (3 ints per block); 0 for invalid extents. */
 size_t bssn_batch_plan_bytes(int nblk, const int* S, int ng);
 
@@ -222,6 +249,32 @@ int bssn_batch_rk4_step_f64(const bssn_batch* b, double* U, double* work, const 
                             int fd_order, double ko_sigma, int bc, const bssn_halo_copy* halo,
                             long long nhalo, double t, double dt, double chi_floor,
                             double alpha_floor, double* tdev, void* stream);
+
+/* Prolongations of n points (device array) into a pool: the reference's prolongate / the halo
+ * fill of sync_across_levels (3D/src/multipatch/amr/kernels.py:66-108, sync.py:84-97). */
+int bssn_prolong_f64(double* pool, const bssn_prolong_point* pts, long long n, void* stream);
+
+/* Run a ghost fill program on a pool: the hierarchy sync of the reference's N-level step,
+ * root level periodic, then each finer level from its parent (prolongation) and from
+ * same-level face neighbours (2D/src/mcs2d/amr/evolve.py:719-735,
+ * 3D/src/multipatch/amr/sync.py:84-122). */
+int bssn_ghost_fill_f64(double* pool, const bssn_ghost_program* prog, void* stream);
+
+/* bssn_batch_rk4_step_f64 with bc = 2 whose ghost refill after every stage and at the end is a
+ * ghost fill program: the reference's shared-dt N-level RK4 step over a block hierarchy
+ * (2D/src/mcs2d/amr/evolve.py:659-775), with enforcement of the BSSN step folded into the last
+ * stage. U must be synced on entry (as every step leaves it). */
+int bssn_batch_rk4_step_prog_f64(const bssn_batch* b, double* U, double* work, const double* S16,
+                                 int fd_order, double ko_sigma, const bssn_ghost_program* prog,
+                                 double t, double dt, double chi_floor, double alpha_floor,
+                                 double* tdev, void* stream);
+
+/* Refinement indicator of every block of a batch: out[24 b + f] = the largest undivided
+ * difference |u(p + e_a) - u(p)| of field f over the block's interior and one ring of ghosts,
+ * all three axes (NaN if any is NaN); blocks in the order they were given to
+ * bssn_batch_create. Replaces compute_indicator_gradient (3D/src/multipatch/amr/kernels.py:
+ * 143-158), per field. Needs ng >= 1. */
+int bssn_batch_indicator_f64(const bssn_batch* b, const double* U, double* out, void* stream);
 
 #ifdef __cplusplus
 }

@@ -58,15 +58,23 @@ class BlockLayout:
     """Pool geometry of a set of blocks (host only).
 
     Block b occupies pool[off[b] : off[b] + 24 P_b] as a (24, S0, S1, S2) state, S = n + 2 ng;
-    offsets are aligned to ``align`` doubles."""
+    offsets are aligned to ``align`` doubles, or given (``offsets`` with ``pool_doubles``: a
+    hierarchy whose blocks live in fixed slots)."""
 
-    def __init__(self, blocks: Sequence[Block], ng: int = 3, align: int = 64):
+    def __init__(self, blocks: Sequence[Block], ng: int = 3, align: int = 64,
+                 offsets: Sequence[int] | None = None, pool_doubles: int | None = None):
         if not blocks:
             raise ValueError("a batch needs at least one block")
         self.blocks = list(blocks)
         self.ng = int(ng)
         self.S = [tuple(int(v) + 2 * self.ng for v in b.n) for b in self.blocks]
         self.P = [s[0] * s[1] * s[2] for s in self.S]
+        if offsets is not None:
+            if pool_doubles is None or len(offsets) != len(self.blocks):
+                raise ValueError("explicit offsets need one per block and pool_doubles")
+            self.off = [int(o) for o in offsets]
+            self.pool_doubles = int(pool_doubles)
+            return
         off, pos = [], 0
         for P in self.P:
             off.append(pos)
@@ -86,8 +94,9 @@ class BlockLayout:
 class BlockBatch(BlockLayout):
     """A pool of blocks and its device plan (block table and brick lists)."""
 
-    def __init__(self, blocks: Sequence[Block], ng: int = 3, device="cuda", align: int = 64):
-        super().__init__(blocks, ng, align)
+    def __init__(self, blocks: Sequence[Block], ng: int = 3, device="cuda", align: int = 64,
+                 offsets: Sequence[int] | None = None, pool_doubles: int | None = None):
+        super().__init__(blocks, ng, align, offsets, pool_doubles)
         self.device = torch.device(device)
         off = self.off
         n = len(self.blocks)

@@ -23,7 +23,8 @@ ARCH = os.environ.get("BSSN_OFFLOAD_ARCH", "gfx950")
 
 # largest compile units first (the fused stage kernels are instantiated per kind)
 SOURCES = ["bssn_fused4.hip", "bssn_fused6.hip", "bssn_batch4.hip", "bssn_batch6.hip",
-           "bssn_aux.hip", "bssn_batch.hip", "bssn_i8.hip", "bssn_bc.hip", "bssn_abi.cpp"]
+           "bssn_aux.hip", "bssn_batch.hip", "bssn_i8.hip", "bssn_bc.hip", "bssn_amr.hip",
+           "bssn_abi.cpp"]
 
 # Packed-FP32 instructions are disabled for every HIP compile (they are not used by this
 # FP64 library; the flag keeps hipcc from forming them from any float code).

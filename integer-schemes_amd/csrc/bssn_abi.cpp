@@ -804,21 +804,22 @@ int bssn_batch_halo_f64(double* pool, const bssn_halo_copy* map, long long n, vo
   return check_hip(launch_halo_copy(pool, map, (long)n, (hipStream_t)stream), "halo copy");
 }
 
-int bssn_batch_rk4_step_f64(const bssn_batch* b, double* U, double* work, const double* S16,
-                            int fd_order, double ko_sigma, int bc, const bssn_halo_copy* halo,
-                            long long nhalo, double t, double dt, double chi_floor,
-                            double alpha_floor, double* tdev, void* stream) {
+}  // extern "C"
+
+namespace {
+
+// One RK4 step of a batch; bc = 2 refills ghosts with `sync` after every stage and at the end.
+template <class Sync>
+int batch_rk4(const bssn_batch* b, double* U, double* work, const double* S16, int fd_order,
+              double ko_sigma, int bc, Sync sync, double t, double dt, double chi_floor,
+              double alpha_floor, double* tdev, hipStream_t st) {
   TRY(check_batch(b));
   if (!U || !work) return fail(BSSN_E_ARG, "null pointer");
   if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4 or 6");
   if (!(ko_sigma >= 0.0)) return fail(BSSN_E_ARG, "ko_sigma must be >= 0");
   if (!(dt != 0.0) || !std::isfinite(dt) || !std::isfinite(t))
     return fail(BSSN_E_ARG, "dt must be finite and nonzero, t finite");
-  if (bc < 0 || bc > 2) return fail(BSSN_E_ARG, "bc must be 0 (edge), 1 (periodic) or 2 (halo)");
-  if (bc == 2 && (nhalo < 0 || (nhalo > 0 && !halo)))
-    return fail(BSSN_E_ARG, "bc = 2 needs a halo map");
   TRY(check_s16_batch(S16));
-  hipStream_t st = (hipStream_t)stream;
   const size_t NS = (size_t)b->pool_doubles;
   double* Ya = work;
   double* Yb = work + NS;
@@ -844,7 +845,7 @@ int bssn_batch_rk4_step_f64(const bssn_batch* b, double* U, double* work, const 
     s.kind = kind;
     s.c0 = c0;
     TRY(run_batch(b, s, a, fd_order, ko, mode, st));
-    if (bc == 2 && y) TRY(bssn_batch_halo_f64(y, halo, nhalo, stream));
+    if (bc == 2 && y) TRY(sync(y));
     return BSSN_OK;
   };
   if (bc == 2 && b->ng > 0)   // stage inputs start with U's ghosts
@@ -854,10 +855,89 @@ int bssn_batch_rk4_step_f64(const bssn_batch* b, double* U, double* work, const 
   TRY(stage(hdt, KIND_RK2, hdt, Ya, Yb));
   TRY(stage(hdt, KIND_RK3, dt, Yb, Ya));
   TRY(stage(dt, KIND_RK4, dt / 6.0, Ya, nullptr));
-  if (bc == 1) TRY(bssn_batch_periodic_sync_f64(b, U, stream));
-  if (bc == 2) TRY(bssn_batch_halo_f64(U, halo, nhalo, stream));
+  if (bc == 1) TRY(bssn_batch_periodic_sync_f64(b, U, st));
+  if (bc == 2) TRY(sync(U));
   if (tdev) TRY(check_hip(launch_time_advance(tdev, dt, st), "time advance"));
   return BSSN_OK;
+}
+
+int check_program(const bssn_ghost_program* p) {
+  if (!p) return fail(BSSN_E_ARG, "null ghost program");
+  if (p->nphase < 0 || (p->nphase > 0 && (!p->prolong_end || !p->copy_end)))
+    return fail(BSSN_E_ARG, "bad ghost program phase table");
+  long long pe = 0, ce = 0;
+  for (int i = 0; i < p->nphase; ++i) {
+    if (p->prolong_end[i] < pe || p->copy_end[i] < ce)
+      return fail(BSSN_E_ARG, "ghost program phase ends must not decrease");
+    pe = p->prolong_end[i];
+    ce = p->copy_end[i];
+  }
+  if ((pe > 0 && !p->prolongs) || (ce > 0 && !p->copies))
+    return fail(BSSN_E_ARG, "ghost program entries missing");
+  return BSSN_OK;
+}
+
+int run_program(double* pool, const bssn_ghost_program* p, hipStream_t st) {
+  long long pb = 0, cb = 0;
+  for (int i = 0; i < p->nphase; ++i) {
+    const long long pe = p->prolong_end[i], ce = p->copy_end[i];
+    if (pe > pb)
+      TRY(check_hip(launch_prolong(pool, p->prolongs + pb, (long)(pe - pb), st), "prolongation"));
+    if (ce > cb)
+      TRY(check_hip(launch_halo_copy(pool, p->copies + cb, (long)(ce - cb), st), "ghost copies"));
+    pb = pe;
+    cb = ce;
+  }
+  return BSSN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bssn_batch_rk4_step_f64(const bssn_batch* b, double* U, double* work, const double* S16,
+                            int fd_order, double ko_sigma, int bc, const bssn_halo_copy* halo,
+                            long long nhalo, double t, double dt, double chi_floor,
+                            double alpha_floor, double* tdev, void* stream) {
+  if (bc < 0 || bc > 2) return fail(BSSN_E_ARG, "bc must be 0 (edge), 1 (periodic) or 2 (halo)");
+  if (bc == 2 && (nhalo < 0 || (nhalo > 0 && !halo)))
+    return fail(BSSN_E_ARG, "bc = 2 needs a halo map");
+  hipStream_t st = (hipStream_t)stream;
+  auto sync = [&](double* pool) {
+    return check_hip(launch_halo_copy(pool, halo, (long)nhalo, st), "halo copy");
+  };
+  return batch_rk4(b, U, work, S16, fd_order, ko_sigma, bc, sync, t, dt, chi_floor, alpha_floor,
+                   tdev, st);
+}
+
+int bssn_prolong_f64(double* pool, const bssn_prolong_point* pts, long long n, void* stream) {
+  if (!pool) return fail(BSSN_E_ARG, "null pool pointer");
+  if (n < 0 || (n > 0 && !pts)) return fail(BSSN_E_ARG, "bad prolongation list");
+  return check_hip(launch_prolong(pool, pts, (long)n, (hipStream_t)stream), "prolongation");
+}
+
+int bssn_ghost_fill_f64(double* pool, const bssn_ghost_program* prog, void* stream) {
+  if (!pool) return fail(BSSN_E_ARG, "null pool pointer");
+  TRY(check_program(prog));
+  return run_program(pool, prog, (hipStream_t)stream);
+}
+
+int bssn_batch_rk4_step_prog_f64(const bssn_batch* b, double* U, double* work, const double* S16,
+                                 int fd_order, double ko_sigma, const bssn_ghost_program* prog,
+                                 double t, double dt, double chi_floor, double alpha_floor,
+                                 double* tdev, void* stream) {
+  TRY(check_program(prog));
+  hipStream_t st = (hipStream_t)stream;
+  auto sync = [&](double* pool) { return run_program(pool, prog, st); };
+  return batch_rk4(b, U, work, S16, fd_order, ko_sigma, 2, sync, t, dt, chi_floor, alpha_floor,
+                   tdev, st);
+}
+
+int bssn_batch_indicator_f64(const bssn_batch* b, const double* U, double* out, void* stream) {
+  TRY(check_batch(b));
+  if (!U || !out) return fail(BSSN_E_ARG, "null pointer");
+  if (b->ng < 1) return fail(BSSN_E_ARG, "the indicator needs ng >= 1");
+  return check_hip(launch_indicator(U, b->blk, b->nblk, out, (hipStream_t)stream), "indicator");
 }
 
 }  // extern "C"

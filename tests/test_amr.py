@@ -1,0 +1,276 @@
+"""GPU: the block hierarchy through the C ABI against the restatement in oracle/amr.py.
+
+Bars:
+  * prolongation (bssn_prolong_f64), the indicator (bssn_batch_indicator_f64) and the ghost
+    fill program (bssn_ghost_fill_f64) bit-identical to the restatement (no contraction in the
+    prolongation on either side; the indicator and copies are exact);
+  * a root-only hierarchy step bit-identical to the single-grid periodic RK4 step (interiors);
+  * two shared-dt RK4 steps of a three-level hierarchy within 1e-12 normwise per field and
+    block of the restatement's step (whole padded blocks);
+  * regrid: new children hold the prolongation of their parent, coarsening injects them back
+    (bit-identical), nesting holds, capacity growth keeps the data; a run with regrids stays
+    finite and refines where the feature is.
+"""
+from __future__ import annotations
+
+import itertools
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import amr as OA
+import oracle as O
+from conftest import gpu_available, normwise
+
+from bssn3d_hip import _lib
+from bssn3d_hip import initial_data as idata
+from bssn3d_hip.amr import AMRHierarchy, prolong_entries
+from bssn3d_hip.evolve import BSSNEvolution
+from bssn3d_hip.grid import Grid
+from bssn3d_hip.rhs import stream_handle
+from bssn3d_hip.state import CHI, PhysicsParams, scalars
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not gpu_available(), reason="needs a HIP device")]
+
+DEV = "cuda:0"
+CORNERS = list(itertools.product((0, 1), repeat=3))
+
+
+def say(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def three_level(BS=8, ng=3, device=DEV, L=1.0):
+    H = AMRHierarchy(nb_root=(2, 2, 2), BS=BS, ng=ng, levels=3, device=device, L=L,
+                     lo=(-L / 2,) * 3)
+    T = H.topo
+    s = 0
+    for c in CORNERS:
+        T.add_block(1, s, tuple(c[a] * BS for a in range(3)), parent=(0, 0))
+        s += 1
+    p1 = T.bbox_ijk[(0, 1)]
+    for c in ((0, 0, 0), (1, 0, 0)):
+        T.add_block(1, s, tuple(2 * p1[a] + c[a] * BS for a in range(3)), parent=(0, 1))
+        s += 1
+    k = [key for key in T.blocks(1) if T.bbox_ijk[key] == (BS, BS, BS)][0]
+    pb = T.bbox_ijk[k]
+    for i, c in enumerate(((0, 0, 0), (1, 0, 0), (1, 1, 1))):
+        T.add_block(2, i, tuple(2 * pb[a] + c[a] * BS for a in range(3)), parent=k)
+    assert T.check_proper_nesting() == []
+    return H
+
+
+def smooth_fn(seed=0, amp=1e-3, L=1.0, lo=-0.5):
+    """Flat space + seeded periodic sinusoids on the box lo + [0, L)^3 (same function on every
+    block)."""
+    pert = idata.Perturbation.draw(np.random.default_rng(seed))
+
+    def fn(g: Grid):
+        x = torch.as_tensor(g.x, device=DEV).view(-1, 1, 1)
+        y = torch.as_tensor(g.y, device=DEV).view(1, -1, 1)
+        z = torch.as_tensor(g.z, device=DEV).view(1, 1, -1)
+        d = idata.minkowski(g, device=DEV)
+        for f in range(24):
+            acc = torch.zeros(g.shape, dtype=torch.float64, device=DEV)
+            for n in range(pert.k.shape[1]):
+                k = [2 * math.pi * int(v) / L for v in pert.k[f, n]]
+                acc += torch.sin(k[0] * (x - lo) + k[1] * (y - lo) + k[2] * (z - lo)
+                                 + float(pert.phi[f, n]))
+            d[f] += amp * acc / pert.k.shape[1]
+        return d
+    return fn
+
+
+def host_blocks(H, U):
+    P = U.cpu().numpy()
+    return {k: H.view(P, *k).copy() for k in H.keys}
+
+
+def hier_of(H):
+    return {k: (k[0], H.topo.bbox_ijk[k]) for k in H.keys}
+
+
+@pytest.mark.parametrize("BS,ng", [(8, 3), (16, 4)])
+def test_prolong_kernel_bitwise(BS, ng):
+    say(f"prolongation BS={BS} ng={ng}")
+    W = BS + 2 * ng
+    P = 24 * W ** 3
+    rng = np.random.default_rng(BS)
+    parent = rng.standard_normal((24, W, W, W))
+    pool = torch.zeros(9 * P, dtype=torch.float64, device=DEV)
+    pool[:P] = torch.from_numpy(parent.reshape(-1)).to(DEV)
+    pts = np.stack(np.meshgrid(*[np.arange(W)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    e = np.concatenate([prolong_entries(pts, (i + 1) * P, 0, c, BS, ng)
+                        for i, c in enumerate(CORNERS)])
+    d = torch.from_numpy(e.view(np.uint8).copy()).to(DEV)
+    _lib.check(_lib.load().bssn_prolong_f64(pool.data_ptr(), d.data_ptr(), len(e),
+                                            stream_handle(pool.device)))
+    got = pool.cpu().numpy()
+    for i, c in enumerate(CORNERS):
+        want = OA.prolongate(parent, c, BS, ng)
+        assert np.array_equal(got[(i + 1) * P:(i + 2) * P].reshape(24, W, W, W), want), c
+
+
+def test_indicator_bitwise():
+    say("indicator")
+    H = three_level()
+    U = torch.from_numpy(np.random.default_rng(3).standard_normal(H.pool_doubles)).to(DEV)
+    keys = H.keys
+    W = H.W
+    U[H.slot_offset(*keys[4]) + 5 * W ** 3 + (5 * W + 5) * W + 5] = float("nan")   # interior
+    out = torch.empty(len(keys) * 24, dtype=torch.float64, device=DEV)
+    _lib.check(_lib.load().bssn_batch_indicator_f64(H.batch._handle, U.data_ptr(),
+                                                    out.data_ptr(), stream_handle(U.device)))
+    got = out.view(len(keys), 24).cpu().numpy()
+    hb = host_blocks(H, U)
+    for i, k in enumerate(keys):
+        want = OA.indicator(hb[k], H.ng)
+        assert np.array_equal(got[i], want, equal_nan=True), k
+    assert np.isnan(got[4, 5]) and not np.isnan(got[4, 4])
+    lv = H.indicators(U, fields=[0, 1])
+    assert lv[keys[0][0]][keys[0][1]] == max(got[0, 0], got[0, 1])
+
+
+def test_sync_program_bitwise():
+    say("ghost fill program, three levels")
+    H = three_level()
+    U = torch.from_numpy(np.random.default_rng(4).standard_normal(H.pool_doubles)).to(DEV)
+    want = OA.sync_all(host_blocks(H, U), hier_of(H), H.nb_root, H.BS, H.ng)
+    H.sync_(U)
+    got = host_blocks(H, U)
+    for k in H.keys:
+        assert np.array_equal(got[k], want[k]), k
+
+
+def test_root_only_step_equals_single_grid():
+    say("root-only hierarchy step vs single-grid periodic step")
+    N, BS, ng = 16, 8, 3
+    H = AMRHierarchy(nb_root=(2, 2, 2), BS=BS, ng=ng, levels=1, device=DEV)
+    h = H.h0
+    g = Grid(N, N, N, h, h, h, ng=ng, xmin=-0.5, ymin=-0.5, zmin=-0.5)
+    F = smooth_fn(1)(g)
+    U = H.from_global(H.empty_pool(), F)
+    p = PhysicsParams()
+    dt = 0.25 * h
+    ev = BSSNEvolution(g, p, order=4, ko_sigma=0.1, bc="periodic")
+    Fg = ev.evolve(F, dt, 3, t0=0.2).data
+    H.evolve(U, 0.2, dt, 3, params=p, order=4, ko_sigma=0.1)
+    assert torch.equal(H.to_global(U), Fg[:, ng:ng + N, ng:ng + N, ng:ng + N])
+
+
+def test_hierarchy_step_vs_oracle():
+    """Two steps of the three-level hierarchy on the box [-8, 8)^3 (spacings 1, 1/2, 1/4)
+    against the restatement's step. Bar per field: within 1e-12, or within twice what the
+    single-grid periodic step itself shows against the oracle on the same data at the finest
+    spacing (flat space + 1e-3 sinusoids: K and At are ~1e-3 while their right-hand sides
+    come from stencils of ~1 values, so FP64 evaluation orders differ there by a few 1e-12
+    on a single grid already)."""
+    say("three-level hierarchy, 2 RK4 steps vs oracle")
+    Lb = 16.0
+    fn = smooth_fn(2, L=Lb, lo=-Lb / 2)
+    H = three_level(L=Lb)
+    U = H.fill(H.empty_pool(), fn)
+    p = PhysicsParams()
+    dt = 0.25 * H.spacing(2)
+    S16 = scalars(p, 1.0, 1.0, 1.0, dt, 0.0)
+    ref = host_blocks(H, U)
+    t = 0.05
+    for _ in range(2):
+        ref = OA.rk4_step(ref, hier_of(H), H.nb_root, H.BS, H.ng, H.h0, S16, t, dt, order=4,
+                          ko_sigma=0.1)
+        t += dt
+    H.evolve(U, 0.05, dt, 2, params=p, order=4, ko_sigma=0.1)
+    got = host_blocks(H, U)
+    per_field = np.zeros(24)
+    for k in H.keys:
+        per_field = np.maximum(per_field, normwise(got[k], ref[k]))
+    # the single-grid path on the same data at the finest spacing
+    N, h = 64, H.spacing(2)
+    g = Grid(N, N, N, h, h, h, ng=3, xmin=-Lb / 2, ymin=-Lb / 2, zmin=-Lb / 2)
+    F = fn(g)
+    sg = BSSNEvolution(g, p, order=4, ko_sigma=0.1, bc="periodic").evolve(F, dt, 2, t0=0.05)
+    want = O.periodic_sync(O.evolve(F.cpu().numpy(), scalars(p, h, h, h, dt, 0.05), dt, 2,
+                                    t0=0.05, order=4, ko_order=6, ko_sigma=0.1, ng=3), 3)
+    single = normwise(sg.data.cpu().numpy(), want)
+    print("hierarchy " + " ".join(f"{v:.1e}" for v in per_field))
+    print("single    " + " ".join(f"{v:.1e}" for v in single))
+    assert np.all(per_field <= np.maximum(1e-12, 2.0 * single))
+
+
+def bump_fn(center=(0.2, 0.2, 0.2), width=0.08, amp=0.05):
+    def fn(g: Grid):
+        x = torch.as_tensor(g.x, device=DEV).view(-1, 1, 1)
+        y = torch.as_tensor(g.y, device=DEV).view(1, -1, 1)
+        z = torch.as_tensor(g.z, device=DEV).view(1, 1, -1)
+        d = idata.minkowski(g, device=DEV)
+        r2 = (x - center[0]) ** 2 + (y - center[1]) ** 2 + (z - center[2]) ** 2
+        d[CHI] += amp * torch.exp(-r2 / width ** 2)
+        return d
+    return fn
+
+
+def test_regrid_refine_then_coarsen():
+    say("regrid: refine a bump's block, then coarsen it back")
+    H = AMRHierarchy(nb_root=(2, 2, 2), BS=8, ng=3, levels=2, device=DEV)
+    U = H.fill(H.empty_pool(), bump_fn())
+    before = host_blocks(H, U)
+    ind = H.indicators(U, fields=[CHI])
+    hot = [k for k in H.keys if ind[0][k[1]] > 1e-3]
+    assert len(hot) >= 1
+    U = H.regrid(U, refine_threshold=1e-3, coarsen_threshold=1e-9, hysteresis_K=1, n_buffer=0)
+    T = H.topo
+    kids = T.blocks(1)
+    assert len(kids) == 8 * len(hot) and T.check_proper_nesting() == []
+    after = host_blocks(H, U)
+    I = (slice(None),) + (slice(3, 11),) * 3
+    for k in kids:
+        par = T.parent[k]
+        want = OA.prolongate(before[par], T.child_corner(k), 8, 3)
+        assert np.array_equal(after[k][I], want[I]), k
+    # every block now reads low: the children coarsen back (injection restores the parent)
+    U = H.regrid(U, refine_threshold=1e9, coarsen_threshold=1e8, hysteresis_K=1)
+    assert T.blocks(1) == [] and T.check_proper_nesting() == []
+    final = host_blocks(H, U)
+    for k in H.keys:
+        assert np.array_equal(final[k][I], before[k][I]), k
+
+
+def test_capacity_growth_keeps_data():
+    say("slot capacity growth")
+    H = AMRHierarchy(nb_root=(2, 2, 2), BS=8, ng=3, levels=2, caps=[8, 4], max_blocks=64,
+                     device=DEV)
+    U = H.fill(H.empty_pool(), bump_fn(center=(0.0, 0.0, 0.0)))
+    before = host_blocks(H, U)
+    flags = np.zeros((2, 64), np.int32)
+    flags[0, 0] = 1
+    U2 = H.apply_flags(U, flags)
+    assert H.topo.caps[1] >= 8 and H.topo.grow_history and U2.numel() == H.pool_doubles
+    after = host_blocks(H, U2)
+    I = (slice(None),) + (slice(3, 11),) * 3
+    for k in before:
+        assert np.array_equal(after[k][I], before[k][I])
+    assert len(H.topo.blocks(1)) == 8
+
+
+def test_evolve_with_regrid_follows_bump():
+    say("evolve with regrids")
+    H = AMRHierarchy(nb_root=(2, 2, 2), BS=8, ng=3, levels=3, device=DEV)
+    U = H.fill(H.empty_pool(), bump_fn())
+    dt = 0.25 * H.spacing(2)
+    seen = []
+    U, t = H.evolve_with_regrid(U, 0.0, dt, 8, 2, refine_threshold=2e-3,
+                                coarsen_threshold=1e-6, hysteresis_K=1, n_buffer=0,
+                                fields=[CHI], on_regrid=lambda n, u, h: seen.append(
+                                    [h.topo.n_active(L) for L in range(3)]),
+                                params=PhysicsParams(), order=4, ko_sigma=0.1)
+    assert abs(t - 8 * dt) < 1e-15 and len(seen) == 3
+    assert seen[0][1] > 0 and H.topo.check_proper_nesting() == []
+    assert bool(torch.isfinite(U).all())
+    # the bump's centre lies inside a level-1 block
+    def holds(k):
+        b = H.block(k)
+        return all(b.lo[a] <= 0.2 < b.lo[a] + 8 * b.h[a] for a in range(3))
+    assert any(holds(k) for k in H.topo.blocks(1))
