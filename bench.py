@@ -1,7 +1,7 @@
 """Benchmark: the fused BSSN RHS + RK4 step at 256^3 FP64 on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--no-cpu] [--no-twopass]
-                    [--no-int8] [--no-batch] [--no-longrun]
+                    [--no-int8] [--no-batch] [--no-longrun] [--no-amr]
 
 A step is one full RK4 step (four fused stage kernels: RHS + Kreiss-Oliger + stage update,
 enforcement folded into the last; then the periodic ghost refill) of the 256^3-interior,
@@ -27,7 +27,9 @@ Secondary forms (extra keys, same metric family):
   longrun     the apples gate (fused and INT8-split schemes against the two-pass scheme on
               the reference's three N=24 configurations) with its verdicts and wall time, and
               a 128^3 long run with the reference's carried ghost values; the carried-ghost
-              RK4 step at 256^3 is reported beside the default one (carried_ghost_step_ms).
+              RK4 step at 256^3 is reported beside the default one (carried_ghost_step_ms);
+  amr         a live three-level block hierarchy (16^3 blocks) around the binary: RK4 step,
+              ghost fill program, a refining regrid pass.
 
 Rank 0 prints one JSON line.
 """
@@ -266,6 +268,65 @@ def bench_longrun() -> dict:
     return out
 
 
+def bench_amr(reps: int) -> dict:
+    """A live three-level hierarchy: 4x4x4 root blocks of 16^3 nodes on [-8, 8)^3 (h = 1/4,
+    1/8, 1/16), Brill-Lindquist binary (0.5 + 0.5, 6 apart; no perturbations), refined by two
+    regrid passes on the chi indicator; shared-dt RK4 steps with the ghost program after every
+    stage (root periodic, parent prolongation, face copies), the program alone, and one regrid
+    pass (indicator, flags, transfers, new plan and program)."""
+    import torch
+    from bssn3d_hip import initial_data as idata
+    from bssn3d_hip.amr import AMRHierarchy
+    from bssn3d_hip.batch import timed
+    from bssn3d_hip.state import CHI, PhysicsParams
+
+    dev = torch.device("cuda:0")
+    H = AMRHierarchy(nb_root=(4, 4, 4), BS=16, ng=3, levels=3, L=16.0, lo=(-8.0,) * 3,
+                     device=dev)
+    pos = [(-3.0 + 0.01, 0.013, 0.007), (3.0 + 0.01, 0.013, 0.007)]
+    fn = lambda g: idata.punctures(g, [0.5, 0.5], pos, amp=0.0, device=dev)
+    U = H.fill(H.empty_pool(), fn)
+    kw = dict(refine_threshold=0.02, coarsen_threshold=1e-6, hysteresis_K=1, n_buffer=0,
+              fields=[CHI])
+    for _ in range(2):
+        U = H.regrid(U, **kw)
+    counts = [H.topo.n_active(L) for L in range(3)]
+    p = PhysicsParams()
+    dt = 0.25 * H.spacing(2)
+    t = [0.0]
+
+    def step():
+        H.step_(U, t[0], dt, p, 4, 0.1)
+        t[0] += dt
+    step()
+    torch.cuda.synchronize()
+    ms_step = timed(step, reps, dev)
+    assert bool(torch.isfinite(U).all()), "non-finite hierarchy state"
+    ms_sync = timed(lambda: H.sync_(U), reps, dev)
+    pts = sum(counts) * 16 ** 3
+    prog = H.program
+    # a regrid pass that changes nothing costs the indicator and the decisions; time one that
+    # refines (a fresh hierarchy), including the new plan and program
+    torch.cuda.synchronize()
+    H2 = AMRHierarchy(nb_root=(4, 4, 4), BS=16, ng=3, levels=3, L=16.0, lo=(-8.0,) * 3,
+                      device=dev)
+    V = H2.fill(H2.empty_pool(), fn)
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    V = H2.regrid(V, **kw)
+    H2.batch, H2.program
+    torch.cuda.synchronize()
+    ms_regrid = (time.perf_counter() - w0) * 1e3
+    del V, H2
+    return {"workload": "bl_binary_amr_3lev_root4x4x4_bs16_box16",
+            "blocks_per_level": counts, "interior_points": pts,
+            "ghost_program": {"prolongations": prog.n_prolong, "copies": prog.n_copy},
+            "steps_timed": reps, "step_ms": round(ms_step, 4),
+            "step_mpts_s": round(pts / (ms_step * 1e-3) / 1e6, 2),
+            "ghost_fill_ms": round(ms_sync, 4),
+            "regrid_refine_pass_ms": round(ms_regrid, 1)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -277,6 +338,7 @@ def main() -> int:
     ap.add_argument("--no-int8", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-longrun", action="store_true")
+    ap.add_argument("--no-amr", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="also run one plain device copy of this many GiB (profiler calibration)")
@@ -442,6 +504,9 @@ def main() -> int:
         torch.cuda.empty_cache()
     if not args.no_batch:
         res["batched"] = bench_batched(reps)
+        torch.cuda.empty_cache()
+    if not args.no_amr:
+        res["amr"] = bench_amr(reps)
         torch.cuda.empty_cache()
     if not args.no_longrun:
         res["longrun"] = bench_longrun()

@@ -250,6 +250,26 @@ int bssn_batch_rk4_step_f64(const bssn_batch* b, double* U, double* work, const 
                             long long nhalo, double t, double dt, double chi_floor,
                             double alpha_floor, double* tdev, void* stream);
 
+/* Block-floating-point storage of the two-pass bundle (the reference's storage-compression
+ * form, common/src/mcs_common/bfp_compress.py:28-90 and bfp48.py:7-60, applied to the 138
+ * derivative arrays between the passes; all arithmetic stays fp64). Array i is stored as
+ * m = clip(rint(d 2^e_i), -2^(k-1), 2^(k-1) - 1) with one exponent per array, k = mant_bits:
+ * int32 for k <= 32, three 16-bit two's-complement limbs (low first) for k <= 48. Storage Q is
+ * a 1280-byte header (2^-e_i as doubles) followed by the mantissas, array-major like D.
+ * bssn_bfp_bytes: Q's size (0 for invalid arguments).
+ * bssn_deriv_bundle_absmax_f64: amax[i] = max |d_i| over the padded grid (138 device doubles),
+ *   from which the caller forms e_i = floor(log2((2^(k-1) - 1) / amax_i)), 0 for amax_i = 0
+ *   (bfp_compress.py:37-46).
+ * bssn_deriv_bundle_bfp: pass 1 into BFP storage (exps: host array of 138 exponents).
+ * bssn_algebra_bfp_f64: pass 2 reading BFP storage (d = m 2^-e, exact). */
+size_t bssn_bfp_bytes(int mant_bits, long long npts);
+int bssn_deriv_bundle_absmax_f64(const double* F, double* amax, int Sx, int Sy, int Sz,
+                                 int fd_order, const double* h3, void* stream);
+int bssn_deriv_bundle_bfp(const double* F, void* Q, const int* exps, int mant_bits, int Sx,
+                          int Sy, int Sz, int fd_order, const double* h3, void* stream);
+int bssn_algebra_bfp_f64(const double* F, const void* Q, int mant_bits, const double* S16,
+                         double* OUT, long long npts, void* stream);
+
 /* Prolongations of n points (device array) into a pool: the reference's prolongate / the halo
  * fill of sync_across_levels (3D/src/multipatch/amr/kernels.py:66-108, sync.py:84-97). */
 int bssn_prolong_f64(double* pool, const bssn_prolong_point* pts, long long n, void* stream);
@@ -263,7 +283,8 @@ int bssn_ghost_fill_f64(double* pool, const bssn_ghost_program* prog, void* stre
 /* bssn_batch_rk4_step_f64 with bc = 2 whose ghost refill after every stage and at the end is a
  * ghost fill program: the reference's shared-dt N-level RK4 step over a block hierarchy
  * (2D/src/mcs2d/amr/evolve.py:659-775), with enforcement of the BSSN step folded into the last
- * stage. U must be synced on entry (as every step leaves it). */
+ * stage. U must be synced on entry (as every step leaves it), and the program must write every
+ * ghost point of every block (the stage buffers' ghosts are not held from U). */
 int bssn_batch_rk4_step_prog_f64(const bssn_batch* b, double* U, double* work, const double* S16,
                                  int fd_order, double ko_sigma, const bssn_ghost_program* prog,
                                  double t, double dt, double chi_floor, double alpha_floor,

@@ -80,6 +80,13 @@ SIGNATURES = {
                                               _c_double, _c_void_p, _c_double, _c_double,
                                               _c_double, _c_double, _c_void_p, _c_void_p]),
     "bssn_batch_indicator_f64": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_void_p]),
+    "bssn_bfp_bytes": (ctypes.c_size_t, [_c_int, _c_ll]),
+    "bssn_deriv_bundle_absmax_f64": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, _c_int,
+                                              _c_int, _PD, _c_void_p]),
+    "bssn_deriv_bundle_bfp": (_c_int, [_c_void_p, _c_void_p, ctypes.POINTER(_c_int), _c_int,
+                                       _c_int, _c_int, _c_int, _c_int, _PD, _c_void_p]),
+    "bssn_algebra_bfp_f64": (_c_int, [_c_void_p, _c_void_p, _c_int, _PD, _c_void_p, _c_ll,
+                                      _c_void_p]),
 }
 
 

@@ -240,6 +240,14 @@ def prolong_entries(points: np.ndarray, dst_off: int, parent_off: int, corner, B
     return e
 
 
+def by_parity(e: np.ndarray) -> np.ndarray:
+    """Prolongation entries ordered by the parity class of their fine indices (stable), so that
+    the device's lanes in one wave evaluate the same number of taps."""
+    f = e["fine"]
+    cls = (f[:, 0] & 1) * 4 + (f[:, 1] & 1) * 2 + (f[:, 2] & 1)
+    return e[np.argsort(cls, kind="stable")]
+
+
 def restrict_entries(parent_off: int, child_off: int, corner, BS: int, ng: int) -> np.ndarray:
     """Injection of a child's coincident interior nodes into its parent's octant
     (kernels.py:113-140): parent ng + corner BS/2 + a <- child ng + 2 a, a in [0, BS/2)^3."""
@@ -308,6 +316,7 @@ def build_ghost_program(topo: AMRTopology, nb_root, ng: int, slot_offset: Callab
     ce.append(sum(len(x) for x in copies))
     topo.rebuild_neighbors()
     for L in range(1, topo.levels):
+        level_pro = []
         for key in topo.blocks(L):
             s = key[1]
             done = np.zeros((W, W, W), bool)
@@ -327,8 +336,10 @@ def build_ghost_program(topo: AMRTopology, nb_root, ng: int, slot_offset: Callab
                 done[d[:, 0], d[:, 1], d[:, 2]] = True
             rest = pts[~done.reshape(-1)]
             par = topo.parent[key]
-            prolongs.append(prolong_entries(rest, slot_offset(L, s), slot_offset(*par),
-                                            topo.child_corner(key), BS, ng))
+            level_pro.append(prolong_entries(rest, slot_offset(L, s), slot_offset(*par),
+                                             topo.child_corner(key), BS, ng))
+        if level_pro:
+            prolongs.append(by_parity(np.concatenate(level_pro)))
         pe.append(sum(len(x) for x in prolongs))
         ce.append(sum(len(x) for x in copies))
     cat = lambda xs, dt: np.concatenate(xs) if xs else np.empty(0, dtype=dt)
@@ -603,20 +614,24 @@ class AMRHierarchy:
         st = stream_handle(self.device)
         changed = False
         for L in range(self.levels - 1, 0, -1):
+            ents = []                     # this level's restrictions: one copy launch
             for (_, s) in topo.blocks(L):
                 key = (L, s)
                 if flags[L, s] != COARSEN or key not in topo.parent:
                     continue
                 par = topo.parent[key]
-                e = restrict_entries(self.slot_offset(*par), self.slot_offset(L, s),
-                                     topo.child_corner(key), BS, ng)
-                d = _upload(e, self.device)
-                _lib.check(lib.bssn_batch_halo_f64(U.data_ptr(), d.data_ptr(), len(e), st))
+                ents.append(restrict_entries(self.slot_offset(*par), self.slot_offset(L, s),
+                                             topo.child_corner(key), BS, ng))
                 topo.remove_block(L, s)
                 changed = True
+            if ents:
+                e = np.concatenate(ents)
+                d = _upload(e, self.device)
+                _lib.check(lib.bssn_batch_halo_f64(U.data_ptr(), d.data_ptr(), len(e), st))
         pts = _grid_points(self.W)
         for L in range(self.levels - 1):
             cl = L + 1
+            new = []                      # this level's new children: one prolongation launch
             for (_, s) in topo.blocks(L):
                 if flags[L, s] != REFINE or not topo.active[L, s]:
                     continue
@@ -637,13 +652,15 @@ class AMRHierarchy:
                                 U = self._grow(U, cl)
                                 free = topo.free_slots(cl)
                             slot = int(free[0])
-                            e = prolong_entries(pts, self.slot_offset(cl, slot),
-                                                self.slot_offset(L, s), c, BS, ng)
-                            d = _upload(e, self.device)
-                            _lib.check(lib.bssn_prolong_f64(U.data_ptr(), d.data_ptr(), len(e),
-                                                            st))
                             topo.add_block(cl, slot, cb, parent=(L, s))
+                            new.append((slot, s, c))
                             changed = True
+            if new:        # offsets after any growth; parents are level L, children level L+1
+                e = by_parity(np.concatenate([
+                    prolong_entries(pts, self.slot_offset(cl, slot), self.slot_offset(L, ps), c,
+                                    BS, ng) for slot, ps, c in new]))
+                d = _upload(e, self.device)
+                _lib.check(lib.bssn_prolong_f64(U.data_ptr(), d.data_ptr(), len(e), st))
         if changed:
             self._invalidate()
             topo.rebuild_neighbors()

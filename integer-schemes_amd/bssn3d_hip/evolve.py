@@ -14,7 +14,8 @@ values advance every padded point with edge-padded stencils (the reference leave
 unsynced too). bc="sommerfeld" gives the ghost zones the radiative outgoing-wave condition
 (bssn_rk4_step_sommerfeld_f64; the reference has none for BSSN and this follows its MCS
 solver's, 3D/src/mcs3d/main.py:230-250) while the interiors advance as above.
-With scheme "twopass" or "int8" (the precision form: derivatives from int8 digits) the step is
+With scheme "twopass", "int8" (the precision form: derivatives from int8 digits) or a "bfp*"
+scheme (the bundle stored in block floating point between the passes) the step is
 the reference's own composition (evolve.py:86-90, 125-132): per stage a periodic sync, the
 materialised 138-derivative bundle, the algebra and KO over the full padded arrays, then the
 RK4 combinations over the full arrays, enforce and (periodic) a final sync.
@@ -30,7 +31,8 @@ import torch
 
 from . import _lib
 from .grid import Grid
-from .rhs import BSSNSolver, as_tensor, rhs_int8_into, rhs_twopass_into, stream_handle
+from .rhs import (BFP_BITS, BSSNSolver, as_tensor, bfp_buffers, rhs_bfp_into, rhs_int8_into,
+                  rhs_twopass_into, stream_handle)
 from .state import BSSNState, PhysicsParams
 
 INTEGRATORS = ("rk4", "rk4_2_2")
@@ -62,6 +64,7 @@ class BSSNEvolution:
         self.ghosts = ghosts
         self._work = None
         self._D = None
+        self._Q = None
 
     @property
     def periodic(self) -> bool:
@@ -178,6 +181,12 @@ class BSSNEvolution:
         if self.periodic:
             self.periodic_sync_(Y)
         S = self._S16(t, dt)
+        if self.solver.scheme in BFP_BITS:
+            k = BFP_BITS[self.solver.scheme]
+            if self._Q is None or self._Q[2] != tuple(Y.shape) or self._Q[0].device != Y.device:
+                self._Q = bfp_buffers(Y, k) + (tuple(Y.shape),)
+            rhs_bfp_into(Y, out, self._Q[0], self._Q[1], S, self.order, self.ko_sigma, k)
+            return out
         D = self._bundle(Y)
         if self.solver.scheme == "int8":
             rhs_int8_into(Y, out, D, S, self.order, self.ko_sigma)

@@ -75,6 +75,54 @@ def gauge_wave(grid: Grid, amplitude: float = 0.01, wavelength: float | None = N
     return d
 
 
+def gowdy(grid: Grid, t: float = 1.0, device="cpu") -> torch.Tensor:
+    """Polarized Gowdy T^3 vacuum cosmology at time t as BSSN variables, inhomogeneous in z,
+    periodic on [-1/2, 1/2]^3 (initial_data.py:200-291):
+        ds^2 = t^(-1/2) e^(L/2) (-dt^2 + dz^2) + t (e^P dx^2 + e^-P dy^2),
+        P = J0(2 pi t) cos(2 pi z),
+        L = -2 pi t J0 J1 cos^2(2 pi z) + 2 pi^2 t^2 (J0^2 + J1^2) - C,
+    Jn = Jn(2 pi t), C fixed by t = 1; alpha = t^(-1/4) e^(L/4), zero shift,
+    K_ij = -d_t gamma_ij / (2 alpha)."""
+    from scipy.special import j0, j1
+
+    tw = 2.0 * math.pi
+    pi2 = math.pi ** 2
+    b0, b1 = float(j0(tw * t)), float(j1(tw * t))
+    c0, c1 = float(j0(tw)), float(j1(tw))
+    C = 0.5 * (tw * tw * (c0 * c0 + c1 * c1) - tw * c0 * c1)
+    _, _, Z = _coords(grid, device)
+    z = Z.expand(grid.shape)
+    cz = torch.cos(tw * z)
+    s2z = torch.sin(2.0 * tw * z)
+    P = b0 * cz
+    P_t = -tw * b1 * cz
+    lam = -tw * t * b0 * b1 * cz ** 2 + 2.0 * pi2 * t * t * (b0 * b0 + b1 * b1) - C
+    lam_z = 4.0 * pi2 * t * b0 * b1 * s2z
+    lam_t = 4.0 * pi2 * t * (b0 * b0 - (b0 * b0 - b1 * b1) * cz ** 2)
+    eP = torch.exp(P)
+    g_xx = t * eP
+    g_yy = t / eP
+    g_zz = t ** (-0.5) * torch.exp(0.5 * lam)
+    alpha = t ** (-0.25) * torch.exp(0.25 * lam)
+    Kxx = -(eP * (1.0 + t * P_t)) / (2.0 * alpha)
+    Kyy = -((1.0 / eP) * (1.0 - t * P_t)) / (2.0 * alpha)
+    Kzz = -(g_zz * (0.5 * lam_t - 0.5 / t)) / (2.0 * alpha)
+    Ktr = Kxx / g_xx + Kyy / g_yy + Kzz / g_zz
+    chi = (g_xx * g_yy * g_zz) ** (-1.0 / 3.0)
+    d = torch.zeros((NUM_VARS,) + grid.shape, dtype=torch.float64, device=device)
+    d[ALPHA] = alpha
+    d[CHI] = chi
+    d[K] = Ktr
+    d[GT0] = chi * g_xx
+    d[GT3] = chi * g_yy
+    d[GT5] = chi * g_zz
+    d[AT0] = chi * (Kxx - g_xx * Ktr / 3.0)
+    d[AT3] = chi * (Kyy - g_yy * Ktr / 3.0)
+    d[AT5] = chi * (Kzz - g_zz * Ktr / 3.0)
+    d[GTILDE0 + 2] = (t / 3.0) * torch.exp(-lam / 3.0) * lam_z
+    return d
+
+
 def hamiltonian_bump(grid: Grid, amplitude: float = 0.01, device="cpu") -> torch.Tensor:
     """Minkowski with chi + A cos(2 pi x) cos(2 pi y) cos(2 pi z) (initial_data.py:33-55): a
     smooth, coherent Hamiltonian-constraint violation with no momentum violation, periodic on

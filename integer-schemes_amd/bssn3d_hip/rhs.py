@@ -15,18 +15,25 @@ Schemes:
             (the data flow of the reference's "verbatim" scheme)
   "int8"    the two-pass path with the bundle taken from int8 digits on the INT8 matrix cores
             (Ozaki scheme I, 7 digits by default; FD order 4); the algebra stays FP64
+  "bfp48" / "bfp40" / "bfp32" / "bfp24"
+            the two-pass path with the bundle stored in block floating point of that many
+            mantissa bits (one exponent per derivative array; the reference's storage-
+            compression form, common/src/mcs_common/bfp_compress.py); arithmetic stays FP64
 """
 from __future__ import annotations
 
+import ctypes
 from typing import Dict
 
+import numpy as np
 import torch
 
 from . import _lib
 from .grid import Grid, ghost_width
 from .state import BSSNState, PhysicsParams, VAR_NAMES, pack_S, scalars
 
-SCHEMES = ("fused", "twopass", "int8")
+SCHEMES = ("fused", "twopass", "int8", "bfp48", "bfp40", "bfp32", "bfp24")
+BFP_BITS = {"bfp48": 48, "bfp40": 40, "bfp32": 32, "bfp24": 24}
 
 
 def as_tensor(state) -> torch.Tensor:
@@ -90,6 +97,51 @@ def rhs_int8_into(F: torch.Tensor, OUT: torch.Tensor, D: torch.Tensor, S16, orde
                                        h3, 0, 0, st))
 
 
+def bfp_exponents(amax: np.ndarray, mant_bits: int) -> np.ndarray:
+    """Per-array exponents floor(log2((2^(k-1) - 1) / max|d|)), 0 for an all-zero array
+    (common/src/mcs_common/bfp_compress.py:37-46)."""
+    amax = np.asarray(amax, dtype=np.float64)
+    if not np.all(np.isfinite(amax)):
+        raise ValueError("non-finite derivative: the bundle cannot be stored in BFP")
+    full = 2.0 ** (mant_bits - 1) - 1.0
+    with np.errstate(divide="ignore"):
+        e = np.where(amax == 0.0, 0.0, np.floor(np.log2(full / np.where(amax == 0.0, 1.0, amax))))
+    return e.astype(np.int64)
+
+
+def bfp_buffers(F: torch.Tensor, mant_bits: int):
+    """(Q, amax): BFP storage of the bundle of F's grid and the 138 per-array maxima."""
+    npts = F[0].numel()
+    nbytes = _lib.load().bssn_bfp_bytes(int(mant_bits), npts)
+    if nbytes == 0:
+        raise ValueError(f"mant_bits {mant_bits} outside 2..48")
+    Q = torch.empty(int(nbytes), dtype=torch.uint8, device=F.device)
+    amax = torch.empty(138, dtype=torch.float64, device=F.device)
+    return Q, amax
+
+
+def rhs_bfp_into(F: torch.Tensor, OUT: torch.Tensor, Q: torch.Tensor, amax: torch.Tensor, S16,
+                 order: int = 4, ko_sigma: float = 0.0, mant_bits: int = 32) -> np.ndarray:
+    """OUT <- algebra(F, bundle(F) stored at mant_bits per value) (+ FP64 KO): per-array maxima
+    (one pass), exponents on the host, the bundle into BFP storage Q, the algebra from Q.
+    Returns the exponents."""
+    _, Sx, Sy, Sz = F.shape
+    lib = _lib.load()
+    st = stream_handle(F.device)
+    h3 = _lib.dbl(S16[13:16])
+    _lib.check(lib.bssn_deriv_bundle_absmax_f64(_ptr(F), _ptr(amax), Sx, Sy, Sz, order, h3, st))
+    exps = bfp_exponents(amax.cpu().numpy(), mant_bits)
+    e = (ctypes.c_int * 138)(*[int(v) for v in exps])
+    _lib.check(lib.bssn_deriv_bundle_bfp(_ptr(F), _ptr(Q), e, int(mant_bits), Sx, Sy, Sz, order,
+                                         h3, st))
+    _lib.check(lib.bssn_algebra_bfp_f64(_ptr(F), _ptr(Q), int(mant_bits), _lib.dbl(S16),
+                                        _ptr(OUT), Sx * Sy * Sz, st))
+    if ko_sigma > 0.0:
+        _lib.check(lib.bssn_ko_add_f64(_ptr(F), _ptr(OUT), Sx, Sy, Sz, order, float(ko_sigma),
+                                       h3, 0, 0, st))
+    return exps
+
+
 def cuda_fused_algebra(F, dx, dy, dz, eta, lmbda, lambda_f, cahd_c, dt, ssl_h, ssl_sigma, t,
                        order: int = 6) -> Dict[str, torch.Tensor]:
     """The reference's fused-scheme callable (fused_rhs_cuda.py:62-72): a dict of 24 fields in,
@@ -150,6 +202,8 @@ class BSSNSolver:
         self.ko_order = ko_order
         self.dt = dt if dt is not None else 0.25 * grid.dx
         self._D = None
+        self._Q = None
+        self.last_exponents = None
 
     def S16(self, t: float = 0.0, dt: float | None = None):
         g, p = self.grid, self.params
@@ -164,6 +218,13 @@ class BSSNSolver:
         S = self.S16(t, dt)
         if self.scheme == "fused":
             rhs_into(F, out, S, self.order)
+        elif self.scheme in BFP_BITS:
+            k = BFP_BITS[self.scheme]
+            if self._Q is None or self._Q[0].device != F.device or \
+                    self._Q[2] != tuple(F.shape):
+                self._Q = bfp_buffers(F, k) + (tuple(F.shape),)
+            self.last_exponents = rhs_bfp_into(F, out, self._Q[0], self._Q[1], S, self.order,
+                                               mant_bits=k)
         else:
             if self._D is None or self._D.device != F.device:
                 self._D = torch.empty((138,) + tuple(F.shape[1:]), dtype=torch.float64,

@@ -261,6 +261,64 @@ int bssn_deriv_bundle_i8_f64(const double* F, double* D, int Sx, int Sy, int Sz,
                    "INT8 bundle launch");
 }
 
+size_t bssn_bfp_bytes(int mant_bits, long long npts) {
+  if (mant_bits < 2 || mant_bits > 48 || npts < 1) return 0;
+  const size_t per = mant_bits <= 32 ? 4 : 6;
+  return (size_t)BFP_HEADER_BYTES + (size_t)BSSN_ND * (size_t)npts * per;
+}
+
+int bssn_deriv_bundle_absmax_f64(const double* F, double* amax, int Sx, int Sy, int Sz,
+                                 int fd_order, const double* h3, void* stream) {
+  if (!F || !amax) return fail(BSSN_E_ARG, "null pointer");
+  if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4 or 6");
+  TRY(check_shape(Sx, Sy, Sz));
+  TRY(check_h(h3));
+  const Scales w = make_scales(h3, fd_order, 0.0);
+  return check_hip(launch_bundle_absmax(F, (unsigned long long*)amax, full_dims(Sx, Sy, Sz, 0),
+                                        w, fd_order, (hipStream_t)stream),
+                   "bundle max launch");
+}
+
+static int check_bfp(const int* exps, int mant_bits, BundleOut* o, BfpScales* sc) {
+  if (!exps) return fail(BSSN_E_ARG, "null exponent array");
+  if (mant_bits < 2 || mant_bits > 48) return fail(BSSN_E_ARG, "mant_bits must be 2..48");
+  for (int i = 0; i < BSSN_ND; ++i) {
+    if (exps[i] < -1000 || exps[i] > 1000)
+      return fail(BSSN_E_ARG, "BFP exponents must lie in [-1000, 1000]");
+    if (o) o->e[i] = exps[i];
+    if (sc) sc->inv[i] = std::ldexp(1.0, -exps[i]);
+  }
+  if (o) o->lim = std::ldexp(1.0, mant_bits - 1);
+  return BSSN_OK;
+}
+
+int bssn_deriv_bundle_bfp(const double* F, void* Q, const int* exps, int mant_bits, int Sx,
+                          int Sy, int Sz, int fd_order, const double* h3, void* stream) {
+  if (!F || !Q) return fail(BSSN_E_ARG, "null pointer");
+  if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4 or 6");
+  TRY(check_shape(Sx, Sy, Sz));
+  TRY(check_h(h3));
+  BundleOut o{};
+  BfpScales sc{};
+  TRY(check_bfp(exps, mant_bits, &o, &sc));
+  const Scales w = make_scales(h3, fd_order, 0.0);
+  return check_hip(launch_bundle_bfp(F, Q, o, sc, full_dims(Sx, Sy, Sz, 0), w, fd_order,
+                                     mant_bits > 32, (hipStream_t)stream),
+                   "BFP bundle launch");
+}
+
+int bssn_algebra_bfp_f64(const double* F, const void* Q, int mant_bits, const double* S16,
+                         double* OUT, long long npts, void* stream) {
+  if (!F || !Q || !OUT) return fail(BSSN_E_ARG, "null pointer");
+  if (npts < 1) return fail(BSSN_E_ARG, "npts must be >= 1");
+  if (mant_bits < 2 || mant_bits > 48) return fail(BSSN_E_ARG, "mant_bits must be 2..48");
+  TRY(check_s16(S16));
+  const Scal sc = make_scal(S16, S16[2], S16[6]);
+  return check_hip(launch_algebra_bfp(F, Q, OUT, (long)npts, sc, mant_bits > 32,
+                                      (hipStream_t)stream),
+                   "BFP algebra launch");
+}
+
 int bssn_algebra_f64(const double* F, const double* D, const double* S16, double* OUT, long npts,
                      void* stream) {
   if (!F || !D || !OUT) return fail(BSSN_E_ARG, "null pointer");
@@ -809,10 +867,11 @@ int bssn_batch_halo_f64(double* pool, const bssn_halo_copy* map, long long n, vo
 namespace {
 
 // One RK4 step of a batch; bc = 2 refills ghosts with `sync` after every stage and at the end.
+// hold = copy U's ghosts into the stage buffers first (a sync that may leave ghosts unwritten).
 template <class Sync>
 int batch_rk4(const bssn_batch* b, double* U, double* work, const double* S16, int fd_order,
-              double ko_sigma, int bc, Sync sync, double t, double dt, double chi_floor,
-              double alpha_floor, double* tdev, hipStream_t st) {
+              double ko_sigma, int bc, Sync sync, bool hold, double t, double dt,
+              double chi_floor, double alpha_floor, double* tdev, hipStream_t st) {
   TRY(check_batch(b));
   if (!U || !work) return fail(BSSN_E_ARG, "null pointer");
   if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4 or 6");
@@ -848,7 +907,7 @@ int batch_rk4(const bssn_batch* b, double* U, double* work, const double* S16, i
     if (bc == 2 && y) TRY(sync(y));
     return BSSN_OK;
   };
-  if (bc == 2 && b->ng > 0)   // stage inputs start with U's ghosts
+  if (bc == 2 && hold && b->ng > 0)   // stage inputs start with U's ghosts
     TRY(check_hip(launch_batch_hold(U, Ya, Yb, b->blk, b->ghost, b->grid_ghost, st),
                   "ghost hold"));
   TRY(stage(0.0, KIND_RK1, hdt, U, Ya));
@@ -906,8 +965,8 @@ int bssn_batch_rk4_step_f64(const bssn_batch* b, double* U, double* work, const 
   auto sync = [&](double* pool) {
     return check_hip(launch_halo_copy(pool, halo, (long)nhalo, st), "halo copy");
   };
-  return batch_rk4(b, U, work, S16, fd_order, ko_sigma, bc, sync, t, dt, chi_floor, alpha_floor,
-                   tdev, st);
+  return batch_rk4(b, U, work, S16, fd_order, ko_sigma, bc, sync, true, t, dt, chi_floor,
+                   alpha_floor, tdev, st);
 }
 
 int bssn_prolong_f64(double* pool, const bssn_prolong_point* pts, long long n, void* stream) {
@@ -929,8 +988,8 @@ int bssn_batch_rk4_step_prog_f64(const bssn_batch* b, double* U, double* work, c
   TRY(check_program(prog));
   hipStream_t st = (hipStream_t)stream;
   auto sync = [&](double* pool) { return run_program(pool, prog, st); };
-  return batch_rk4(b, U, work, S16, fd_order, ko_sigma, 2, sync, t, dt, chi_floor, alpha_floor,
-                   tdev, st);
+  return batch_rk4(b, U, work, S16, fd_order, ko_sigma, 2, sync, false, t, dt, chi_floor,
+                   alpha_floor, tdev, st);
 }
 
 int bssn_batch_indicator_f64(const bssn_batch* b, const double* U, double* out, void* stream) {

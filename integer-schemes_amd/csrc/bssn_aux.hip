@@ -27,9 +27,50 @@ __host__ __device__ constexpr int s2slot(int f) {
 }
 
 // ------------------------------------------------------------------------- pass 1: bundle
-template <int ORD>
-__global__ __launch_bounds__(64) void k_deriv_bundle(const double* __restrict__ F,
-                                                     double* __restrict__ D, Dims d, Scales w) {
+// One kernel body, four stores (so every store sees the same derivative values):
+//   BS_F64  the fp64 bundle;
+//   BS_MAX  per-array max |d| only (wave max, then a vector atomicMax on the bit pattern of the
+//           non-negative double; NaN propagates) -- the shared exponents of the BFP form;
+//   BS_Q32  / BS_Q48  block-floating-point storage (common/src/mcs_common/bfp_compress.py:65-90,
+//           bfp48.py:7-25): m = clip(rint(d 2^e), -2^(k-1), 2^(k-1) - 1), e per array, as int32
+//           (k <= 32) or as three 16-bit two's-complement limbs (k <= 48).
+enum { BS_F64 = 0, BS_MAX = 1, BS_Q32 = 2, BS_Q48 = 3 };
+
+__device__ __forceinline__ double nanmax_d(double m, double v) {
+  return (v > m || v != v) ? v : m;
+}
+
+template <int STORE>
+__device__ __forceinline__ void bundle_put(const BundleOut& o, int slot, long P, long gc,
+                                           double v, bool valid) {
+  if constexpr (STORE == BS_F64) {
+    if (valid) o.D[(long)slot * P + gc] = v;
+  } else if constexpr (STORE == BS_MAX) {
+    double a = valid ? fabs(v) : 0.0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) a = nanmax_d(a, __shfl_xor(a, off, 64));
+    if (threadIdx.x == 0)
+      atomicMax(o.amax + slot, (unsigned long long)__double_as_longlong(a));
+  } else {
+    if (!valid) return;
+    double m = rint(ldexp(v, o.e[slot]));
+    m = m < -o.lim ? -o.lim : (m > o.lim - 1.0 ? o.lim - 1.0 : m);
+    const long i = (long)slot * P + gc;
+    if constexpr (STORE == BS_Q32) {
+      ((int*)o.Q)[i] = (int)m;
+    } else {
+      const long long mi = (long long)m;
+      unsigned short* q = (unsigned short*)o.Q + 3 * i;
+      q[0] = (unsigned short)(mi & 0xFFFF);
+      q[1] = (unsigned short)((mi >> 16) & 0xFFFF);
+      q[2] = (unsigned short)((mi >> 32) & 0xFFFF);
+    }
+  }
+}
+
+template <int ORD, int STORE>
+__global__ __launch_bounds__(64) void k_deriv_bundle(const double* __restrict__ F, BundleOut o,
+                                                     Dims d, Scales w) {
   using G = Geo<ORD, false>;
   __shared__ double lds[G::VOL];
   int ox, oy, oz;
@@ -48,26 +89,65 @@ __global__ __launch_bounds__(64) void k_deriv_bundle(const double* __restrict__ 
 #pragma unroll
       for (int q = 0; q < 6; ++q) v[3 + q] = s.d2(pi_(q), pj_(q));
     }
-    if (valid) {
 #pragma unroll
-      for (int a = 0; a < 3; ++a) D[(long)(3 * f + a) * d.P + s.gc] = v[a];
-      if (sl >= 0) {
+    for (int a = 0; a < 3; ++a) bundle_put<STORE>(o, 3 * f + a, d.P, s.gc, v[a], valid);
+    if (sl >= 0) {
 #pragma unroll
-        for (int q = 0; q < 6; ++q) D[(long)(72 + 6 * sl + q) * d.P + s.gc] = v[3 + q];
-      }
+      for (int q = 0; q < 6; ++q)
+        bundle_put<STORE>(o, 72 + 6 * sl + q, d.P, s.gc, v[3 + q], valid);
     }
     __builtin_amdgcn_wave_barrier();
   }
 }
 
+template <int STORE>
+static void launch_bundle_store(const double* F, const BundleOut& o, const Dims& d,
+                                const Scales& w, int ord, hipStream_t st) {
+  if (ord == 4)
+    hipLaunchKernelGGL((k_deriv_bundle<4, STORE>), dim3(brick_blocks<Geo<4, false>>(d)),
+                       dim3(64), 0, st, F, o, d, w);
+  else
+    hipLaunchKernelGGL((k_deriv_bundle<6, STORE>), dim3(brick_blocks<Geo<6, false>>(d)),
+                       dim3(64), 0, st, F, o, d, w);
+}
+
 hipError_t launch_deriv_bundle(const double* F, double* D, const Dims& d, const Scales& w,
                                int ord, hipStream_t st) {
-  if (ord == 4)
-    hipLaunchKernelGGL(k_deriv_bundle<4>, dim3(brick_blocks<Geo<4, false>>(d)), dim3(64), 0, st,
-                       F, D, d, w);
+  BundleOut o{};
+  o.D = D;
+  launch_bundle_store<BS_F64>(F, o, d, w, ord, st);
+  return hipGetLastError();
+}
+
+__global__ void k_clear_u64(unsigned long long* p, int n) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < n) p[i] = 0ull;
+}
+
+__global__ void k_put_scales(double* dst, BfpScales s) {
+  const int i = (int)threadIdx.x;
+  if (i < BSSN_ND) dst[i] = s.inv[i];
+}
+
+hipError_t launch_bundle_absmax(const double* F, unsigned long long* amax, const Dims& d,
+                                const Scales& w, int ord, hipStream_t st) {
+  hipLaunchKernelGGL(k_clear_u64, dim3(1), dim3(256), 0, st, amax, BSSN_ND);
+  BundleOut o{};
+  o.amax = amax;
+  launch_bundle_store<BS_MAX>(F, o, d, w, ord, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_bundle_bfp(const double* F, void* Q, const BundleOut& o_in, const BfpScales& sc,
+                             const Dims& d, const Scales& w, int ord, bool wide,
+                             hipStream_t st) {
+  hipLaunchKernelGGL(k_put_scales, dim3(1), dim3(256), 0, st, (double*)Q, sc);
+  BundleOut o = o_in;
+  o.Q = (unsigned char*)Q + BFP_HEADER_BYTES;
+  if (wide)
+    launch_bundle_store<BS_Q48>(F, o, d, w, ord, st);
   else
-    hipLaunchKernelGGL(k_deriv_bundle<6>, dim3(brick_blocks<Geo<6, false>>(d)), dim3(64), 0, st,
-                       F, D, d, w);
+    launch_bundle_store<BS_Q32>(F, o, d, w, ord, st);
   return hipGetLastError();
 }
 
@@ -93,6 +173,60 @@ struct PlainOut {
   __device__ __forceinline__ void put(int f, double v) { OUT[(long)f * n + p] = v; }
   __device__ __forceinline__ void metric_done() {}
 };
+
+// the bundle read back from BFP storage: d = m 2^-e, exact (the header holds 2^-e per array)
+template <bool WIDE>
+struct BundleSrcQ {
+  const double* F;
+  const double* inv;
+  const void* Q;
+  long n, p;
+  int cur;
+  __device__ __forceinline__ void stage(int f) { cur = f; }
+  __device__ __forceinline__ double center(int f) const { return F[(long)f * n + p]; }
+  __device__ __forceinline__ double val() const { return F[(long)cur * n + p]; }
+  __device__ __forceinline__ double get(int slot) const {
+    const long i = (long)slot * n + p;
+    double m;
+    if constexpr (WIDE) {
+      const unsigned short* q = (const unsigned short*)Q + 3 * i;
+      long long c = (long long)q[0] | ((long long)q[1] << 16) | ((long long)q[2] << 32);
+      c = c >= (1ll << 47) ? c - (1ll << 48) : c;
+      m = (double)c;
+    } else {
+      m = (double)((const int*)Q)[i];
+    }
+    return m * inv[slot];
+  }
+  __device__ __forceinline__ double d1(int a) const { return get(3 * cur + a); }
+  __device__ __forceinline__ double d2(int i, int j) const {
+    return get(72 + 6 * s2slot(cur) + sym(i, j));
+  }
+  __device__ __forceinline__ double ko() const { return 0.0; }
+};
+
+template <bool WIDE>
+__global__ __launch_bounds__(256) void k_algebra_bfp(const double* __restrict__ F,
+                                                     const unsigned char* __restrict__ Q,
+                                                     double* __restrict__ OUT, long n, Scal sc) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  BundleSrcQ<WIDE> s{F, (const double*)Q, Q + BFP_HEADER_BYTES, n, p, 0};
+  PlainOut o{OUT, n, p};
+  bssn_rhs_point<false>(s, sc, o);
+}
+
+hipError_t launch_algebra_bfp(const double* F, const void* Q, double* OUT, long npts,
+                              const Scal& sc, bool wide, hipStream_t st) {
+  const unsigned nb = (unsigned)((npts + 255) / 256);
+  if (wide)
+    hipLaunchKernelGGL(k_algebra_bfp<true>, dim3(nb), dim3(256), 0, st, F,
+                       (const unsigned char*)Q, OUT, npts, sc);
+  else
+    hipLaunchKernelGGL(k_algebra_bfp<false>, dim3(nb), dim3(256), 0, st, F,
+                       (const unsigned char*)Q, OUT, npts, sc);
+  return hipGetLastError();
+}
 
 __global__ __launch_bounds__(256) void k_algebra(const double* __restrict__ F,
                                                  const double* __restrict__ D,

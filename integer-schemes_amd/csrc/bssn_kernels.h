@@ -106,6 +106,26 @@ hipError_t launch_sommerfeld(const StageArgs& a, const Dims& d, const Scales& w,
                              int ord, hipStream_t st);
 
 // two-pass (materialised derivative bundle) path
+// BFP storage header (2^-e per derivative array), bytes
+constexpr long BFP_HEADER_BYTES = 1280;
+
+// where a bundle pass puts its values (one of these, per the pass)
+struct BundleOut {
+  double* D;                      // fp64 bundle
+  unsigned long long* amax;       // per-array max |d| (bit patterns)
+  void* Q;                        // BFP mantissas
+  double lim;                     // 2^(mant_bits - 1)
+  int e[BSSN_ND];                 // per-array exponents
+};
+struct BfpScales {
+  double inv[BSSN_ND];            // 2^-e per array
+};
+hipError_t launch_bundle_absmax(const double* F, unsigned long long* amax, const Dims& d,
+                                const Scales& w, int ord, hipStream_t st);
+hipError_t launch_bundle_bfp(const double* F, void* Q, const BundleOut& o, const BfpScales& sc,
+                             const Dims& d, const Scales& w, int ord, bool wide, hipStream_t st);
+hipError_t launch_algebra_bfp(const double* F, const void* Q, double* OUT, long npts,
+                              const Scal& sc, bool wide, hipStream_t st);
 hipError_t launch_deriv_bundle(const double* F, double* D, const Dims& d, const Scales& w, int ord,
                                hipStream_t st);
 hipError_t launch_algebra(const double* F, const double* D, double* OUT, long npts,
