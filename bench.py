@@ -1,7 +1,7 @@
 """Benchmark: the fused BSSN RHS + RK4 step at 256^3 FP64 on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--no-cpu] [--no-twopass]
-                    [--no-int8] [--no-batch] [--no-longrun] [--no-amr]
+                    [--no-int8] [--no-batch] [--no-longrun] [--no-amr] [--no-bfp]
 
 A step is one full RK4 step (four fused stage kernels: RHS + Kreiss-Oliger + stage update,
 enforcement folded into the last; then the periodic ghost refill) of the 256^3-interior,
@@ -29,7 +29,10 @@ Secondary forms (extra keys, same metric family):
               a 128^3 long run with the reference's carried ghost values; the carried-ghost
               RK4 step at 256^3 is reported beside the default one (carried_ghost_step_ms);
   amr         a live three-level block hierarchy (16^3 blocks) around the binary: RK4 step,
-              ghost fill program, a refining regrid pass.
+              ghost fill program, a refining regrid pass;
+  bfp         the two-pass RHS with the bundle stored at 32 and 48 mantissa bits per value
+              (block floating point, one exponent per derivative array) against the fp64
+              bundle: time, stored bytes, error.
 
 Rank 0 prints one JSON line.
 """
@@ -327,6 +330,66 @@ def bench_amr(reps: int) -> dict:
             "regrid_refine_pass_ms": round(ms_regrid, 1)}
 
 
+def bench_bfp(N: int, reps: int) -> dict:
+    """The two-pass RHS with its bundle stored in block floating point (BFP32, BFP48) against
+    the fp64 bundle, on the 256^3 binary: per evaluation a maxima pass, the quantizing bundle
+    pass, the dequantizing algebra (and the 138 maxima to the host for the exponents); and with
+    exponents fixed from an earlier evaluation (no maxima pass)."""
+    import numpy as np
+    import torch
+    from bssn3d_hip import initial_data as idata
+    from bssn3d_hip.rhs import bfp_buffers, rhs_bfp_into, rhs_twopass_into
+    from bssn3d_hip.state import PhysicsParams, scalars
+
+    dev = torch.device("cuda:0")
+    grid, U, _ = idata.config_binary(seed=0, N=N, device=dev)
+    S16 = scalars(PhysicsParams(), grid.dx, grid.dy, grid.dz, 0.25 * grid.dx, 0.0)
+    Sx, Sy, Sz = grid.shape
+    P = Sx * Sy * Sz
+    ref = torch.empty_like(U)
+    D = torch.empty((138, Sx, Sy, Sz), dtype=torch.float64, device=dev)
+    rhs_twopass_into(U, ref, D, S16, 4)
+    torch.cuda.synchronize()
+    a, b = _events()
+    a.record()
+    for _ in range(reps):
+        rhs_twopass_into(U, ref, D, S16, 4)
+    b.record()
+    torch.cuda.synchronize()
+    out = {"workload": f"bssn_rhs_{N}^3_bl_binary_fd4", "fp64_bundle_bytes": 138 * 8 * P,
+           "twopass_fp64_ms": round(a.elapsed_time(b) / reps, 4)}
+    del D
+    torch.cuda.empty_cache()
+    o = torch.empty_like(U)
+    for k in (32, 48):
+        Q, amax = bfp_buffers(U, k)
+        rhs_bfp_into(U, o, Q, amax, S16, 4, mant_bits=k)
+        torch.cuda.synchronize()
+        a.record()
+        for _ in range(reps):
+            rhs_bfp_into(U, o, Q, amax, S16, 4, mant_bits=k)
+        b.record()
+        torch.cuda.synchronize()
+        I = (slice(None),) + (slice(3, -3),) * 3
+        d = (o[I] - ref[I]).double()
+        err = float(torch.sqrt(torch.mean(d * d)) / torch.sqrt(torch.mean(ref[I] * ref[I])))
+        ms = a.elapsed_time(b) / reps
+        e_fix = rhs_bfp_into(U, o, Q, amax, S16, 4, mant_bits=k)
+        torch.cuda.synchronize()
+        a.record()
+        for _ in range(reps):
+            rhs_bfp_into(U, o, Q, amax, S16, 4, mant_bits=k, exps=e_fix)
+        b.record()
+        torch.cuda.synchronize()
+        out[f"bfp{k}"] = {"ms": round(ms, 4),
+                          "fixed_exponents_ms": round(a.elapsed_time(b) / reps, 4),
+                          "stored_bundle_bytes": int(Q.numel()),
+                          "rel_l2_err_vs_fp64": float("%.3e" % err)}
+        del Q, amax
+        torch.cuda.empty_cache()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -339,6 +402,7 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-longrun", action="store_true")
     ap.add_argument("--no-amr", action="store_true")
+    ap.add_argument("--no-bfp", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="also run one plain device copy of this many GiB (profiler calibration)")
@@ -504,6 +568,9 @@ def main() -> int:
         torch.cuda.empty_cache()
     if not args.no_batch:
         res["batched"] = bench_batched(reps)
+        torch.cuda.empty_cache()
+    if not args.no_bfp:
+        res["bfp"] = bench_bfp(N, reps)
         torch.cuda.empty_cache()
     if not args.no_amr:
         res["amr"] = bench_amr(reps)

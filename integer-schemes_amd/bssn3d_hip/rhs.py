@@ -121,16 +121,24 @@ def bfp_buffers(F: torch.Tensor, mant_bits: int):
 
 
 def rhs_bfp_into(F: torch.Tensor, OUT: torch.Tensor, Q: torch.Tensor, amax: torch.Tensor, S16,
-                 order: int = 4, ko_sigma: float = 0.0, mant_bits: int = 32) -> np.ndarray:
+                 order: int = 4, ko_sigma: float = 0.0, mant_bits: int = 32,
+                 exps: np.ndarray | None = None) -> np.ndarray:
     """OUT <- algebra(F, bundle(F) stored at mant_bits per value) (+ FP64 KO): per-array maxima
     (one pass), exponents on the host, the bundle into BFP storage Q, the algebra from Q.
-    Returns the exponents."""
+    Given ``exps`` (fixed exponents, e.g. from an earlier call), the maxima pass and the host
+    round trip are skipped and out-of-range values saturate (bfp_compress.py:76-83).
+    Returns the exponents used."""
     _, Sx, Sy, Sz = F.shape
     lib = _lib.load()
     st = stream_handle(F.device)
     h3 = _lib.dbl(S16[13:16])
-    _lib.check(lib.bssn_deriv_bundle_absmax_f64(_ptr(F), _ptr(amax), Sx, Sy, Sz, order, h3, st))
-    exps = bfp_exponents(amax.cpu().numpy(), mant_bits)
+    if exps is None:
+        _lib.check(lib.bssn_deriv_bundle_absmax_f64(_ptr(F), _ptr(amax), Sx, Sy, Sz, order, h3,
+                                                    st))
+        exps = bfp_exponents(amax.cpu().numpy(), mant_bits)
+    exps = np.asarray(exps, dtype=np.int64)
+    if exps.shape != (138,):
+        raise ValueError("exps must hold 138 exponents")
     e = (ctypes.c_int * 138)(*[int(v) for v in exps])
     _lib.check(lib.bssn_deriv_bundle_bfp(_ptr(F), _ptr(Q), e, int(mant_bits), Sx, Sy, Sz, order,
                                          h3, st))

@@ -27,14 +27,12 @@ __host__ __device__ constexpr int s2slot(int f) {
 }
 
 // ------------------------------------------------------------------------- pass 1: bundle
-// One kernel body, four stores (so every store sees the same derivative values):
+// One kernel body, three stores (so every store sees the same derivative values):
 //   BS_F64  the fp64 bundle;
-//   BS_MAX  per-array max |d| only (wave max, then a vector atomicMax on the bit pattern of the
-//           non-negative double; NaN propagates) -- the shared exponents of the BFP form;
 //   BS_Q32  / BS_Q48  block-floating-point storage (common/src/mcs_common/bfp_compress.py:65-90,
 //           bfp48.py:7-25): m = clip(rint(d 2^e), -2^(k-1), 2^(k-1) - 1), e per array, as int32
 //           (k <= 32) or as three 16-bit two's-complement limbs (k <= 48).
-enum { BS_F64 = 0, BS_MAX = 1, BS_Q32 = 2, BS_Q48 = 3 };
+enum { BS_F64 = 0, BS_Q32 = 2, BS_Q48 = 3 };
 
 __device__ __forceinline__ double nanmax_d(double m, double v) {
   return (v > m || v != v) ? v : m;
@@ -45,12 +43,6 @@ __device__ __forceinline__ void bundle_put(const BundleOut& o, int slot, long P,
                                            double v, bool valid) {
   if constexpr (STORE == BS_F64) {
     if (valid) o.D[(long)slot * P + gc] = v;
-  } else if constexpr (STORE == BS_MAX) {
-    double a = valid ? fabs(v) : 0.0;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) a = nanmax_d(a, __shfl_xor(a, off, 64));
-    if (threadIdx.x == 0)
-      atomicMax(o.amax + slot, (unsigned long long)__double_as_longlong(a));
   } else {
     if (!valid) return;
     double m = rint(ldexp(v, o.e[slot]));
@@ -129,12 +121,69 @@ __global__ void k_put_scales(double* dst, BfpScales s) {
   if (i < BSSN_ND) dst[i] = s.inv[i];
 }
 
+// Per-array maxima with few atomics: each wave walks many bricks, keeps the running maxima in
+// registers (lane l holds arrays l, l + 64, l + 128) and ends with one atomicMax per array.
+template <int ORD>
+__global__ __launch_bounds__(64) void k_bundle_absmax(const double* __restrict__ F,
+                                                      unsigned long long* __restrict__ amax,
+                                                      Dims d, Scales w) {
+  using G = Geo<ORD, false>;
+  __shared__ double lds[G::VOL];
+  const int lane = (int)threadIdx.x;
+  const int nbx = (d.n[0] + G::B - 1) / G::B, nby = (d.n[1] + G::B - 1) / G::B,
+            nbz = (d.n[2] + G::B - 1) / G::B;
+  const int nb = nbx * nby * nbz;
+  double r0 = 0.0, r1 = 0.0, r2 = 0.0;
+  auto keep = [&](int slot, double v, bool valid) {
+    double a = valid ? fabs(v) : 0.0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) a = nanmax_d(a, __shfl_xor(a, off, 64));
+    if (lane == (slot & 63)) {
+      const int q = slot >> 6;
+      if (q == 0) r0 = nanmax_d(r0, a);
+      else if (q == 1) r1 = nanmax_d(r1, a);
+      else r2 = nanmax_d(r2, a);
+    }
+  };
+  for (int L = (int)blockIdx.x; L < nb; L += (int)gridDim.x) {
+    const int bz = L % nbz, t = L / nbz, by = t % nby, bx = t / nby;
+    const int ox = d.lo[0] + bx * G::B, oy = d.lo[1] + by * G::B, oz = d.lo[2] + bz * G::B;
+    const bool valid = lane_valid<G>(d, ox, oy, oz, lane);
+    WaveSrc<ORD, false, CLAMP, StateOrder> s(lds, F, w, d, ox, oy, oz, lane);
+#pragma unroll 1
+    for (int f = 0; f < BSSN_NF; ++f) {
+      s.stage(f);
+      double v[9];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) v[a] = s.d1(a);
+      const int sl = s2slot(f);
+      if (sl >= 0) {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) v[3 + q] = s.d2(pi_(q), pj_(q));
+      }
+#pragma unroll
+      for (int a = 0; a < 3; ++a) keep(3 * f + a, v[a], valid);
+      if (sl >= 0) {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) keep(72 + 6 * sl + q, v[3 + q], valid);
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+  atomicMax(amax + lane, (unsigned long long)__double_as_longlong(r0));
+  atomicMax(amax + 64 + lane, (unsigned long long)__double_as_longlong(r1));
+  if (128 + lane < BSSN_ND)
+    atomicMax(amax + 128 + lane, (unsigned long long)__double_as_longlong(r2));
+}
+
 hipError_t launch_bundle_absmax(const double* F, unsigned long long* amax, const Dims& d,
                                 const Scales& w, int ord, hipStream_t st) {
   hipLaunchKernelGGL(k_clear_u64, dim3(1), dim3(256), 0, st, amax, BSSN_ND);
-  BundleOut o{};
-  o.amax = amax;
-  launch_bundle_store<BS_MAX>(F, o, d, w, ord, st);
+  const unsigned grid = 2048;
+  if (ord == 4)
+    hipLaunchKernelGGL(k_bundle_absmax<4>, dim3(grid), dim3(64), 0, st, F, amax, d, w);
+  else
+    hipLaunchKernelGGL(k_bundle_absmax<6>, dim3(grid), dim3(64), 0, st, F, amax, d, w);
   return hipGetLastError();
 }
 
