@@ -189,16 +189,33 @@ typedef struct {
   int pS[3];
 } bssn_prolong_point;
 
-/* A ghost fill program for a block hierarchy: phases run in order; phase p applies prolongs
- * [prolong_end[p-1], prolong_end[p]) and copies [copy_end[p-1], copy_end[p]) (ends cumulative,
- * host arrays of nphase entries; the entry arrays are device memory). Within a phase the
- * destinations must be disjoint and no entry may read a point the phase writes; a phase may
- * read what earlier phases wrote (a level's prolongation reads its parent level's ghosts). */
+/* A whole-block prolongation job: the same values as bssn_prolong_point entries for the points
+ * of one child block, evaluated separably (x, y and z passes shared by neighbouring points).
+ * dst / parent: pool offsets of the child's and the parent's first padded point (field 0);
+ * both blocks have padded extent W = BS + 2 ng per axis (W <= 24); corner: the child's octant;
+ * skip: bits 0..5 leave out the ghost slab over the interior extent of face -x, +x, -y, +y,
+ * -z, +z (those come from neighbours), bit 6 writes the interior too. */
+typedef struct {
+  long long dst, parent;
+  int corner[3];
+  int skip;
+  int BS, ng;
+} bssn_prolong_block;
+
+/* A ghost fill program for a block hierarchy: phases run in order; phase p applies the
+ * prolongation entries [prolong_end[p-1], prolong_end[p]), the block jobs
+ * [block_end[p-1], block_end[p]) and the copies [copy_end[p-1], copy_end[p]) (ends cumulative,
+ * host arrays of nphase entries; the entry arrays are device memory; any of the three may be
+ * empty). Within a phase the destinations must be disjoint and no entry may read a point the
+ * phase writes; a phase may read what earlier phases wrote (a level's prolongation reads its
+ * parent level's ghosts). */
 typedef struct {
   int nphase;
   const long long* prolong_end;
+  const long long* block_end;
   const long long* copy_end;
   const bssn_prolong_point* prolongs;
+  const bssn_prolong_block* blocks;
   const bssn_halo_copy* copies;
 } bssn_ghost_program;
 
@@ -273,6 +290,10 @@ int bssn_algebra_bfp_f64(const double* F, const void* Q, int mant_bits, const do
 /* Prolongations of n points (device array) into a pool: the reference's prolongate / the halo
  * fill of sync_across_levels (3D/src/multipatch/amr/kernels.py:66-108, sync.py:84-97). */
 int bssn_prolong_f64(double* pool, const bssn_prolong_point* pts, long long n, void* stream);
+
+/* n whole-block prolongation jobs (device array). */
+int bssn_prolong_blocks_f64(double* pool, const bssn_prolong_block* jobs, long long n,
+                            void* stream);
 
 /* Run a ghost fill program on a pool: the hierarchy sync of the reference's N-level step,
  * root level periodic, then each finer level from its parent (prolongation) and from

@@ -76,6 +76,7 @@ SIGNATURES = {
                                          _c_double, _c_double, _c_void_p, _c_void_p]),
     "bssn_prolong_f64": (_c_int, [_c_void_p, _c_void_p, _c_ll, _c_void_p]),
     "bssn_ghost_fill_f64": (_c_int, [_c_void_p, _c_void_p, _c_void_p]),
+    "bssn_prolong_blocks_f64": (_c_int, [_c_void_p, _c_void_p, _c_ll, _c_void_p]),
     "bssn_batch_rk4_step_prog_f64": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _PD, _c_int,
                                               _c_double, _c_void_p, _c_double, _c_double,
                                               _c_double, _c_double, _c_void_p, _c_void_p]),
@@ -94,8 +95,10 @@ class GhostProgram(ctypes.Structure):
     """bssn_ghost_program: phase table (host) and entry arrays (device)."""
     _fields_ = [("nphase", ctypes.c_int),
                 ("prolong_end", ctypes.POINTER(ctypes.c_longlong)),
+                ("block_end", ctypes.POINTER(ctypes.c_longlong)),
                 ("copy_end", ctypes.POINTER(ctypes.c_longlong)),
                 ("prolongs", ctypes.c_void_p),
+                ("blocks", ctypes.c_void_p),
                 ("copies", ctypes.c_void_p)]
 
 _lib = None

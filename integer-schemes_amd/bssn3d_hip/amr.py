@@ -20,8 +20,9 @@ Device side: every block lives in a fixed slot of one float64 pool (level L's sl
 level L-1's); the active blocks form a BlockBatch (its plan lists only active blocks, so idle
 slots cost nothing); the ghost sync is a ghost fill program (bssn_ghost_fill_f64: one phase per
 level, prolongations and copies) run after every RK stage by bssn_batch_rk4_step_prog_f64.
-Regrid transfers run as prolongation lists (bssn_prolong_f64) and copy lists
-(bssn_batch_halo_f64); the indicator is bssn_batch_indicator_f64.
+Prolongations run as whole-block jobs (bssn_prolong_blocks_f64: separable x, y, z passes
+through LDS, padded blocks up to 24 per axis) or point lists (bssn_prolong_f64); restrictions
+as copy lists (bssn_batch_halo_f64); the indicator is bssn_batch_indicator_f64.
 
 One deliberate difference: a slot's hysteresis streak restarts at 0 when a block is created in
 it (the reference keeps the streak of the block that last held the slot).
@@ -48,6 +49,12 @@ NFACE = 6
 PROLONG_DTYPE = np.dtype([("dst", "<i8"), ("parent", "<i8"), ("dst_stride", "<i8"),
                           ("fine", "<i4", (3,)), ("pS", "<i4", (3,))])
 assert PROLONG_DTYPE.itemsize == 48
+# bssn_prolong_block: a whole-block job (padded extent W = BS + 2 ng <= BLOCK_WMAX)
+BLOCK_DTYPE = np.dtype([("dst", "<i8"), ("parent", "<i8"), ("corner", "<i4", (3,)),
+                        ("skip", "<i4"), ("BS", "<i4"), ("ng", "<i4")])
+assert BLOCK_DTYPE.itemsize == 40
+BLOCK_WMAX = 24
+SKIP_INTERIOR_WRITE = 64
 
 
 # --- topology (host) --------------------------------------------------------------------------
@@ -281,14 +288,28 @@ def _box(r):
     return np.stack(np.meshgrid(*[np.arange(a, b) for a, b in r], indexing="ij"), -1).reshape(-1, 3)
 
 
-def build_ghost_program(topo: AMRTopology, nb_root, ng: int, slot_offset: Callable):
+def block_job(dst_off: int, parent_off: int, corner, BS: int, ng: int, skip: int) -> np.ndarray:
+    e = np.zeros(1, dtype=BLOCK_DTYPE)
+    e["dst"], e["parent"], e["corner"] = dst_off, parent_off, corner
+    e["skip"], e["BS"], e["ng"] = skip, BS, ng
+    return e
+
+
+def build_ghost_program(topo: AMRTopology, nb_root, ng: int, slot_offset: Callable,
+                        use_blocks: bool | None = None):
     """Entries of the hierarchy sync, phase by phase: phase 0 = periodic copies for every ghost
     point of every root block; phase L >= 1 = for every active level-L block, face copies from
-    its same-level neighbours and prolongations from its parent for all its other ghost points.
-    Returns (prolongs, copies, prolong_end, copy_end)."""
+    its same-level neighbours and prolongations from its parent for all its other ghost points
+    -- one whole-block job per block when the padded extent allows (use_blocks, default when
+    W <= BLOCK_WMAX), else one entry per point.
+    Returns (prolongs, blocks, copies, prolong_end, block_end, copy_end)."""
     BS = topo.BS
     W = BS + 2 * ng
     P = W ** 3
+    if use_blocks is None:
+        use_blocks = W <= BLOCK_WMAX
+    if use_blocks and W > BLOCK_WMAX:
+        raise ValueError(f"whole-block prolongation needs BS + 2 ng <= {BLOCK_WMAX}")
     pts = _grid_points(W)
     ghost = pts[np.any((pts < ng) | (pts >= ng + BS), axis=1)]
     nb = np.asarray(nb_root, dtype=np.int64)
@@ -299,7 +320,7 @@ def build_ghost_program(topo: AMRTopology, nb_root, ng: int, slot_offset: Callab
         root_off[tuple(b)] = slot_offset(*k)
     if np.any(root_off < 0) or len(topo.blocks(0)) != int(np.prod(nb)):
         raise ValueError("the root level must tile the box")
-    prolongs, copies, pe, ce = [], [], [], []
+    prolongs, jobs, copies, pe, be, ce = [], [], [], [], [], []
     # phase 0: root periodic
     for key in topo.blocks(0):
         b0 = np.asarray(topo.bbox_ijk[key])
@@ -313,6 +334,7 @@ def build_ghost_program(topo: AMRTopology, nb_root, ng: int, slot_offset: Callab
         e["src_stride"] = P
         copies.append(e)
     pe.append(sum(len(x) for x in prolongs))
+    be.append(sum(len(x) for x in jobs))
     ce.append(sum(len(x) for x in copies))
     topo.rebuild_neighbors()
     for L in range(1, topo.levels):
@@ -321,10 +343,12 @@ def build_ghost_program(topo: AMRTopology, nb_root, ng: int, slot_offset: Callab
             s = key[1]
             done = np.zeros((W, W, W), bool)
             done[ng:ng + BS, ng:ng + BS, ng:ng + BS] = True
+            skip = 0
             for f in range(NFACE):
                 nb = topo.neighbors.get((L, s, f))
                 if nb is None:
                     continue
+                skip |= 1 << f
                 dr, sr = _face_slab(f, BS, ng)
                 d, sp = _box(dr), _box(sr)
                 e = np.empty(d.shape[0], dtype=HALO_DTYPE)
@@ -334,32 +358,38 @@ def build_ghost_program(topo: AMRTopology, nb_root, ng: int, slot_offset: Callab
                 e["src_stride"] = P
                 copies.append(e)
                 done[d[:, 0], d[:, 1], d[:, 2]] = True
-            rest = pts[~done.reshape(-1)]
             par = topo.parent[key]
-            level_pro.append(prolong_entries(rest, slot_offset(L, s), slot_offset(*par),
-                                             topo.child_corner(key), BS, ng))
+            if use_blocks:
+                jobs.append(block_job(slot_offset(L, s), slot_offset(*par),
+                                      topo.child_corner(key), BS, ng, skip))
+            else:
+                rest = pts[~done.reshape(-1)]
+                level_pro.append(prolong_entries(rest, slot_offset(L, s), slot_offset(*par),
+                                                 topo.child_corner(key), BS, ng))
         if level_pro:
             prolongs.append(by_parity(np.concatenate(level_pro)))
         pe.append(sum(len(x) for x in prolongs))
+        be.append(sum(len(x) for x in jobs))
         ce.append(sum(len(x) for x in copies))
     cat = lambda xs, dt: np.concatenate(xs) if xs else np.empty(0, dtype=dt)
-    return cat(prolongs, PROLONG_DTYPE), cat(copies, HALO_DTYPE), pe, ce
+    return (cat(prolongs, PROLONG_DTYPE), cat(jobs, BLOCK_DTYPE), cat(copies, HALO_DTYPE),
+            pe, be, ce)
 
 
 class _DeviceProgram:
     """A ghost fill program uploaded to the device (keeps the arrays alive)."""
 
-    def __init__(self, prolongs: np.ndarray, copies: np.ndarray, pe, ce, device):
-        self.n_prolong, self.n_copy = int(len(prolongs)), int(len(copies))
-        self._p = torch.from_numpy(np.ascontiguousarray(prolongs).view(np.uint8).copy()).to(device) \
-            if len(prolongs) else None
-        self._c = torch.from_numpy(np.ascontiguousarray(copies).view(np.uint8).copy()).to(device) \
-            if len(copies) else None
+    def __init__(self, prolongs: np.ndarray, jobs: np.ndarray, copies: np.ndarray, pe, be, ce,
+                 device):
+        self.n_prolong, self.n_block, self.n_copy = len(prolongs), len(jobs), len(copies)
+        up = lambda a: _upload(a, device) if len(a) else None
+        self._p, self._b, self._c = up(prolongs), up(jobs), up(copies)
         self._pe = (ctypes.c_longlong * len(pe))(*pe)
+        self._be = (ctypes.c_longlong * len(be))(*be)
         self._ce = (ctypes.c_longlong * len(ce))(*ce)
-        self.struct = _lib.GhostProgram(len(pe), self._pe, self._ce,
-                                        self._p.data_ptr() if self._p is not None else None,
-                                        self._c.data_ptr() if self._c is not None else None)
+        ptr = lambda t: t.data_ptr() if t is not None else None
+        self.struct = _lib.GhostProgram(len(pe), self._pe, self._be, self._ce, ptr(self._p),
+                                        ptr(self._b), ptr(self._c))
 
     @property
     def ptr(self):
@@ -477,18 +507,25 @@ class AMRHierarchy:
     @property
     def program(self) -> _DeviceProgram:
         if self._prog is None:
-            pr, cp, pe, ce = self.program_entries()
-            self._check_entries(pr, cp)
-            self._prog = _DeviceProgram(pr, cp, pe, ce, self.device)
+            pr, jb, cp, pe, be, ce = self.program_entries()
+            self._check_entries(pr, cp, jb)
+            self._prog = _DeviceProgram(pr, jb, cp, pe, be, ce, self.device)
         return self._prog
 
-    def _check_entries(self, prolongs: np.ndarray, copies: np.ndarray):
+    def _check_entries(self, prolongs: np.ndarray, copies: np.ndarray, jobs=None):
         span = (NUM_VARS - 1) * self.W ** 3
         hi = self.pool_doubles - span
-        for name, arr in (("dst", prolongs["dst"]), ("parent", prolongs["parent"]),
-                          ("dst", copies["dst"]), ("src", copies["src"])):
-            if len(arr) and (arr.min() < 0 or arr.max() >= hi):
+        blk_hi = self.pool_doubles - NUM_VARS * self.W ** 3
+        for name, arr, top in (("dst", prolongs["dst"], hi), ("parent", prolongs["parent"], hi),
+                               ("dst", copies["dst"], hi), ("src", copies["src"], hi)) + \
+                (() if jobs is None else (("job dst", jobs["dst"], blk_hi + 1),
+                                          ("job parent", jobs["parent"], blk_hi + 1))):
+            if len(arr) and (arr.min() < 0 or arr.max() >= top):
                 raise RuntimeError(f"ghost program {name} offset outside the pool")
+        if jobs is not None and len(jobs) and (self.W > BLOCK_WMAX or
+                                               np.any(jobs["BS"] != self.BS) or
+                                               np.any(jobs["ng"] != self.ng)):
+            raise RuntimeError("whole-block jobs need this hierarchy's BS, ng and W <= 24")
         if len(prolongs) and (prolongs["fine"].min() < 0 or
                               prolongs["fine"].max() >= 2 * self.W):
             raise RuntimeError("prolongation index outside the refined parent")
@@ -656,11 +693,19 @@ class AMRHierarchy:
                             new.append((slot, s, c))
                             changed = True
             if new:        # offsets after any growth; parents are level L, children level L+1
-                e = by_parity(np.concatenate([
-                    prolong_entries(pts, self.slot_offset(cl, slot), self.slot_offset(L, ps), c,
-                                    BS, ng) for slot, ps, c in new]))
-                d = _upload(e, self.device)
-                _lib.check(lib.bssn_prolong_f64(U.data_ptr(), d.data_ptr(), len(e), st))
+                if self.W <= BLOCK_WMAX:
+                    e = np.concatenate([block_job(self.slot_offset(cl, slot),
+                                                  self.slot_offset(L, ps), c, BS, ng,
+                                                  SKIP_INTERIOR_WRITE) for slot, ps, c in new])
+                    d = _upload(e, self.device)
+                    _lib.check(lib.bssn_prolong_blocks_f64(U.data_ptr(), d.data_ptr(), len(e),
+                                                           st))
+                else:
+                    e = by_parity(np.concatenate([
+                        prolong_entries(pts, self.slot_offset(cl, slot), self.slot_offset(L, ps),
+                                        c, BS, ng) for slot, ps, c in new]))
+                    d = _upload(e, self.device)
+                    _lib.check(lib.bssn_prolong_f64(U.data_ptr(), d.data_ptr(), len(e), st))
         if changed:
             self._invalidate()
             topo.rebuild_neighbors()

@@ -922,29 +922,35 @@ int batch_rk4(const bssn_batch* b, double* U, double* work, const double* S16, i
 
 int check_program(const bssn_ghost_program* p) {
   if (!p) return fail(BSSN_E_ARG, "null ghost program");
-  if (p->nphase < 0 || (p->nphase > 0 && (!p->prolong_end || !p->copy_end)))
+  if (p->nphase < 0 ||
+      (p->nphase > 0 && (!p->prolong_end || !p->block_end || !p->copy_end)))
     return fail(BSSN_E_ARG, "bad ghost program phase table");
-  long long pe = 0, ce = 0;
+  long long pe = 0, be = 0, ce = 0;
   for (int i = 0; i < p->nphase; ++i) {
-    if (p->prolong_end[i] < pe || p->copy_end[i] < ce)
+    if (p->prolong_end[i] < pe || p->block_end[i] < be || p->copy_end[i] < ce)
       return fail(BSSN_E_ARG, "ghost program phase ends must not decrease");
     pe = p->prolong_end[i];
+    be = p->block_end[i];
     ce = p->copy_end[i];
   }
-  if ((pe > 0 && !p->prolongs) || (ce > 0 && !p->copies))
+  if ((pe > 0 && !p->prolongs) || (be > 0 && !p->blocks) || (ce > 0 && !p->copies))
     return fail(BSSN_E_ARG, "ghost program entries missing");
   return BSSN_OK;
 }
 
 int run_program(double* pool, const bssn_ghost_program* p, hipStream_t st) {
-  long long pb = 0, cb = 0;
+  long long pb = 0, bb = 0, cb = 0;
   for (int i = 0; i < p->nphase; ++i) {
-    const long long pe = p->prolong_end[i], ce = p->copy_end[i];
+    const long long pe = p->prolong_end[i], be = p->block_end[i], ce = p->copy_end[i];
     if (pe > pb)
       TRY(check_hip(launch_prolong(pool, p->prolongs + pb, (long)(pe - pb), st), "prolongation"));
+    if (be > bb)
+      TRY(check_hip(launch_prolong_blocks(pool, p->blocks + bb, (long)(be - bb), st),
+                    "block prolongation"));
     if (ce > cb)
       TRY(check_hip(launch_halo_copy(pool, p->copies + cb, (long)(ce - cb), st), "ghost copies"));
     pb = pe;
+    bb = be;
     cb = ce;
   }
   return BSSN_OK;
@@ -973,6 +979,14 @@ int bssn_prolong_f64(double* pool, const bssn_prolong_point* pts, long long n, v
   if (!pool) return fail(BSSN_E_ARG, "null pool pointer");
   if (n < 0 || (n > 0 && !pts)) return fail(BSSN_E_ARG, "bad prolongation list");
   return check_hip(launch_prolong(pool, pts, (long)n, (hipStream_t)stream), "prolongation");
+}
+
+int bssn_prolong_blocks_f64(double* pool, const bssn_prolong_block* jobs, long long n,
+                            void* stream) {
+  if (!pool) return fail(BSSN_E_ARG, "null pool pointer");
+  if (n < 0 || (n > 0 && !jobs)) return fail(BSSN_E_ARG, "bad prolongation job list");
+  return check_hip(launch_prolong_blocks(pool, jobs, (long)n, (hipStream_t)stream),
+                   "block prolongation");
 }
 
 int bssn_ghost_fill_f64(double* pool, const bssn_ghost_program* prog, void* stream) {

@@ -47,38 +47,61 @@ __constant__ double kWMid[6] = {3.0 / 256.0, -25.0 / 256.0, 150.0 / 256.0,
 
 }  // namespace
 
-__global__ __launch_bounds__(256) void k_prolong(double* __restrict__ pool,
-                                                 const bssn_prolong_point* __restrict__ pts,
-                                                 long n) {
+// one destination point, all 24 fields, with the tap counts of its parity class fixed at
+// compile time (OX: x fine index odd, ...)
+template <bool OX, bool OY, bool OZ>
+__device__ __forceinline__ void prolong_point(double* __restrict__ pool,
+                                              const bssn_prolong_point& m) {
 #pragma clang fp contract(off)
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
-  const bssn_prolong_point m = pts[e];
+  constexpr int NX = OX ? 6 : 1, NY = OY ? 6 : 1, NZ = OZ ? 6 : 1;
   const Taps tx = taps(m.fine[0], m.pS[0]);
   const Taps ty = taps(m.fine[1], m.pS[1]);
   const Taps tz = taps(m.fine[2], m.pS[2]);
   const long sy = m.pS[2], sx = (long)m.pS[1] * m.pS[2];
   const long P = sx * m.pS[0];
+#pragma unroll 1
   for (int f = 0; f < BSSN_NF; ++f) {
     const double* c = pool + m.parent + (long)f * P;
     double vz = 0.0;
-    for (int kz = 0; kz < tz.n; ++kz) {
+#pragma unroll
+    for (int kz = 0; kz < NZ; ++kz) {
       double vy = 0.0;
-      for (int ky = 0; ky < ty.n; ++ky) {
+#pragma unroll
+      for (int ky = 0; ky < NY; ++ky) {
         const double* row = c + (long)ty.idx[ky] * sy + tz.idx[kz];
         double vx;
-        if (tx.n == 1) {
-          vx = row[(long)tx.idx[0] * sx];
-        } else {
+        if constexpr (OX) {
           vx = 0.0;
 #pragma unroll
           for (int kx = 0; kx < 6; ++kx) vx = vx + kWMid[kx] * row[(long)tx.idx[kx] * sx];
+        } else {
+          vx = row[(long)tx.idx[0] * sx];
         }
-        vy = ty.n == 1 ? vx : vy + kWMid[ky] * vx;
+        if constexpr (OY) vy = vy + kWMid[ky] * vx; else vy = vx;
       }
-      vz = tz.n == 1 ? vy : vz + kWMid[kz] * vy;
+      if constexpr (OZ) vz = vz + kWMid[kz] * vy; else vz = vy;
     }
     pool[m.dst + (long)f * m.dst_stride] = vz;
+  }
+}
+
+// One thread per entry. Entries sorted by parity class keep a wave on one branch.
+__global__ __launch_bounds__(256) void k_prolong(double* __restrict__ pool,
+                                                 const bssn_prolong_point* __restrict__ pts,
+                                                 long n) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const bssn_prolong_point m = pts[e];
+  const int cls = (m.fine[0] & 1) << 2 | (m.fine[1] & 1) << 1 | (m.fine[2] & 1);
+  switch (cls) {
+    case 0: prolong_point<false, false, false>(pool, m); break;
+    case 1: prolong_point<false, false, true>(pool, m); break;
+    case 2: prolong_point<false, true, false>(pool, m); break;
+    case 3: prolong_point<false, true, true>(pool, m); break;
+    case 4: prolong_point<true, false, false>(pool, m); break;
+    case 5: prolong_point<true, false, true>(pool, m); break;
+    case 6: prolong_point<true, true, false>(pool, m); break;
+    default: prolong_point<true, true, true>(pool, m); break;
   }
 }
 
@@ -87,6 +110,98 @@ hipError_t launch_prolong(double* pool, const void* pts, long n, hipStream_t st)
   const long blocks = (n + 255) / 256;
   hipLaunchKernelGGL(k_prolong, dim3((unsigned)blocks), dim3(256), 0, st, pool,
                      (const bssn_prolong_point*)pts, n);
+  return hipGetLastError();
+}
+
+// Separable whole-block prolongation: one workgroup per (job, field). The x pass fills
+// X[rx][ky][kz] (fine x index, coarse y and z window) from the parent, the y pass
+// Y[rx][ry][kz] from X, and the z sums are formed only at the points the job writes. Window
+// index k stands for parent node kmin + k, read clamped, so the taps of every sum are the
+// clamped nodes of the per-point kernel, summed in the same order.
+constexpr int PB_WMAX = 24;
+constexpr int PB_KMAX = PB_WMAX / 2 + 7;
+
+__global__ __launch_bounds__(1024) void k_prolong_block(double* __restrict__ pool,
+                                                       const bssn_prolong_block* __restrict__ jobs) {
+#pragma clang fp contract(off)
+  __shared__ double X[PB_WMAX * PB_KMAX * PB_KMAX];
+  __shared__ double Y[PB_WMAX * PB_WMAX * PB_KMAX];
+  const bssn_prolong_block jb = jobs[blockIdx.x / BSSN_NF];
+  const int f = (int)(blockIdx.x % BSSN_NF);
+  const int W = jb.BS + 2 * jb.ng;
+  const long sy = W, sx = (long)W * W, P = sx * W;
+  int r0[3], kmin[3], K[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    r0[a] = jb.ng + jb.corner[a] * jb.BS;
+    kmin[a] = (r0[a] >> 1) - 2;
+    K[a] = ((r0[a] + W - 1) >> 1) + 3 - kmin[a] + 1;
+  }
+  const double* c = pool + jb.parent + (long)f * P;
+  const int nX = W * K[1] * K[2];
+  for (int i = (int)threadIdx.x; i < nX; i += (int)blockDim.x) {
+    const int kz = i % K[2], ky = (i / K[2]) % K[1], x = i / (K[2] * K[1]);
+    const int rx = r0[0] + x;
+    const int ny = clampi(kmin[1] + ky, 0, W - 1), nz = clampi(kmin[2] + kz, 0, W - 1);
+    const double* row = c + (long)ny * sy + nz;
+    double v;
+    if ((rx & 1) == 0) {
+      v = row[(long)clampi(rx >> 1, 0, W - 1) * sx];
+    } else {
+      v = 0.0;
+#pragma unroll
+      for (int j = 0; j < 6; ++j) v = v + kWMid[j] * row[(long)clampi((rx >> 1) - 2 + j, 0, W - 1) * sx];
+    }
+    X[(x * PB_KMAX + ky) * PB_KMAX + kz] = v;
+  }
+  __syncthreads();
+  const int nY = W * W * K[2];
+  for (int i = (int)threadIdx.x; i < nY; i += (int)blockDim.x) {
+    const int kz = i % K[2], y = (i / K[2]) % W, x = i / (K[2] * W);
+    const int ry = r0[1] + y;
+    const double* col = X + (long)x * PB_KMAX * PB_KMAX + kz;
+    double v;
+    if ((ry & 1) == 0) {
+      v = col[((ry >> 1) - kmin[1]) * PB_KMAX];
+    } else {
+      v = 0.0;
+#pragma unroll
+      for (int j = 0; j < 6; ++j) v = v + kWMid[j] * col[((ry >> 1) - 2 + j - kmin[1]) * PB_KMAX];
+    }
+    Y[(x * PB_WMAX + y) * PB_KMAX + kz] = v;
+  }
+  __syncthreads();
+  const int lo = jb.ng, hi = jb.ng + jb.BS;
+  double* out = pool + jb.dst + (long)f * P;
+  for (int i = (int)threadIdx.x; i < W * W * W; i += (int)blockDim.x) {
+    const int z = i % W, y = (i / W) % W, x = i / (W * W);
+    const bool ix = x >= lo && x < hi, iy = y >= lo && y < hi, iz = z >= lo && z < hi;
+    if (ix && iy && iz) {
+      if (!(jb.skip & 64)) continue;
+    } else {
+      // a face slab: ghost along exactly one axis, interior along the other two
+      if (iy && iz && (jb.skip & (x < lo ? 1 : 2))) continue;
+      if (ix && iz && (jb.skip & (y < lo ? 4 : 8))) continue;
+      if (ix && iy && (jb.skip & (z < lo ? 16 : 32))) continue;
+    }
+    const int rz = r0[2] + z;
+    const double* yy = Y + (x * PB_WMAX + y) * PB_KMAX;
+    double v;
+    if ((rz & 1) == 0) {
+      v = yy[(rz >> 1) - kmin[2]];
+    } else {
+      v = 0.0;
+#pragma unroll
+      for (int j = 0; j < 6; ++j) v = v + kWMid[j] * yy[(rz >> 1) - 2 + j - kmin[2]];
+    }
+    out[i] = v;
+  }
+}
+
+hipError_t launch_prolong_blocks(double* pool, const void* jobs, long n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_prolong_block, dim3((unsigned)(n * BSSN_NF)), dim3(1024), 0, st, pool,
+                     (const bssn_prolong_block*)jobs);
   return hipGetLastError();
 }
 

@@ -91,6 +91,7 @@ hipError_t launch_batch_periodic(double* pool, const BlockDesc* blk, const Brick
                                  int grid, hipStream_t st);
 hipError_t launch_halo_copy(double* pool, const void* map, long n, hipStream_t st);
 hipError_t launch_prolong(double* pool, const void* pts, long n, hipStream_t st);
+hipError_t launch_prolong_blocks(double* pool, const void* jobs, long n, hipStream_t st);
 hipError_t launch_indicator(const double* pool, const BlockDesc* blk, int nblk, double* out,
                             hipStream_t st);
 hipError_t launch_batch_hold(const double* U, double* Ya, double* Yb, const BlockDesc* blk,

@@ -60,9 +60,36 @@ def apply_prolongs(pool: np.ndarray, e: np.ndarray) -> None:
         pool[e["dst"] + f * e["dst_stride"]] = out[f]
 
 
-def run_program(pool: np.ndarray, prolongs, copies, pe, ce) -> None:
-    pb = cb = 0
-    for p_end, c_end in zip(pe, ce):
+def job_points(job) -> np.ndarray:
+    """The padded-block points a whole-block job writes: ghosts outside the skipped face slabs
+    (a face slab = ghost along one axis, interior along the other two), and the interior if
+    bit 6 is set."""
+    BS, ng = int(job["BS"]), int(job["ng"])
+    W = BS + 2 * ng
+    c = np.stack(np.meshgrid(*[np.arange(W)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    inside = (c >= ng) & (c < ng + BS)
+    keep = ~np.all(inside, axis=1) if not (job["skip"] & 64) else np.ones(len(c), bool)
+    for ax in range(3):
+        slab = ~inside[:, ax] & np.all(np.delete(inside, ax, axis=1), axis=1)
+        lo = c[:, ax] < ng
+        keep &= ~(slab & lo & bool(job["skip"] & (1 << (2 * ax))))
+        keep &= ~(slab & ~lo & bool(job["skip"] & (1 << (2 * ax + 1))))
+    return c[keep]
+
+
+def expand_jobs(jobs) -> np.ndarray:
+    """Whole-block jobs as the equivalent point entries."""
+    from bssn3d_hip.amr import prolong_entries
+    out = [prolong_entries(job_points(j), int(j["dst"]), int(j["parent"]), tuple(j["corner"]),
+                           int(j["BS"]), int(j["ng"])) for j in jobs]
+    from bssn3d_hip.amr import PROLONG_DTYPE
+    return np.concatenate(out) if out else np.empty(0, dtype=PROLONG_DTYPE)
+
+
+def run_program(pool: np.ndarray, prolongs, jobs, copies, pe, be, ce) -> None:
+    pb = bb = cb = 0
+    for p_end, b_end, c_end in zip(pe, be, ce):
         apply_prolongs(pool, prolongs[pb:p_end])
+        apply_prolongs(pool, expand_jobs(jobs[bb:b_end]))
         apply_copies(pool, copies[cb:c_end])
-        pb, cb = p_end, c_end
+        pb, bb, cb = p_end, b_end, c_end

@@ -114,6 +114,37 @@ def test_prolong_kernel_bitwise(BS, ng):
         assert np.array_equal(got[(i + 1) * P:(i + 2) * P].reshape(24, W, W, W), want), c
 
 
+@pytest.mark.parametrize("BS,ng", [(8, 3), (16, 4)])
+def test_prolong_blocks_bitwise(BS, ng):
+    """Whole-block jobs (separable passes through LDS) write exactly the restatement's
+    prolongation at their points and nothing elsewhere, for every octant and several skip
+    masks."""
+    say(f"block prolongation BS={BS} ng={ng}")
+    from amr_emul import job_points
+    from bssn3d_hip.amr import block_job
+    W = BS + 2 * ng
+    P = 24 * W ** 3
+    parent = np.random.default_rng(BS + 1).standard_normal((24, W, W, W))
+    skips = [0, 64, 0b010101, 0b101010 | 64]
+    jobs = [(c, sk) for c in CORNERS for sk in skips]
+    pool = torch.full(((len(jobs) + 1) * P,), -7.0, dtype=torch.float64, device=DEV)
+    pool[:P] = torch.from_numpy(parent.reshape(-1)).to(DEV)
+    e = np.concatenate([block_job((i + 1) * P, 0, c, BS, ng, sk) for i, (c, sk) in
+                        enumerate(jobs)])
+    d = torch.from_numpy(e.view(np.uint8).copy()).to(DEV)
+    _lib.check(_lib.load().bssn_prolong_blocks_f64(pool.data_ptr(), d.data_ptr(), len(e),
+                                                   stream_handle(pool.device)))
+    got = pool.cpu().numpy()
+    for i, (c, sk) in enumerate(jobs):
+        blk = got[(i + 1) * P:(i + 2) * P].reshape(24, W, W, W)
+        want = OA.prolongate(parent, c, BS, ng)
+        pts = job_points(e[i])
+        mask = np.zeros((W, W, W), bool)
+        mask[pts[:, 0], pts[:, 1], pts[:, 2]] = True
+        assert np.array_equal(blk[:, mask], want[:, mask]), (c, sk)
+        assert np.all(blk[:, ~mask] == -7.0), (c, sk)
+
+
 def test_indicator_bitwise():
     say("indicator")
     H = three_level()

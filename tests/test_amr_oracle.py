@@ -26,10 +26,11 @@ import pytest
 
 import amr as OA
 import oracle as O
-from amr_emul import apply_prolongs, run_program
+from amr_emul import apply_prolongs, expand_jobs, run_program
 
 from bssn3d_hip import initial_data as idata
-from bssn3d_hip.amr import AMRHierarchy, prolong_entries, restrict_entries
+from bssn3d_hip.amr import (AMRHierarchy, build_ghost_program, prolong_entries,
+                            restrict_entries)
 from bssn3d_hip.grid import Grid
 from bssn3d_hip.state import PhysicsParams, scalars
 
@@ -137,17 +138,21 @@ def test_program_emulation_equals_oracle_sync():
     pool = _rand(H.pool_doubles, 3)
     want = OA.sync_all({k: v.copy() for k, v in _views(H, pool).items()}, _hier(H), H.nb_root,
                        H.BS, H.ng)
-    pr, cp, pe, ce = H.program_entries()
-    assert len(pe) == 3 and pe[0] == 0 and ce[0] > 0
-    run_program(pool, pr, cp, pe, ce)
-    for k, v in _views(H, pool).items():
-        assert np.array_equal(v, want[k]), k
+    for use_blocks in (True, False):
+        P = pool.copy()
+        pr, jb, cp, pe, be, ce = build_ghost_program(H.topo, H.nb_root, H.ng, H.slot_offset,
+                                                     use_blocks)
+        assert len(pe) == 3 and pe[0] == 0 and ce[0] > 0
+        assert (len(jb) > 0) == use_blocks and (len(pr) > 0) != use_blocks
+        run_program(P, pr, jb, cp, pe, be, ce)
+        for k, v in _views(H, P).items():
+            assert np.array_equal(v, want[k]), (use_blocks, k)
 
 
 def test_program_covers_every_ghost_once():
     H = three_level()
-    pr, cp, pe, ce = H.program_entries()
-    dst = np.concatenate([pr["dst"], cp["dst"]])
+    pr, jb, cp, pe, be, ce = H.program_entries()
+    dst = np.concatenate([pr["dst"], expand_jobs(jb)["dst"], cp["dst"]])
     assert len(np.unique(dst)) == len(dst)
     W, ng, BS = H.W, H.ng, H.BS
     nghost = W ** 3 - BS ** 3
