@@ -37,7 +37,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .batch import HALO_DTYPE, Block, BlockBatch
+from .batch import HALO_DTYPE, Block, BlockBatch, _device
 from .grid import Grid
 from .rhs import stream_handle
 from .state import NUM_VARS, PhysicsParams
@@ -420,7 +420,7 @@ class AMRHierarchy:
         self.nb_root = tuple(int(v) for v in nb_root)
         self.BS, self.ng, self.levels = int(BS), int(ng), int(levels)
         self.W = self.BS + 2 * self.ng
-        self.device = torch.device(device)
+        self.device = _device(device)
         self.L = float(L)
         self.lo = tuple(float(v) for v in lo)
         nroot = int(np.prod(self.nb_root))

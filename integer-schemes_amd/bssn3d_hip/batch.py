@@ -40,6 +40,14 @@ HALO_DTYPE = np.dtype([("dst", "<i8"), ("src", "<i8"), ("dst_stride", "<i8"),
                        ("src_stride", "<i8")])
 
 
+def _device(device) -> torch.device:
+    """A device with its index (so that "cuda" and "cuda:0" compare equal)."""
+    d = torch.device(device)
+    if d.type == "cuda" and d.index is None:
+        d = torch.device("cuda", torch.cuda.current_device())
+    return d
+
+
 @dataclass(frozen=True)
 class Block:
     """One block: interior extents n, spacings h, coordinates lo of its first interior point,
@@ -97,7 +105,7 @@ class BlockBatch(BlockLayout):
     def __init__(self, blocks: Sequence[Block], ng: int = 3, device="cuda", align: int = 64,
                  offsets: Sequence[int] | None = None, pool_doubles: int | None = None):
         super().__init__(blocks, ng, align, offsets, pool_doubles)
-        self.device = torch.device(device)
+        self.device = _device(device)
         off = self.off
         n = len(self.blocks)
         self._S = (ctypes.c_int * (3 * n))(*[v for s in self.S for v in s])
