@@ -153,8 +153,18 @@ def bench_int8(N: int, reps: int) -> dict:
     for f in range(24):
         den = max(float(ref[f].abs().max()), 1e-12)
         err.append(float((out8[f] - ref[f]).abs().max()) / den)
+    # per derivative: the INT8 bundle (still in D) against the FP64 bundle
+    D64 = torch.empty_like(D)
+    _lib.check(lib.bssn_deriv_bundle_f64(U.data_ptr(), D64.data_ptr(), Sx, Sy, Sz, 4, h3,
+                                         stream_handle(dev)))
+    _lib.check(lib.bssn_deriv_bundle_i8_f64(U.data_ptr(), D.data_ptr(), Sx, Sy, Sz, 4, h3, 7,
+                                            stream_handle(dev)))
+    derr = []
+    for i in range(138):
+        den = max(float(D64[i].abs().max()), 1e-12)
+        derr.append(float((D[i] - D64[i]).abs().max()) / den)
     npts = N ** 3
-    del D, out8, ref
+    del D, D64, out8, ref
     torch.cuda.empty_cache()
     # the RK4 step in this form: per stage sync + INT8 bundle + FP64 algebra + KO, full-array
     # RK4 updates, enforce, sync (the reference's composition)
@@ -180,7 +190,10 @@ def bench_int8(N: int, reps: int) -> dict:
             "bundle_ms": round(ms_bundle, 4),
             "bundle_hbm_frac": round(138 * 8 * Sx * Sy * Sz / (ms_bundle * 1e-3) / PEAK_HBM, 4),
             "max_normwise_err_vs_fp64_rhs": float("%.3e" % max(err)),
-            "median_normwise_err_vs_fp64_rhs": float("%.3e" % float(np.median(err)))}
+            "median_normwise_err_vs_fp64_rhs": float("%.3e" % float(np.median(err))),
+            "max_normwise_err_vs_fp64_derivative": float("%.3e" % max(derr)),
+            "median_normwise_err_vs_fp64_derivative": float("%.3e" % float(np.median(derr))),
+            "worst_derivative_index": int(np.argmax(derr))}
 
 
 def bench_batched(reps: int) -> dict:
