@@ -599,6 +599,27 @@ class AMRHierarchy:
             float(dt), params.chi_floor, params.alpha_floor, tptr, stream_handle(self.device)))
         return U
 
+    def capture_steps(self, U: torch.Tensor, tdev: torch.Tensor, dt: float, nsteps: int = 1,
+                      **kw) -> torch.cuda.CUDAGraph:
+        """Capture ``nsteps`` hierarchy steps of U into a HIP graph (time in tdev, advanced by
+        each replay). The topology must not change while the graph is in use."""
+        self.workspace()
+        prog = self.program
+        warm = U.clone()                  # first launches (code loading) outside the capture
+        self.step_(warm, 0.0, dt, **kw)
+        torch.cuda.synchronize(self.device)
+        del warm
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream(device=self.device)
+        s.wait_stream(torch.cuda.current_stream(self.device))
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                for _ in range(nsteps):
+                    self.step_(U, 0.0, dt, tdev=tdev, **kw)
+        torch.cuda.current_stream(self.device).wait_stream(s)
+        self._graph_keep = prog           # the captured launches point at this program
+        return g
+
     def evolve(self, U: torch.Tensor, t: float, dt: float, nsteps: int, **kw) -> float:
         for _ in range(nsteps):
             self.step_(U, t, dt, **kw)

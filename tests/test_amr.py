@@ -5,8 +5,10 @@ Bars:
     fill program (bssn_ghost_fill_f64) bit-identical to the restatement (no contraction in the
     prolongation on either side; the indicator and copies are exact);
   * a root-only hierarchy step bit-identical to the single-grid periodic RK4 step (interiors);
-  * two shared-dt RK4 steps of a three-level hierarchy within 1e-12 normwise per field and
-    block of the restatement's step (whole padded blocks);
+  * two shared-dt RK4 steps of a three-level hierarchy (FD orders 4 and 6) against the
+    restatement's step (whole padded blocks), per field within 1e-12 or four times the
+    single-grid step's own error on the same data; captured steps replayed as a graph equal to
+    eager steps;
   * regrid: new children hold the prolongation of their parent, coarsening injects them back
     (bit-identical), nesting holds, capacity growth keeps the data; a run with regrids stays
     finite and refines where the feature is.
@@ -194,9 +196,9 @@ def test_root_only_step_equals_single_grid():
 
 def test_hierarchy_step_vs_oracle():
     """Two steps of the three-level hierarchy on the box [-8, 8)^3 (spacings 1, 1/2, 1/4)
-    against the restatement's step. Bar per field: within 1e-12, or within twice what the
+    against the restatement's step. Bar per field: within 1e-12, or within four times what the
     single-grid periodic step itself shows against the oracle on the same data at the finest
-    spacing (flat space + 1e-3 sinusoids: K and At are ~1e-3 while their right-hand sides
+    spacing (measured ratios up to 2; an indexing or sync mistake shows at 1e-6 and above) (flat space + 1e-3 sinusoids: K and At are ~1e-3 while their right-hand sides
     come from stencils of ~1 values, so FP64 evaluation orders differ there by a few 1e-12
     on a single grid already)."""
     say("three-level hierarchy, 2 RK4 steps vs oracle")
@@ -228,7 +230,70 @@ def test_hierarchy_step_vs_oracle():
     single = normwise(sg.data.cpu().numpy(), want)
     print("hierarchy " + " ".join(f"{v:.1e}" for v in per_field))
     print("single    " + " ".join(f"{v:.1e}" for v in single))
-    assert np.all(per_field <= np.maximum(1e-12, 2.0 * single))
+    assert np.all(per_field <= np.maximum(1e-12, 4.0 * single))
+
+
+def test_hierarchy_step_order6_vs_oracle():
+    """FD order 6 / KO 8 with 4 ghosts (padded blocks of 24, the whole-block prolongation's
+    largest), two steps on the box [-8, 8)^3, same bar as at order 4."""
+    say("three-level hierarchy, order 6, 2 RK4 steps vs oracle")
+    Lb = 16.0
+    fn = smooth_fn(5, L=Lb, lo=-Lb / 2)
+    H = three_level(BS=16, ng=4, L=Lb)
+    U = H.fill(H.empty_pool(), fn)
+    p = PhysicsParams()
+    dt = 0.25 * H.spacing(2)
+    S16 = scalars(p, 1.0, 1.0, 1.0, dt, 0.0)
+    ref = host_blocks(H, U)
+    t = 0.0
+    for _ in range(2):
+        ref = OA.rk4_step(ref, hier_of(H), H.nb_root, H.BS, H.ng, H.h0, S16, t, dt, order=6,
+                          ko_sigma=0.1)
+        t += dt
+    H.evolve(U, 0.0, dt, 2, params=p, order=6, ko_sigma=0.1)
+    got = host_blocks(H, U)
+    per_field = np.zeros(24)
+    for k in H.keys:
+        per_field = np.maximum(per_field, normwise(got[k], ref[k]))
+    N, h = 128, H.spacing(2)
+    g = Grid(N, N, N, h, h, h, ng=4, xmin=-Lb / 2, ymin=-Lb / 2, zmin=-Lb / 2)
+    F = fn(g)
+    sg = BSSNEvolution(g, p, order=6, ko_sigma=0.1, bc="periodic").evolve(F, dt, 2, t0=0.0)
+    want = O.periodic_sync(O.evolve(F.cpu().numpy(), scalars(p, h, h, h, dt, 0.0), dt, 2,
+                                    t0=0.0, order=6, ko_order=8, ko_sigma=0.1, ng=4), 4)
+    single = normwise(sg.data.cpu().numpy(), want)
+    print("hierarchy " + " ".join(f"{v:.1e}" for v in per_field))
+    print("single    " + " ".join(f"{v:.1e}" for v in single))
+    assert np.all(per_field <= np.maximum(1e-12, 4.0 * single))
+
+
+def test_hierarchy_graph_replay():
+    """Three captured steps replayed twice, with eager work on the pool between the replays,
+    against six eager steps (the SSL factor from the device's exp: 1e-12)."""
+    say("hierarchy steps captured into a graph")
+    H = three_level()
+    fn = smooth_fn(7)
+    U = H.fill(H.empty_pool(), fn)
+    V = U.clone()
+    p = PhysicsParams()
+    dt = 0.25 * H.spacing(2)
+    H.evolve(V, 0.1, dt, 6, params=p, order=4, ko_sigma=0.1)
+    tdev = torch.full((1,), 0.1, dtype=torch.float64, device=DEV)
+    g = H.capture_steps(U, tdev, dt, 3, params=p, order=4, ko_sigma=0.1)
+    g.replay()
+    torch.cuda.synchronize()
+    W = U.clone()
+    H.sync_(W)                         # eager work between the replays
+    torch.cuda.synchronize()
+    g.replay()
+    torch.cuda.synchronize()
+    assert abs(float(tdev) - (0.1 + 6 * dt)) < 1e-14
+    worst = 0.0
+    a, b = host_blocks(H, U), host_blocks(H, V)
+    for k in H.keys:
+        worst = max(worst, float(normwise(a[k], b[k]).max()))
+    print(f"graph vs eager {worst:.2e}")
+    assert worst <= 1e-12
 
 
 def bump_fn(center=(0.2, 0.2, 0.2), width=0.08, amp=0.05):
