@@ -773,6 +773,11 @@ static const real MS_A30 = -241289.0 / 5880000.0, MS_A31 = 22846301.0 / 16170000
 static const real MS_B0 = -191.0 / 882.0, MS_B1 = 48241.0 / 59994.0,
                     MS_B2 = 193750.0 / 4351347.0, MS_B3 = 100000.0 / 271791.0;
 
+void orc_msrk_coeffs(real* c) {
+  c[0] = MS_A20; c[1] = MS_A21; c[2] = MS_A30; c[3] = MS_A31; c[4] = MS_A32;
+  c[5] = MS_B0; c[6] = MS_B1; c[7] = MS_B2; c[8] = MS_B3;
+}
+
 int orc_msrk_step(real* U, real* fprev, int Sx, int Sy, int Sz, const real* S16,
                   int order, int ko_order, real ko_sigma, int ng, int periodic, real t,
                   real dt, real chi_floor, real alpha_floor, real* work) {

@@ -83,6 +83,9 @@ int orc_rk4_step(real* U, int Sx, int Sy, int Sz, const real* S16, int order, in
                  real ko_sigma, int ng, int periodic, real t, real dt,
                  real chi_floor, real alpha_floor, real* work);
 
+/* The rk4_2_2 coefficients the step uses: a20, a21, a30, a31, a32, b0, b1, b2, b3. */
+void orc_msrk_coeffs(real* c);
+
 /* One MSRK rk4_2_2 step (evolve.py:139-153). fprev holds f(y_{n-1}) on entry and f(y_n) on
  * exit. work must hold 6 states. */
 int orc_msrk_step(real* U, real* fprev, int Sx, int Sy, int Sz, const real* S16,

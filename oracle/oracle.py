@@ -325,6 +325,13 @@ def msrk_step(F: np.ndarray, fprev: np.ndarray, S16, t: float, dt: float, order:
     return U, fp
 
 
+def msrk_coeffs() -> np.ndarray:
+    """a20, a21, a30, a31, a32, b0, b1, b2, b3 of the oracle's rk4_2_2 step."""
+    c = np.zeros(9)
+    lib().orc_msrk_coeffs(_d(c))
+    return c
+
+
 def evolve(F: np.ndarray, S16, dt: float, nsteps: int, t0: float = 0.0,
            integrator: str = "rk4", **kw) -> np.ndarray:
     """BSSNEvolution.evolve semantics (3D/src/bssn3d/evolve.py:155-194)."""

@@ -231,6 +231,40 @@ def test_msrk_coefficients():
     assert F_(-241289, 5880000) + F_(22846301, 16170000) + F_(-936169, 2587200) == F_(101, 100)
 
 
+def _intercept(radius, hi=4.0, tol=1e-7):
+    """Largest B with spectral radius <= 1 at z = iB, by bisection along the imaginary axis."""
+    lo = 0.0
+    while radius(1j * hi) <= 1.0 + 1e-9:
+        hi *= 2.0
+    while hi - lo > tol:
+        mid = 0.5 * (lo + hi)
+        lo, hi = (mid, hi) if radius(1j * mid) <= 1.0 + 1e-9 else (lo, mid)
+    return 0.5 * (lo + hi)
+
+
+def test_msrk_stability_intercept_matches_paper():
+    """The oracle's rk4_2_2 coefficients, applied to y' = lambda y (z = lambda dt), give the
+    two-step recurrence y_{n+1} = P(z) y_n + Q(z) y_{n-1}; the companion matrix [[P, Q], [1, 0]]
+    has spectral radius <= 1 on the imaginary axis up to 2.46201, the published intercept the
+    reference checks (2D/tests/integration/test_msrk.py:18-28, msrk.py:112-157). Classic RK4's
+    sqrt(8) is the control."""
+    a20, a21, a30, a31, a32, b0, b1, b2, b3 = O.msrk_coeffs()
+
+    def step(z, yn, ynm1):
+        g0, g1 = z * ynm1, z * yn
+        g2 = z * (yn + a20 * g0 + a21 * g1)
+        g3 = z * (yn + a30 * g0 + a31 * g1 + a32 * g2)
+        return yn + b0 * g0 + b1 * g1 + b2 * g2 + b3 * g3
+
+    def radius(z):
+        C = np.array([[step(z, 1.0, 0.0), step(z, 0.0, 1.0)], [1.0, 0.0]], dtype=complex)
+        return float(np.max(np.abs(np.linalg.eigvals(C))))
+    rk4 = lambda z: abs(1 + z + z * z / 2 + z ** 3 / 6 + z ** 4 / 24)
+    assert abs(_intercept(rk4) - np.sqrt(8.0)) < 1e-5
+    assert abs(_intercept(radius) - 2.46201) < 1e-4
+    assert abs(sum((b0, b1, b2, b3)) - 1.0) < 1e-15
+
+
 def _gauge_wave_np(n, amp=0.01, ng=3):
     h = 1.0 / n
     x = -0.5 + (np.arange(n + 2 * ng) - ng) * h
