@@ -137,3 +137,36 @@ def test_binary_config_shape():
     g, st, pos = idata.config_binary(seed=0, N=16, L=16.0)
     assert st.shape == (24, 22, 22, 22) and g.dx == 1.0
     assert abs((pos[1][0] - pos[0][0]) - 6.0) < 1e-12
+
+
+def _det_gt(d):
+    g = d[3:9]
+    return (g[0] * (g[3] * g[5] - g[4] * g[4]) - g[1] * (g[1] * g[5] - g[4] * g[2]) +
+            g[2] * (g[1] * g[4] - g[3] * g[2]))
+
+
+def _conformal_trace_At(d):
+    g, A = d[3:9], d[12:18]
+    iu = [g[3] * g[5] - g[4] * g[4], g[2] * g[4] - g[1] * g[5], g[1] * g[4] - g[2] * g[3],
+          g[0] * g[5] - g[2] * g[2], g[1] * g[2] - g[0] * g[4], g[0] * g[3] - g[1] * g[1]]
+    return (iu[0] * A[0] + iu[3] * A[3] + iu[5] * A[5] +
+            2.0 * (iu[1] * A[1] + iu[2] * A[2] + iu[4] * A[4]))
+
+
+def test_gauge_wave_invariants():
+    """3D/tests/unit/test_bssn_state.py:93-102, and the wave on each axis (:105-...): unit-
+    determinant conformal metric, trace-free At, alpha^2 = chi^-3, Gt only along the axis."""
+    g = Grid.from_domain(12, order=4)
+    for axis, amp in ((0, 0.01), (1, 0.02), (2, 0.02)):
+        d = idata.gauge_wave(g, amplitude=amp, axis=axis).numpy()
+        assert np.max(np.abs(_det_gt(d) - 1.0)) < 1e-12
+        assert np.max(np.abs(_conformal_trace_At(d))) < 1e-12
+        assert np.all(d[1] > 0.0) and np.all(d[0] > 0.0)
+        assert np.max(np.abs(d[0] ** 2 - d[1] ** -3.0)) < 1e-12
+        H = d[0] ** 2                                    # alpha = sqrt(H)
+        for a, slot in enumerate((3, 6, 8)):             # H^(2/3) on the wave axis, else chi
+            want = H ** (2.0 / 3.0) if a == axis else d[1]
+            assert np.max(np.abs(d[slot] - want)) < 1e-12
+        for a in range(3):
+            gt = np.abs(d[18 + a]).max()
+            assert (gt > 0.0) if a == axis else (gt == 0.0)
