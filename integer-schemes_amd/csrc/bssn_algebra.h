@@ -24,12 +24,18 @@
 //   double ko()                Kreiss-Oliger sum over the three axes (KO builds only)
 //   double center(int f)       field f at the point, any time
 // Out must provide: void put(int f, double rhs)   -- called exactly once per field,
-//                   void metric_done()             -- after the last gt_ij and At_ij put.
+//                   void metric_done()             -- after the last gt_ij and At_ij put;
+// and may overload out_phase<PH>(Out&), called at the start of phase PH (0 gt, 1 chi, 2 Gt,
+// 3 alpha, 4 beta, 5 At, 6 K, 7 B), to issue loads ahead of its puts.
 #pragma once
 
 #include "bssn_params.h"
 
 namespace bssn {
+
+// phase hook of bssn_rhs_point: nothing by default
+template <int PH, class Out>
+__device__ __forceinline__ void out_phase(Out&) {}
 
 #ifndef BSSN_PIN
 #define BSSN_PIN(x) (x) = pin(x)
@@ -94,6 +100,7 @@ __device__ __forceinline__ void bssn_rhs_point(Src& s, const Scal& P, Out& o) {
   for (int q = 0; q < 6; ++q) BSSN_PIN(ig[q]);
 
   // ---------------------------------------------------------------- gt_ij: Christoffels, Rt
+  out_phase<0>(o);
   double Rt[6];        // accumulates the Ricci tensor
   double rg[6];        // gt right-hand sides (advection + KO so far)
   double C2[3][6], CalGt[3];
@@ -179,6 +186,7 @@ __device__ __forceinline__ void bssn_rhs_point(Src& s, const Scal& P, Out& o) {
   }
 
   // ---------------------------------------------------------------- chi: Ricci chi part
+  out_phase<1>(o);
   double chi, dchi[3];
   double rchi;
   {
@@ -212,6 +220,7 @@ __device__ __forceinline__ void bssn_rhs_point(Src& s, const Scal& P, Out& o) {
   }
 
   // ---------------------------------------------------------------- Gt^k: d_(i Gt_j) term
+  out_phase<2>(o);
   double advGt[3], koGt[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -230,6 +239,7 @@ __device__ __forceinline__ void bssn_rhs_point(Src& s, const Scal& P, Out& o) {
   // R_ij is complete from here on.
 
   // ---------------------------------------------------------------- alpha: D_iD_j alpha
+  out_phase<3>(o);
   const double K = s.center(KTR);
   double al, dal[3], rGt[3], rK, rA[6];
   {
@@ -274,6 +284,7 @@ __device__ __forceinline__ void bssn_rhs_point(Src& s, const Scal& P, Out& o) {
   // The Christoffels and the Ricci tensor are dead from here on.
 
   // ---------------------------------------------------------------- beta^i
+  out_phase<4>(o);
   double db[3][3];   // db[j][i] = d_j beta^i
   {
     const double Bf = 0.75 * (P.lamf[0] + P.lamf[1] * al);
@@ -306,6 +317,7 @@ __device__ __forceinline__ void bssn_rhs_point(Src& s, const Scal& P, Out& o) {
   o.put(CHI, rchi);
 
   // ---------------------------------------------------------------- At_ij, and gt_ij
+  out_phase<5>(o);
   {
     double A[6];
 #pragma unroll
@@ -345,6 +357,7 @@ __device__ __forceinline__ void bssn_rhs_point(Src& s, const Scal& P, Out& o) {
   }
 
   // ---------------------------------------------------------------- K
+  out_phase<6>(o);
   {
     s.stage(KTR);
     double dK[3];
@@ -360,6 +373,7 @@ __device__ __forceinline__ void bssn_rhs_point(Src& s, const Scal& P, Out& o) {
   }
 
   // ---------------------------------------------------------------- B^i and Gt^i
+  out_phase<7>(o);
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     s.stage(BB0 + i);

@@ -36,7 +36,8 @@ void k_fused(StageArgs args, Dims d, Scales w, Scal sc) {
   if (!brick_origin<G>(d, &ox, &oy, &oz)) return;
   const int lane = (int)threadIdx.x;
   WaveSrc<ORD, KO, MODE, RhsOrder, fused_depth<KIND>()> src(lds, args.Yin, w, d, ox, oy, oz, lane);
-  StageOut<KIND> out(args, d.P, src.gc, lane_valid<G>(d, ox, oy, oz, lane));
+  StageOut<KIND> out(args, d.P, src.gc, lane_valid<G>(d, ox, oy, oz, lane),
+                     true);
   bssn_rhs_point<KO>(src, stage_scal(sc, args.tdev, args.toff, args.ssl_sig), out);
 }
 

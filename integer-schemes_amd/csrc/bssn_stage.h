@@ -52,15 +52,29 @@ constexpr int fused_waves() {
   return KIND == KIND_RHS ? BSSN_RHS_WAVES : BSSN_STAGE_WAVES;
 }
 
-// Fields in flight while one is computed: deeper at 1 wave per SIMD, where no other wave hides
-// the load latency.
+// Fields in flight while one is computed (at 1 wave per SIMD, where no other wave hides the
+// load latency). Measured at 256^3 (FD4/KO6): the bare RHS is faster with two fields in flight
+// (6.37 vs 7.08 ms); the RK stages, which also hold write-back operands, spill with two and
+// are slower (42.5 vs 40.7 ms per step), so they keep one.
+#ifndef BSSN_RHS_DEPTH
+#define BSSN_RHS_DEPTH 2
+#endif
 #ifndef BSSN_STAGE_DEPTH
 #define BSSN_STAGE_DEPTH 1
 #endif
 template <int KIND>
 constexpr int fused_depth() {
-  return fused_waves<KIND>() == 1 ? BSSN_STAGE_DEPTH : 1;
+  return fused_waves<KIND>() != 1 ? 1 : (KIND == KIND_RHS ? BSSN_RHS_DEPTH : BSSN_STAGE_DEPTH);
 }
+
+// Write-back operands are loaded this many phases of bssn_rhs_point before the phase that
+// stores their field (phases: 0 gt, 1 chi, 2 Gt, 3 alpha, 4 beta, 5 At, 6 K, 7 B). Measured at
+// 256^3 (FD4/KO6), ms per RK4 / MSRK step: all at kernel start 43.0 / 35.1, lead 0 40.9 / 32.7,
+// lead 1 40.8 / 32.7, lead 2 40.7 / 32.6: loaded at the start they stay live through the
+// register-heaviest phases (the Christoffels and the Ricci tensor) and push values into AGPRs.
+#ifndef BSSN_WB_LEAD
+#define BSSN_WB_LEAD 2
+#endif
 
 // The write-back kind is a template parameter: each launch runs the code of its own kind only
 // (a runtime switch over all kinds made the stage kernel half again as large).
@@ -72,22 +86,50 @@ struct StageOut {
   bool valid;
   double gnew[6], anew[6];
 
-  // The write-back operands of all 24 fields are loaded at the start, so their latency is
-  // hidden behind the whole evaluation instead of stalling each put.
+  // The write-back operands are loaded ahead of their stores, so their latency is hidden
+  // behind the evaluation instead of stalling each put: all at construction (deferred =
+  // false), or, under bssn_rhs_point, field by field BSSN_WB_LEAD phases before the phase
+  // that stores the field (deferred = true, loads issued by phase<PH>()).
   static constexpr bool NU = KIND != KIND_RHS;
   static constexpr bool NK = KIND == KIND_RK2 || KIND == KIND_RK4 || KIND == KIND_MS2;
   static constexpr bool NF = KIND == KIND_MS1 || KIND == KIND_MS2 || KIND == KIND_MS3;
   double pu[NU ? BSSN_NF : 1], pk[NK ? BSSN_NF : 1], pp[NF ? BSSN_NF : 1];
+  bool deferred;
 
-  __device__ __forceinline__ StageOut(const StageArgs& a, long P_, long gp_, bool v)
-      : A(a), P(P_), gp(gp_), valid(v) {
+  __device__ __forceinline__ StageOut(const StageArgs& a, long P_, long gp_, bool v,
+                                      bool defer = false)
+      : A(a), P(P_), gp(gp_), valid(v), deferred(defer) {
 #pragma unroll
-    for (int f = 0; f < BSSN_NF; ++f) {
-      const long i = (long)f * P + gp;
-      if constexpr (NU) pu[f] = A.U[i];
-      if constexpr (NK) pk[f] = A.K1[i];
-      if constexpr (NF) pp[f] = A.FP[i];
-    }
+    for (int f = 0; f < BSSN_NF; ++f)
+      if (!defer) load(f);
+  }
+
+  __device__ __forceinline__ void load(int f) {
+    const long i = (long)f * P + gp;
+    if constexpr (NU) pu[f] = A.U[i];
+    if constexpr (NK) pk[f] = A.K1[i];
+    if constexpr (NF) pp[f] = A.FP[i];
+  }
+
+  // the bssn_rhs_point phase that stores field f, and the one that loads its operands
+  static constexpr int put_phase(int f) {
+    return f == ALPHA ? 3
+         : (f >= BETA0 && f <= BETA2) || f == CHI ? 4
+         : (f >= GT0 && f <= GT5) || (f >= AT0 && f <= AT5) ? 5
+         : f == KTR ? 6
+         : 7;   // B^i, Gt^i
+  }
+  static constexpr int load_phase(int f) {
+    return put_phase(f) - BSSN_WB_LEAD < 0 ? 0 : put_phase(f) - BSSN_WB_LEAD;
+  }
+
+  // called by bssn_rhs_point at the start of phase PH
+  template <int PH>
+  __device__ __forceinline__ void phase() {
+    if (!deferred) return;
+#pragma unroll
+    for (int f = 0; f < BSSN_NF; ++f)
+      if (load_phase(f) == PH) load(f);
   }
 
   __device__ __forceinline__ void final_value(int f, double u) {
@@ -141,6 +183,13 @@ struct StageOut {
     }
   }
 };
+
+// bssn_rhs_point's phase hook for the stage write-back (more specialised than its generic,
+// empty hook)
+template <int PH, int KIND>
+__device__ __forceinline__ void out_phase(StageOut<KIND>& o) {
+  o.template phase<PH>();
+}
 
 // Call fn(std::integral_constant<int, K>) with K the template kind of the runtime stage kind
 // (MSRK kinds only if MSRK is set).
