@@ -262,6 +262,7 @@ struct WaveSrc : BoxStencil<Geo<ORD, KO>, Coef<ORD>> {
   int cel[G::PER];      // box indices of this lane's staged cells
   double pf[DEPTH][G::PER];   // the next DEPTH fields' cells, in flight (slot = position % DEPTH)
   long gc;              // flat index of the lane's point (clamped into the grid)
+  static constexpr bool kEarlyCentres = true;   // bssn_rhs_point: prefetch centres
 
   __device__ __forceinline__ WaveSrc(double* lds_, const double* F_, const Scales& sc_,
                                      const Dims& d, int ox, int oy, int oz, int lane)
