@@ -365,8 +365,13 @@ __global__ __launch_bounds__(256) void k_periodic_sync(double* __restrict__ F, D
   }
   if (!ghost) return;
   const long q = ((long)s[0] * d.S[1] + s[1]) * d.S[2] + s[2];
-#pragma unroll 4
-  for (int f = 0; f < BSSN_NF; ++f) F[(long)f * d.P + p] = F[(long)f * d.P + q];
+  // all 24 loads before the stores (the source is interior, the destination a ghost), so they
+  // are in flight together instead of each waiting behind the previous store
+  double v[BSSN_NF];
+#pragma unroll
+  for (int f = 0; f < BSSN_NF; ++f) v[f] = F[(long)f * d.P + q];
+#pragma unroll
+  for (int f = 0; f < BSSN_NF; ++f) F[(long)f * d.P + p] = v[f];
 }
 
 hipError_t launch_periodic_sync(double* F, const Dims& d, hipStream_t st) {

@@ -57,8 +57,13 @@ __global__ __launch_bounds__(64) void k_batch_periodic(double* __restrict__ pool
   const long dst = ((long)c[0] * blk.S[1] + c[1]) * blk.S[2] + c[2];
   const long src = ((long)s[0] * blk.S[1] + s[1]) * blk.S[2] + s[2];
   double* u = pool + blk.off;
-#pragma unroll 4
-  for (int f = 0; f < BSSN_NF; ++f) u[(long)f * P + dst] = u[(long)f * P + src];
+  // all 24 loads before the stores (a ghost destination is never an interior source), so they
+  // are in flight together instead of each waiting behind the previous store
+  double v[BSSN_NF];
+#pragma unroll
+  for (int f = 0; f < BSSN_NF; ++f) v[f] = u[(long)f * P + src];
+#pragma unroll
+  for (int f = 0; f < BSSN_NF; ++f) u[(long)f * P + dst] = v[f];
 }
 
 hipError_t launch_batch_periodic(double* pool, const BlockDesc* blk, const BrickList& ghost,
@@ -118,9 +123,13 @@ __global__ __launch_bounds__(256) void k_halo_copy(double* __restrict__ pool,
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   const bssn_halo_copy m = map[e];
-#pragma unroll 4
-  for (int f = 0; f < BSSN_NF; ++f)
-    pool[m.dst + (long)f * m.dst_stride] = pool[m.src + (long)f * m.src_stride];
+  // all 24 loads before the stores (an entry's interior source is never a ghost destination),
+  // so they are in flight together instead of each waiting behind the previous store
+  double v[BSSN_NF];
+#pragma unroll
+  for (int f = 0; f < BSSN_NF; ++f) v[f] = pool[m.src + (long)f * m.src_stride];
+#pragma unroll
+  for (int f = 0; f < BSSN_NF; ++f) pool[m.dst + (long)f * m.dst_stride] = v[f];
 }
 
 hipError_t launch_halo_copy(double* pool, const void* map, long n, hipStream_t st) {
