@@ -27,7 +27,9 @@ Secondary forms (extra keys, same metric family):
   longrun     the apples gate (fused and INT8-split schemes against the two-pass scheme on
               the reference's three N=24 configurations) with its verdicts and wall time, and
               a 128^3 long run with the reference's carried ghost values; the carried-ghost
-              RK4 step at 256^3 is reported beside the default one (carried_ghost_step_ms);
+              RK4 step at 256^3 is reported beside the default one (carried_ghost_step_ms),
+              and so is the RK4 step with the radiative outer boundary in place of periodic
+              ghosts (sommerfeld_step_ms);
   amr         a live three-level block hierarchy (16^3 blocks) around the binary: RK4 step,
               ghost fill program, a refining regrid pass;
   bfp         the two-pass RHS with the bundle stored at 32 and 48 mantissa bits per value
@@ -492,6 +494,22 @@ def main() -> int:
     ms_carried = c0.elapsed_time(c1) / reps
     del Uc, evo_c
 
+    # the same step with the radiative outer boundary instead of periodic ghosts (per stage:
+    # the fused kernel over the interior with clamped stencils, then the boundary kernel over
+    # the ghost zones), on a copy of the state
+    evo_s = BSSNEvolution(grid, p, order=4, ko_sigma=0.1, bc="sommerfeld", integrator="rk4")
+    Us = U.clone()
+    evo_s.step_(Us, t, dt)
+    b0, b1 = ev(), ev()
+    b0.record()
+    for i in range(reps):
+        evo_s.step_(Us, t + i * dt, dt)
+    b1.record()
+    torch.cuda.synchronize()
+    ms_sommerfeld = b0.elapsed_time(b1) / reps
+    sommerfeld_finite = bool(torch.isfinite(Us).all())
+    del Us, evo_s
+
     # ---------------------------------------------------------------- bare fused RHS
     S16 = scalars(p, grid.dx, grid.dy, grid.dz, dt, t)
     out = torch.empty_like(U)
@@ -568,6 +586,9 @@ def main() -> int:
         "rhs_mpts_s": round(npts / (ms_rhs * 1e-3) / 1e6, 2),
         "sync_ms": round(ms_sync, 4),
         "carried_ghost_step_ms": round(ms_carried, 4),
+        "sommerfeld_step_ms": round(ms_sommerfeld, 4),
+        "sommerfeld_step_mpts_s": round(npts / (ms_sommerfeld * 1e-3) / 1e6, 2),
+        "sommerfeld_finite": sommerfeld_finite,
         "roofline": roof,
     }
     if ms_twopass is not None:
