@@ -340,30 +340,20 @@ hipError_t launch_ko_add(const double* F, double* OUT, const Dims& d, const Scal
 // ------------------------------------------------------------------------- periodic sync
 // Every ghost point takes the value of its periodic image in the interior. One pass suffices:
 // the sources are interior points, which is also what the reference's x-then-y-then-z slab
-// copies produce at edges and corners.
+// copies produce at edges and corners. One thread per ghost point (ghost_point).
 __global__ __launch_bounds__(256) void k_periodic_sync(double* __restrict__ F, Dims d) {
-  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= d.P) return;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ghost_count(d)) return;
   int c[3];
-  long r = p;
-  c[2] = (int)(r % d.S[2]);
-  r /= d.S[2];
-  c[1] = (int)(r % d.S[1]);
-  c[0] = (int)(r / d.S[1]);
-  bool ghost = false;
+  ghost_point(d, t, c);
+  const long p = ((long)c[0] * d.S[1] + c[1]) * d.S[2] + c[2];
   int s[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     const int N = d.S[a] - 2 * d.ng;
-    int v = c[a] - d.ng;
-    if (v < 0 || v >= N) {
-      ghost = true;
-      v %= N;
-      v = v < 0 ? v + N : v;
-    }
-    s[a] = d.ng + v;
+    int v = (c[a] - d.ng) % N;
+    s[a] = d.ng + (v < 0 ? v + N : v);
   }
-  if (!ghost) return;
   const long q = ((long)s[0] * d.S[1] + s[1]) * d.S[2] + s[2];
   // all 24 loads before the stores (the source is interior, the destination a ghost), so they
   // are in flight together instead of each waiting behind the previous store
@@ -375,7 +365,8 @@ __global__ __launch_bounds__(256) void k_periodic_sync(double* __restrict__ F, D
 }
 
 hipError_t launch_periodic_sync(double* F, const Dims& d, hipStream_t st) {
-  const unsigned nb = (unsigned)((d.P + 255) / 256);
+  if (d.ng <= 0) return hipSuccess;
+  const unsigned nb = (unsigned)((ghost_count(d) + 255) / 256);
   hipLaunchKernelGGL(k_periodic_sync, dim3(nb), dim3(256), 0, st, F, d);
   return hipGetLastError();
 }

@@ -21,44 +21,6 @@
 
 namespace bssn {
 
-// The ghost points of a padded block, enumerated without the interior: x-ghost planes
-// (2 ng x S1 x S2), then x-interior / y-ghost slabs (N0 x 2 ng x S2), then x- and y-interior /
-// z-ghost bars (N0 x N1 x 2 ng), N = S - 2 ng; one thread per ghost point keeps every lane of a
-// wave busy (a launch over all points would run the boundary work for 3 lanes of the waves at
-// each end of every interior z row).
-__host__ __device__ inline long ghost_count(const Dims& d) {
-  const long g2 = 2L * d.ng, n0 = d.S[0] - g2, n1 = d.S[1] - g2;
-  return g2 * d.S[1] * d.S[2] + n0 * g2 * d.S[2] + n0 * n1 * g2;
-}
-
-// coordinates of ghost point t < ghost_count(d)
-__device__ __forceinline__ void ghost_point(const Dims& d, long t, int* c) {
-  const int ng = d.ng, g2 = 2 * ng;
-  const long nA = (long)g2 * d.S[1] * d.S[2];
-  const long nB = (long)(d.S[0] - g2) * g2 * d.S[2];
-  // ghost layer k in [0, 2 ng) of an axis of extent S: the low layers, then the high ones
-  auto layer = [ng, g2](int k, int S) { return k < ng ? k : S - g2 + k; };
-  if (t < nA) {
-    c[2] = (int)(t % d.S[2]);
-    t /= d.S[2];
-    c[1] = (int)(t % d.S[1]);
-    c[0] = layer((int)(t / d.S[1]), d.S[0]);
-  } else if (t < nA + nB) {
-    t -= nA;
-    c[2] = (int)(t % d.S[2]);
-    t /= d.S[2];
-    c[1] = layer((int)(t % g2), d.S[1]);
-    c[0] = ng + (int)(t / g2);
-  } else {
-    t -= nA + nB;
-    const int n1 = d.S[1] - g2;
-    c[2] = layer((int)(t % g2), d.S[2]);
-    t /= g2;
-    c[1] = ng + (int)(t % n1);
-    c[0] = ng + (int)(t / n1);
-  }
-}
-
 template <int ORD, int KIND>
 __global__ __launch_bounds__(256) void k_sommerfeld(StageArgs a, Dims d, Scales w, BcGeom g) {
   using C = Coef<ORD>;
