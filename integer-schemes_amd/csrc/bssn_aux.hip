@@ -13,12 +13,31 @@
 
 #include <math.h>
 
+#include <type_traits>
+
 #include "bssn_algebra.h"
 #include "bssn_brick.h"
 #include "bssn_kernels.h"
 #include "bssn_params.h"
 
 namespace bssn {
+
+// field(f, slot) for f = 0..23 in a loop that is not unrolled: one field per iteration, or with
+// two staging slots two per iteration, so that each slot index is a compile-time constant
+template <int DEPTH, class Fn>
+__device__ __forceinline__ void for_fields(Fn&& field) {
+  if constexpr (DEPTH == 1) {
+#pragma unroll 1
+    for (int f = 0; f < BSSN_NF; ++f) field(f, std::integral_constant<int, 0>());
+  } else {
+    static_assert(DEPTH == 2 && BSSN_NF % 2 == 0, "fields are stepped in pairs");
+#pragma unroll 1
+    for (int f = 0; f < BSSN_NF; f += 2) {
+      field(f, std::integral_constant<int, 0>());
+      field(f + 1, std::integral_constant<int, 1>());
+    }
+  }
+}
 
 // second-derivative slot of a field (alpha, chi, gt0..5, beta0..2), -1 otherwise
 __host__ __device__ constexpr int s2slot(int f) {
@@ -69,10 +88,14 @@ __global__ __launch_bounds__(64) void k_deriv_bundle(const double* __restrict__ 
   if (!brick_origin<G>(d, &ox, &oy, &oz)) return;
   const int lane = (int)threadIdx.x;
   const bool valid = lane_valid<G>(d, ox, oy, oz, lane);
-  WaveSrc<ORD, false, CLAMP, StateOrder> s(lds, F, w, d, ox, oy, oz, lane);
-#pragma unroll 1
-  for (int f = 0; f < BSSN_NF; ++f) {
-    s.stage(f);
+  // FD4 fp64 / BFP32 stores: two fields in flight, stepped in pairs so each staging slot is a
+  // constant (256^3: two-pass RHS with KO 28.6 -> 26.8 ms, BFP32 with fixed exponents 15.9 ->
+  // 13.1 ms). The BFP48 store and FD order 6 keep one (their extra registers cost more waves
+  // per SIMD than the second field gains: BFP48 19.0 -> 21.1 ms with two).
+  constexpr int DEPTH = ORD == 4 && STORE != BS_Q48 ? 2 : 1;
+  WaveSrc<ORD, false, CLAMP, StateOrder, DEPTH> s(lds, F, w, d, ox, oy, oz, lane);
+  auto field = [&](int f, auto slot) {
+    s.template stage_slot<decltype(slot)::value>(f);
     double v[9];
 #pragma unroll
     for (int a = 0; a < 3; ++a) v[a] = s.d1(a);
@@ -89,7 +112,8 @@ __global__ __launch_bounds__(64) void k_deriv_bundle(const double* __restrict__ 
         bundle_put<STORE>(o, 72 + 6 * sl + q, d.P, s.gc, v[3 + q], valid);
     }
     __builtin_amdgcn_wave_barrier();
-  }
+  };
+  for_fields<DEPTH>(field);
 }
 
 template <int STORE>
@@ -149,10 +173,10 @@ __global__ __launch_bounds__(64) void k_bundle_absmax(const double* __restrict__
     const int bz = L % nbz, t = L / nbz, by = t % nby, bx = t / nby;
     const int ox = d.lo[0] + bx * G::B, oy = d.lo[1] + by * G::B, oz = d.lo[2] + bz * G::B;
     const bool valid = lane_valid<G>(d, ox, oy, oz, lane);
+    // one field in flight (two measured no faster here)
     WaveSrc<ORD, false, CLAMP, StateOrder> s(lds, F, w, d, ox, oy, oz, lane);
-#pragma unroll 1
-    for (int f = 0; f < BSSN_NF; ++f) {
-      s.stage(f);
+    auto field = [&](int f, auto slot) {
+      s.template stage_slot<decltype(slot)::value>(f);
       double v[9];
 #pragma unroll
       for (int a = 0; a < 3; ++a) v[a] = s.d1(a);
@@ -168,7 +192,8 @@ __global__ __launch_bounds__(64) void k_bundle_absmax(const double* __restrict__
         for (int q = 0; q < 6; ++q) keep(72 + 6 * sl + q, v[3 + q], valid);
       }
       __builtin_amdgcn_wave_barrier();
-    }
+    };
+    for_fields<1>(field);
   }
   atomicMax(amax + lane, (unsigned long long)__double_as_longlong(r0));
   atomicMax(amax + 64 + lane, (unsigned long long)__double_as_longlong(r1));
@@ -304,19 +329,21 @@ __global__ __launch_bounds__(64) void k_ko_add(const double* __restrict__ F,
   if (!brick_origin<G>(d, &ox, &oy, &oz)) return;
   const int lane = (int)threadIdx.x;
   const bool valid = lane_valid<G>(d, ox, oy, oz, lane);
-  WaveSrc<ORD, true, MODE, StateOrder> s(lds, F, w, d, ox, oy, oz, lane);
-  // the next field's RHS value is loaded one iteration ahead (gc is clamped into the grid, so
-  // the load is in bounds for every lane): two-pass RHS with KO at 256^3 29.0 -> 28.6 ms
+  // FD4: two fields in flight, stepped in pairs (constant staging slots); FD6 keeps one. The
+  // next field's RHS value is loaded one iteration ahead (gc is clamped into the grid, so the
+  // load is in bounds for every lane).
+  constexpr int DEPTH = ORD == 4 ? 2 : 1;
+  WaveSrc<ORD, true, MODE, StateOrder, DEPTH> s(lds, F, w, d, ox, oy, oz, lane);
   double o = OUT[s.gc];
-#pragma unroll 1
-  for (int f = 0; f < BSSN_NF; ++f) {
-    s.stage(f);
+  auto field = [&](int f, auto slot) {
+    s.template stage_slot<decltype(slot)::value>(f);
     const double k = s.ko();
     const long i = (long)f * d.P + s.gc;
     if (valid) OUT[i] = o + k;
     if (f + 1 < BSSN_NF) o = OUT[i + d.P];
     __builtin_amdgcn_wave_barrier();
-  }
+  };
+  for_fields<DEPTH>(field);
 }
 
 hipError_t launch_ko_add(const double* F, double* OUT, const Dims& d, const Scales& w, int ord,

@@ -304,6 +304,18 @@ struct WaveSrc : BoxStencil<Geo<ORD, KO>, Coef<ORD>> {
     if (at + DEPTH < ORDER::N) fetch(slot, ORDER::at(at + DEPTH));
   }
 
+  // stage(f) with the slot given at compile time (SLOT must equal ORDER::pos(f) % DEPTH): for
+  // per-field loops that are not unrolled, which step DEPTH fields per iteration (a slot index
+  // known only at run time would put the in-flight cells in scratch memory)
+  template <int SLOT>
+  __device__ __forceinline__ void stage_slot(int f) {
+    const int at = ORDER::pos(f);
+#pragma unroll
+    for (int k = 0; k < G::PER; ++k) lds[cel[k]] = pf[SLOT][k];
+    __builtin_amdgcn_wave_barrier();
+    if (at + DEPTH < ORDER::N) fetch(SLOT, ORDER::at(at + DEPTH));
+  }
+
   __device__ __forceinline__ double center(int f) const { return F[(long)f * P + gc]; }
 };
 
