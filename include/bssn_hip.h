@@ -48,6 +48,15 @@ const char* bssn_last_error(void);
 int bssn_rhs_f64(const double* F, const double* S16, double* OUT, int Sx, int Sy, int Sz,
                  int fd_order, double ko_sigma, void* stream);
 
+/* bssn_rhs_f64 with the scalar pack in device memory (16 doubles, the S16 order): the
+ * reference's handler takes S as a device buffer, Fused(stream, F: (24,Sx,Sy,Sz), S: (16,))
+ * (3D/src/bssn3d/cuda/rhs_fused.cu:22-61; S built on the device by _pack_S,
+ * fused_rhs_cuda.py:52-55,70-71). Nothing is read on the host, so the call neither copies
+ * nor synchronises and can be captured into a graph. The pack's contents are not validated
+ * (the host never sees them); results equal bssn_rhs_f64's with the same pack bit for bit. */
+int bssn_rhs_sdev_f64(const double* F, const double* S16dev, double* OUT, int Sx, int Sy, int Sz,
+                      int fd_order, double ko_sigma, void* stream);
+
 /* Two-pass RHS, pass 1: the 138-derivative bundle at every padded point.
  * Replaces derivative_bundle (3D/src/bssn3d/derivative_bundle.py:31-48) and the device
  * derivative stage deriv_kernel FFI (3D/src/bssn3d/cuda/deriv_2p5d.cu:30-113).
@@ -110,6 +119,16 @@ int bssn_rk4_step_devtime_f64(double* U, double* work, int Sx, int Sy, int Sz,
                               const double* S16, int fd_order, double ko_sigma, int ng,
                               int periodic, double* tdev, double dt, double chi_floor,
                               double alpha_floor, void* stream);
+
+/* bssn_rk4_step_f64 with the scalar pack in device memory: dt = S16dev[2], the step time is
+ * S16dev[6], the stage times and coefficients are formed on the device, and S16dev[6] is
+ * advanced by dt at the end (a captured step replays step after step). The state, the time
+ * and every scalar stay on the device: the reference's evolve loop hands Fused a device S per
+ * stage (3D/src/bssn3d/evolve.py:86-90,125-132 over rhs_fused.cu:22-61). Results equal
+ * bssn_rk4_step_f64's with the same pack, t = S16dev[6], dt = S16dev[2], bit for bit. */
+int bssn_rk4_step_sdev_f64(double* U, double* work, int Sx, int Sy, int Sz, double* S16dev,
+                           int fd_order, double ko_sigma, int ng, int periodic, double chi_floor,
+                           double alpha_floor, void* stream);
 
 /* Radiative (Sommerfeld) outer boundary, for puncture runs on a finite domain: the right-hand
  * side of the ng-deep ghost zones, d_t u = -(x d_x u + y d_y u + z d_z u) / r with the normal

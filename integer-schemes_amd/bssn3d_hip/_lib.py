@@ -28,6 +28,11 @@ SIGNATURES = {
     "bssn_last_error": (ctypes.c_char_p, []),
     "bssn_rhs_f64": (_c_int, [_c_void_p, _PD, _c_void_p, _c_int, _c_int, _c_int, _c_int,
                               _c_double, _c_void_p]),
+    "bssn_rhs_sdev_f64": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_int, _c_int, _c_int,
+                                   _c_int, _c_double, _c_void_p]),
+    "bssn_rk4_step_sdev_f64": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, _c_int,
+                                        _c_void_p, _c_int, _c_double, _c_int, _c_int, _c_double,
+                                        _c_double, _c_void_p]),
     "bssn_deriv_bundle_f64": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, _c_int, _c_int,
                                        _PD, _c_void_p]),
     "bssn_deriv_bundle_i8_f64": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, _c_int, _c_int,

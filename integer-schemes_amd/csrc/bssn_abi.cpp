@@ -40,7 +40,9 @@ Scal make_scal(const double* S, double dt, double t) {
   s.lamf[0] = S[11];
   s.lamf[1] = S[12];
   s.ssl_h = S[4];
-  s.ssl_e = std::exp(-0.5 * t * t / (S[5] * S[5]));
+  s.ssl_t = t;
+  s.ssl_sig = S[5];
+  s.ssl_e = 0.0;   // formed on the device from ssl_t and ssl_sig
   s.cahd = cahd_factor(S[1], S[3], dt);
   return s;
 }
@@ -134,18 +136,22 @@ int carried_input(const double*& yin, double* spare, size_t NS, const Dims& full
   return check_hip(launch_periodic_sync(const_cast<double*>(yin), full, st), "stage sync");
 }
 
-// RK4 step; with tdev set, every stage reads its time from device memory (*tdev + offset)
-int rk4_step(double* U, double* work, int Sx, int Sy, int Sz, const double* S16, int fd_order,
-             double ko_sigma, int ng, int periodic, double t, const double* tdev, double dt,
-             double chi_floor, double alpha_floor, void* stream) {
+// RK4 step; with tdev set, every stage reads its time from device memory (*tdev + offset);
+// with Sdev set, the whole scalar pack is read from device memory (S16, t and dt unused):
+// each stage forms dt, its time and its coefficient there, in the host's rounding.
+int rk4_step(double* U, double* work, int Sx, int Sy, int Sz, const double* S16,
+             const double* Sdev, int fd_order, double ko_sigma, int ng, int periodic, double t,
+             const double* tdev, double dt, double chi_floor, double alpha_floor, void* stream) {
   if (!U || !work) return fail(BSSN_E_ARG, "null pointer");
   if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4 or 6");
   if (!(ko_sigma >= 0.0)) return fail(BSSN_E_ARG, "ko_sigma must be >= 0");
   if (periodic < 0 || periodic > 2) return fail(BSSN_E_ARG, "periodic must be 0, 1 or 2");
-  if (!(dt != 0.0) || !std::isfinite(dt) || !std::isfinite(t))
-    return fail(BSSN_E_ARG, "dt must be finite and nonzero, t finite");
+  if (!Sdev) {
+    if (!(dt != 0.0) || !std::isfinite(dt) || !std::isfinite(t))
+      return fail(BSSN_E_ARG, "dt must be finite and nonzero, t finite");
+    TRY(check_s16(S16));
+  }
   TRY(check_shape(Sx, Sy, Sz));
-  TRY(check_s16(S16));
   if (periodic) TRY(check_ng(Sx, Sy, Sz, ng));
   hipStream_t st = (hipStream_t)stream;
   const long P = (long)Sx * Sy * Sz;
@@ -153,14 +159,16 @@ int rk4_step(double* U, double* work, int Sx, int Sy, int Sz, const double* S16,
   double* Ya = work;
   double* Yb = work + NS;
   double* KS = work + 2 * NS;
-  const double h3[3] = {S16[13], S16[14], S16[15]};
+  // host-side scalars (a device pack replaces them in the kernel)
+  static const double kNoPack[16] = {0, 0, 1, 1, 0, 1, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1};
+  const double* S = Sdev ? kNoPack : S16;
+  const double h3[3] = {S[13], S[14], S[15]};
   const Scales w = make_scales(h3, fd_order, ko_sigma);
   const bool wrap = periodic == GH_WRAP, carry = periodic == GH_CARRY && ng > 0;
   const Dims full = full_dims(Sx, Sy, Sz, ng);
   const Dims d = wrap ? interior_dims(Sx, Sy, Sz, ng) : full;
   const int mode = wrap ? WRAP : CLAMP;
   const bool ko = ko_sigma > 0.0;
-  const double hdt = 0.5 * dt;
   StageArgs a{};
   a.U = U;
   a.K1 = KS;
@@ -169,20 +177,26 @@ int rk4_step(double* U, double* work, int Sx, int Sy, int Sz, const double* S16,
   a.chi_floor = chi_floor;
   a.alpha_floor = alpha_floor;
   a.tdev = tdev;
-  a.ssl_sig = S16[5];
-  auto stage = [&](int kind, double c0, double toff, const double* yin, double* y, double* spare) {
+  a.sdev = Sdev;
+  a.ko_sigma = ko_sigma;
+  // stage time t + tcoef dt, coefficient c0 = dt c0mul / c0div (host and device alike)
+  auto stage = [&](int kind, double tcoef, double c0mul, double c0div, const double* yin,
+                   double* y, double* spare) {
     if (carry) TRY(carried_input(yin, spare, NS, full, st));
-    a.Yin = yin; a.Y = y; a.kind = kind; a.c0 = c0; a.toff = toff;
-    return run_stage(a, d, w, make_scal(S16, dt, t + toff), fd_order, ko, mode, st);
+    const double toff = tcoef * dt;
+    a.Yin = yin; a.Y = y; a.kind = kind; a.toff = toff;
+    a.c0 = dt * c0mul / c0div;
+    a.tcoef = tcoef; a.c0mul = c0mul; a.c0div = c0div;
+    return run_stage(a, d, w, make_scal(S, dt, t + toff), fd_order, ko, mode, st);
   };
   // k1 = f(y, t);          Y = y + (dt/2) k1    (carried: the synced copy of y lives in Yb)
-  TRY(stage(KIND_RK1, hdt, 0.0, U, Ya, Yb));
+  TRY(stage(KIND_RK1, 0.0, 0.5, 1.0, U, Ya, Yb));
   // k2 = f(Y, t + dt/2);   Y = y + (dt/2) k2
-  TRY(stage(KIND_RK2, hdt, hdt, Ya, Yb, nullptr));
+  TRY(stage(KIND_RK2, 0.5, 0.5, 1.0, Ya, Yb, nullptr));
   // k3 = f(Y, t + dt/2);   Y = y + dt k3
-  TRY(stage(KIND_RK3, dt, hdt, Yb, Ya, nullptr));
+  TRY(stage(KIND_RK3, 0.5, 1.0, 1.0, Yb, Ya, nullptr));
   // k4 = f(Y, t + dt);     y + (dt/6)(k1 + 2k2 + 2k3 + k4), then enforce
-  TRY(stage(KIND_RK4, dt / 6.0, dt, Ya, nullptr, nullptr));
+  TRY(stage(KIND_RK4, 1.0, 1.0, 6.0, Ya, nullptr, nullptr));
   if (wrap && ng > 0)
     return check_hip(launch_periodic_sync(U, full, st), "periodic sync");
   return BSSN_OK;
@@ -211,6 +225,27 @@ int bssn_rhs_f64(const double* F, const double* S16, double* OUT, int Sx, int Sy
   a.Yin = F;
   a.Y = OUT;
   a.kind = KIND_RHS;
+  return run_stage(a, d, w, sc, fd_order, ko_sigma > 0.0, CLAMP, (hipStream_t)stream);
+}
+
+int bssn_rhs_sdev_f64(const double* F, const double* S16dev, double* OUT, int Sx, int Sy, int Sz,
+                      int fd_order, double ko_sigma, void* stream) {
+  if (!F || !OUT || !S16dev) return fail(BSSN_E_ARG, "null pointer");
+  if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4 or 6");
+  if (!(ko_sigma >= 0.0)) return fail(BSSN_E_ARG, "ko_sigma must be >= 0");
+  TRY(check_shape(Sx, Sy, Sz));
+  const double h3[3] = {1.0, 1.0, 1.0};
+  const Scales w = make_scales(h3, fd_order, ko_sigma);   // replaced from the pack
+  Scal sc{};
+  const Dims d = full_dims(Sx, Sy, Sz, 0);
+  StageArgs a{};
+  a.Yin = F;
+  a.Y = OUT;
+  a.kind = KIND_RHS;
+  a.sdev = S16dev;
+  a.ko_sigma = ko_sigma;
+  a.c0mul = 0.0;
+  a.c0div = 1.0;
   return run_stage(a, d, w, sc, fd_order, ko_sigma > 0.0, CLAMP, (hipStream_t)stream);
 }
 
@@ -368,8 +403,8 @@ size_t bssn_step_workspace_bytes(int integrator, int Sx, int Sy, int Sz) {
 int bssn_rk4_step_f64(double* U, double* work, int Sx, int Sy, int Sz, const double* S16,
                       int fd_order, double ko_sigma, int ng, int periodic, double t, double dt,
                       double chi_floor, double alpha_floor, void* stream) {
-  return rk4_step(U, work, Sx, Sy, Sz, S16, fd_order, ko_sigma, ng, periodic, t, nullptr, dt,
-                  chi_floor, alpha_floor, stream);
+  return rk4_step(U, work, Sx, Sy, Sz, S16, nullptr, fd_order, ko_sigma, ng, periodic, t,
+                  nullptr, dt, chi_floor, alpha_floor, stream);
 }
 
 int bssn_rk4_step_devtime_f64(double* U, double* work, int Sx, int Sy, int Sz,
@@ -377,9 +412,18 @@ int bssn_rk4_step_devtime_f64(double* U, double* work, int Sx, int Sy, int Sz,
                               int periodic, double* tdev, double dt, double chi_floor,
                               double alpha_floor, void* stream) {
   if (!tdev) return fail(BSSN_E_ARG, "null device time");
-  TRY(rk4_step(U, work, Sx, Sy, Sz, S16, fd_order, ko_sigma, ng, periodic, 0.0, tdev, dt,
-               chi_floor, alpha_floor, stream));
+  TRY(rk4_step(U, work, Sx, Sy, Sz, S16, nullptr, fd_order, ko_sigma, ng, periodic, 0.0, tdev,
+               dt, chi_floor, alpha_floor, stream));
   return check_hip(launch_time_advance(tdev, dt, (hipStream_t)stream), "time advance");
+}
+
+int bssn_rk4_step_sdev_f64(double* U, double* work, int Sx, int Sy, int Sz, double* S16dev,
+                           int fd_order, double ko_sigma, int ng, int periodic, double chi_floor,
+                           double alpha_floor, void* stream) {
+  if (!S16dev) return fail(BSSN_E_ARG, "null scalar pack");
+  TRY(rk4_step(U, work, Sx, Sy, Sz, nullptr, S16dev, fd_order, ko_sigma, ng, periodic, 0.0,
+               nullptr, 1.0, chi_floor, alpha_floor, stream));
+  return check_hip(launch_pack_time_advance(S16dev, (hipStream_t)stream), "time advance");
 }
 
 int bssn_sommerfeld_rhs_f64(const double* F, double* OUT, int Sx, int Sy, int Sz, int fd_order,
@@ -446,7 +490,6 @@ int bssn_rk4_step_sommerfeld_f64(double* U, double* work, int Sx, int Sy, int Sz
   a.dt = dt;
   a.chi_floor = chi_floor;
   a.alpha_floor = alpha_floor;
-  a.ssl_sig = S16[5];
   // interiors by the fused kernel (clamped stencils read the ghost values as data), ghosts by
   // the radiative condition; both read the stage input and write disjoint points
   auto stage = [&](int kind, double c0, double toff, const double* yin, double* y) {
@@ -895,7 +938,6 @@ int batch_rk4(const bssn_batch* b, double* U, double* work, const double* S16, i
   s.chi_floor = chi_floor;
   s.alpha_floor = alpha_floor;
   s.tdev = tdev;
-  s.ssl_sig = S16[5];
   auto stage = [&](double toff, int kind, double c0, const double* yin, double* y) {
     const BatchArgs a = batch_args(b, interior, S16, dt, t + toff, ko_sigma);
     s.toff = toff;

@@ -287,7 +287,7 @@ __global__ __launch_bounds__(256) void k_algebra_bfp(const double* __restrict__ 
   if (p >= n) return;
   BundleSrcQ<WIDE> s{F, (const double*)Q, Q + BFP_HEADER_BYTES, n, p, 0};
   PlainOut o{OUT, n, p};
-  bssn_rhs_point<false>(s, sc, o);
+  bssn_rhs_point<false>(s, with_ssl(sc, nullptr, 0.0), o);
 }
 
 hipError_t launch_algebra_bfp(const double* F, const void* Q, double* OUT, long npts,
@@ -309,7 +309,7 @@ __global__ __launch_bounds__(256) void k_algebra(const double* __restrict__ F,
   if (p >= n) return;
   BundleSrc s{F, D, n, p, 0};
   PlainOut o{OUT, n, p};
-  bssn_rhs_point<false>(s, sc, o);
+  bssn_rhs_point<false>(s, with_ssl(sc, nullptr, 0.0), o);
 }
 
 hipError_t launch_algebra(const double* F, const double* D, double* OUT, long npts,

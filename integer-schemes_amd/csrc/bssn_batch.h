@@ -81,7 +81,7 @@ __device__ __forceinline__ void batch_brick(double* lds, const StageArgs& args,
     w.sm[a] = uniform(w.sm[a]);
     w.sko[a] = uniform(w.sko[a]);
   }
-  Scal sc = stage_scal(b.base, args.tdev, args.toff, args.ssl_sig);
+  Scal sc = with_ssl(b.base, args.tdev, args.toff);
   sc.cahd = uniform(cahd_factor(b.cahd_c, blk.h[0], b.dt));
   StageArgs a = args;
   a.Yin = shifted(args.Yin, blk.off);

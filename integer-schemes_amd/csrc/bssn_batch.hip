@@ -150,4 +150,14 @@ hipError_t launch_time_advance(double* tdev, double dt, hipStream_t st) {
   return hipGetLastError();
 }
 
+// t += dt inside a scalar pack held in device memory
+__global__ void k_pack_time_advance(double* S) {
+  if (threadIdx.x == 0) S[PK_T] = S[PK_T] + S[PK_DT];
+}
+
+hipError_t launch_pack_time_advance(double* S, hipStream_t st) {
+  hipLaunchKernelGGL(k_pack_time_advance, dim3(1), dim3(64), 0, st, S);
+  return hipGetLastError();
+}
+
 }  // namespace bssn

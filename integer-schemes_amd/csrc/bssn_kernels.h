@@ -33,9 +33,27 @@ struct StageArgs {
   double c0, c1, c2;   // per-kind coefficients
   double dt;
   double chi_floor, alpha_floor;
-  const double* tdev;  // if set, the stage time is *tdev + toff and ssl_e is formed on the device
-  double toff, ssl_sig;
+  const double* tdev;  // if set, the stage time is *tdev + toff
+  double toff;
+  // If sdev is set, the launch's scalars come from the 16-double pack in device memory
+  // instead of the Scales / Scal arguments: dt = sdev[2], stage time sdev[6] + tcoef dt,
+  // c0 = dt c0mul / c0div, stencil scales from sdev[13..15] and ko_sigma.
+  const double* sdev;
+  double ko_sigma, tcoef, c0mul, c0div;
 };
+
+// The scalars of a launch whose pack is in device memory (StageArgs::sdev): the same rounding
+// as the host's unpacking of the same pack (bssn_abi.cpp), so both give identical results.
+template <int ORD>
+__device__ __forceinline__ void args_from_pack(StageArgs& a, Scales& w, Scal& sc) {
+#pragma clang fp contract(off)
+  const double* S = a.sdev;
+  const double dt = S[PK_DT];
+  w = scales_from_pack(S, ORD, a.ko_sigma);
+  sc = scal_from_pack(S, dt, S[PK_T] + a.tcoef * dt);
+  a.dt = uniform(dt);
+  a.c0 = uniform(dt * a.c0mul / a.c0div);
+}
 
 hipError_t launch_fused(const StageArgs& a, const Dims& d, const Scales& w, const Scal& sc, int ord,
                         bool ko, int mode, hipStream_t st);
@@ -43,6 +61,11 @@ hipError_t launch_fused_o4(const StageArgs& a, const Dims& d, const Scales& w, c
                            bool ko, int mode, hipStream_t st);
 hipError_t launch_fused_o6(const StageArgs& a, const Dims& d, const Scales& w, const Scal& sc,
                            bool ko, int mode, hipStream_t st);
+// device-pack instances (StageArgs::sdev set): the bare RHS and RK4 kinds
+hipError_t launch_fused_o4s(const StageArgs& a, const Dims& d, const Scales& w, const Scal& sc,
+                            bool ko, int mode, hipStream_t st);
+hipError_t launch_fused_o6s(const StageArgs& a, const Dims& d, const Scales& w, const Scal& sc,
+                            bool ko, int mode, hipStream_t st);
 
 // ----------------------------------------------------------------------------- block batches
 // A batch holds many blocks of different extents and spacings in one pool: block b is a
@@ -97,6 +120,7 @@ hipError_t launch_indicator(const double* pool, const BlockDesc* blk, int nblk, 
 hipError_t launch_batch_hold(const double* U, double* Ya, double* Yb, const BlockDesc* blk,
                              const BrickList& ghost, int grid, hipStream_t st);
 hipError_t launch_time_advance(double* tdev, double dt, hipStream_t st);
+hipError_t launch_pack_time_advance(double* S, hipStream_t st);
 
 // radiative outer boundary: grid origin (coordinate of the first interior point) and spacings
 struct BcGeom {

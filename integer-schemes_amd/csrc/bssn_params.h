@@ -30,14 +30,23 @@ enum Field : int {
 
 // Physics scalars, unpacked from the reference's 16-double pack
 // (eta, C_CAHD, dt, dx_i, h_ssl, sig_ssl, t, lambda0..3, lambda_f0..1, dx, dy, dz;
-// 3D/src/bssn3d/fused_rhs_cuda.py:52-55), plus two host-folded factors.
+// 3D/src/bssn3d/fused_rhs_cuda.py:52-55), plus two folded factors.
 struct Scal {
   double eta;
   double lam[4];
   double lamf[2];
   double ssl_h;        // h_ssl
-  double ssl_e;        // exp(-t^2 / (2 sig_ssl^2)) at the evaluation time
+  double ssl_t;        // evaluation time t
+  double ssl_sig;      // sig_ssl
+  double ssl_e;        // exp(-t^2 / (2 sig_ssl^2)): always formed on the device (with_ssl),
+                       // so host-pack, device-pack and device-time launches agree bit for bit
   double cahd;         // C_CAHD * dx_i^2 / (dt * (10 dx_i^2 + 1))
+};
+
+// Offsets into the 16-double scalar pack.
+enum PackSlot : int {
+  PK_ETA = 0, PK_CAHD = 1, PK_DT = 2, PK_DXI = 3, PK_SSL_H = 4, PK_SSL_SIG = 5, PK_T = 6,
+  PK_LAM = 7, PK_LAMF = 11, PK_H = 13
 };
 
 // Per-axis scales of the integer stencils: the centred coefficients are applied as their
@@ -98,15 +107,46 @@ __device__ __forceinline__ double uniform(double x) {
   return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
-// The SSL factor exp(-t^2 / (2 sig^2)) at a stage time held in device memory (a captured step
-// replayed with its time advanced on the device), else the host's value in sc.
-__device__ __forceinline__ Scal stage_scal(Scal sc, const double* tdev, double toff,
-                                           double sig) {
-  if (tdev) {
-    const double t = *tdev + toff;
-    sc.ssl_e = uniform(exp(-0.5 * t * t / (sig * sig)));
-  }
+// The SSL factor exp(-t^2 / (2 sig^2)), formed on the device at the evaluation time: sc.ssl_t,
+// or a stage time held in device memory (*tdev + toff: a captured step replayed with its
+// time advanced on the device).
+__device__ __forceinline__ Scal with_ssl(Scal sc, const double* tdev, double toff) {
+  const double t = tdev ? *tdev + toff : sc.ssl_t;
+  sc.ssl_e = uniform(exp(-0.5 * t * t / (sc.ssl_sig * sc.ssl_sig)));
   return sc;
+}
+
+// Scalars of one evaluation from a pack S held in device memory (the reference's Fused(stream,
+// F, S) passes S as a device buffer): the same values, in the same rounding, as the host's
+// unpacking of S at time t (contraction off).
+__device__ __forceinline__ Scal scal_from_pack(const double* S, double dt, double t) {
+#pragma clang fp contract(off)
+  Scal sc;
+  sc.eta = uniform(S[PK_ETA]);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) sc.lam[i] = uniform(S[PK_LAM + i]);
+  sc.lamf[0] = uniform(S[PK_LAMF]);
+  sc.lamf[1] = uniform(S[PK_LAMF + 1]);
+  sc.ssl_h = uniform(S[PK_SSL_H]);
+  sc.ssl_sig = uniform(S[PK_SSL_SIG]);
+  sc.ssl_t = uniform(t);
+  sc.ssl_e = 0.0;
+  sc.cahd = uniform(cahd_factor(S[PK_CAHD], S[PK_DXI], dt));
+  return sc;
+}
+
+// Stencil scales from a device pack (spacings S[13..15]), moved to scalar registers.
+__device__ __forceinline__ Scales scales_from_pack(const double* S, int ord, double sigma) {
+  const double h3[3] = {S[PK_H], S[PK_H + 1], S[PK_H + 2]};
+  Scales w = make_scales(h3, ord, sigma);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    w.s1[a] = uniform(w.s1[a]);
+    w.s2[a] = uniform(w.s2[a]);
+    w.sm[a] = uniform(w.sm[a]);
+    w.sko[a] = uniform(w.sko[a]);
+  }
+  return w;
 }
 
 // Keep a value where it is computed: an empty volatile asm that claims to rewrite x. The

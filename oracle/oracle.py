@@ -359,3 +359,45 @@ def num_threads() -> int:
         return len(os.sched_getaffinity(0))
     except AttributeError:
         return os.cpu_count() or 1
+
+
+_PLD = ctypes.POINTER(ctypes.c_longdouble)
+
+
+def deriv_points(F: np.ndarray, h3, idx, order: int = 4, long_double: bool = False) -> np.ndarray:
+    """The 138-derivative bundle (orc_deriv_points: edge-padded stencils, mixed = nested first
+    derivatives, derivative_bundle.py:31-48) at flat padded indices idx only -> (138, len(idx))
+    float64; long_double evaluates it in the long-double build from the same double inputs
+    (returned rounded to float64)."""
+    idx = np.ascontiguousarray(idx, dtype=np.int64)
+    _, Sx, Sy, Sz = np.shape(F)
+    if long_double:
+        Fl = np.ascontiguousarray(np.asarray(F, dtype=np.float64).astype(np.longdouble))
+        hl = np.ascontiguousarray(np.asarray(h3, dtype=np.float64).astype(np.longdouble))
+        out = np.zeros((138, idx.size), dtype=np.longdouble)
+        lib_ld().orc_deriv_points(Fl.ctypes.data_as(_PLD), Sx, Sy, Sz, hl.ctypes.data_as(_PLD),
+                                  order, ctypes.c_long(idx.size), idx.ctypes.data_as(_PL),
+                                  out.ctypes.data_as(_PLD))
+        return out.astype(np.float64)
+    out = np.zeros((138, idx.size))
+    lib().orc_deriv_points(_d(_c(F)), Sx, Sy, Sz, _d(_c(h3)), order, ctypes.c_long(idx.size),
+                           idx.ctypes.data_as(_PL), _d(out))
+    return out
+
+
+def rk4_step_ld(F: np.ndarray, S16, t: float, dt: float, order: int = 4, ko_order: int = 0,
+                ko_sigma: float = 0.0, ng: int = 3, periodic: bool = True,
+                chi_floor: float = 1e-4, alpha_floor: float = 1e-4) -> np.ndarray:
+    """rk4_step evaluated in long double from the same double inputs: the accuracy reference
+    for FP64 steps. Returns the long-double result (numpy longdouble)."""
+    U = np.ascontiguousarray(np.asarray(F, dtype=np.float64).astype(np.longdouble))
+    Sl = np.ascontiguousarray(np.asarray(S16, dtype=np.float64).astype(np.longdouble))
+    _, Sx, Sy, Sz = U.shape
+    work = np.zeros((6,) + U.shape, dtype=np.longdouble)
+    L = ctypes.c_longdouble
+    rc = lib_ld().orc_rk4_step(U.ctypes.data_as(_PLD), Sx, Sy, Sz, Sl.ctypes.data_as(_PLD),
+                               order, ko_order, L(ko_sigma), ng, int(periodic), L(t), L(dt),
+                               L(chi_floor), L(alpha_floor), work.ctypes.data_as(_PLD))
+    if rc:
+        raise ValueError("orc_rk4_step (long double) failed")
+    return U

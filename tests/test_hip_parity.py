@@ -353,3 +353,28 @@ def test_full_size_binary_sampled():
     rhs_into(mink, out, S16, 4, 0.1)
     torch.cuda.synchronize()
     assert float(out.abs().max()) < 1e-12
+
+
+def test_rk4_stage_times_with_fast_ssl():
+    """RK4 step with sig_ssl = 0.4 (the SSL factor changes ~20 % within the step, so each
+    stage's time matters): the 20^3 puncture against the oracle (1e-12), and a uniform lapse
+    state against the closed-form solution of its slice-locking ODE (oracle-pinned bound)."""
+    say("RK4 step with a fast SSL factor: stage times")
+    from test_oracle import _uniform_lapse_state, ssl_exact
+    p = PhysicsParams(ssl_sigma=0.4)
+    grid, st, _ = idata.config_single_puncture(seed=9, N=20)
+    F = st.numpy()
+    dt, t0 = 0.25 * grid.dx, 0.35
+    evo = BSSNEvolution(grid, p, order=4, ko_sigma=0.1, bc="periodic")
+    got = to_host(evo.step(to_dev(F), t0, dt).data)
+    S16 = scalars(p, grid.dx, grid.dy, grid.dz, dt, t0)
+    ref = O.periodic_sync(O.rk4_step(F, S16, t0, dt, order=4, ko_sigma=0.1, ng=3), 3)
+    inc = np.abs(ref - F).reshape(24, -1).max(axis=1)
+    assert normwise(got, ref, den=inc).max() < TOL
+    a0, dtu = 1.3, 0.02
+    g2 = Grid.from_domain(8, order=4, ko_order=6)
+    U = _uniform_lapse_state(a0, g2.shape)
+    evo2 = BSSNEvolution(g2, p, order=4, ko_sigma=0.1, bc="periodic")
+    out = to_host(evo2.step(to_dev(U), t0, dtu).data)
+    exact = ssl_exact(a0, t0, t0 + dtu, p.ssl_h, p.ssl_sigma)
+    assert abs(out[0, 7, 7, 7] - exact) / (a0 - 1.0) < 1e-9

@@ -57,6 +57,14 @@ def test_argument_errors_before_device_work():
     assert lib.bssn_msrk_step_f64(fake, fake, fake, fake, 12, 12, 12, S, 4, 0.1, 3, 1, 0.0, 0.1,
                                   1e-4, 1e-4, None) == 1
     assert b"distinct" in lib.bssn_last_error()
+    # device scalar pack: null pack, bad order, bad shape are caught on the host
+    assert lib.bssn_rhs_sdev_f64(fake, None, fake, 8, 8, 8, 4, 0.0, None) == 1
+    assert b"null" in lib.bssn_last_error()
+    assert lib.bssn_rhs_sdev_f64(fake, fake, fake, 8, 8, 8, 8, 0.0, None) == 1
+    assert lib.bssn_rk4_step_sdev_f64(fake, fake, 8, 8, 8, None, 4, 0.1, 3, 1, 1e-4, 1e-4,
+                                      None) == 1
+    assert lib.bssn_rk4_step_sdev_f64(fake, fake, 5, 5, 5, fake, 4, 0.1, 3, 1, 1e-4, 1e-4,
+                                      None) == 1   # interior < ng
     assert lib.bssn_step_workspace_bytes(0, 10, 10, 10) == 3 * 24 * 1000 * 8
     assert lib.bssn_step_workspace_bytes(1, 10, 10, 10) == 2 * 24 * 1000 * 8
 

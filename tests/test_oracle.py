@@ -309,3 +309,62 @@ def test_constraint_convergence_gauge_wave():
         errs.append((H, M))
     assert np.log2(errs[0][0] / errs[1][0]) > 3.5
     assert np.log2(errs[0][1] / errs[1][1]) > 3.5
+
+
+def test_sampled_bundle_and_long_double_step():
+    """The sampled-derivative wrapper equals the full bundle at the same points bit for bit;
+    the long-double builds of the bundle and of the RK4 step agree with the FP64 ones to
+    round-off (they are the accuracy references of the full-size GPU checks)."""
+    rng = np.random.default_rng(4)
+    F = 0.05 * rng.standard_normal((24, 11, 10, 12))
+    F[[0, 1, 3, 6, 8]] += 1.0          # alpha, chi and the diagonal of gt near 1
+    h3 = [0.1, 0.11, 0.12]
+    D = O.deriv_bundle(F, h3, 4).reshape(138, -1)
+    idx = rng.integers(0, 11 * 10 * 12, size=40)
+    np.testing.assert_array_equal(O.deriv_points(F, h3, idx, 4), D[:, idx])
+    Dl = O.deriv_points(F, h3, idx, 4, long_double=True)
+    assert np.max(np.abs(Dl - D[:, idx])) <= 1e-12 * np.max(np.abs(D))
+    S16 = [2.0, 0.06, 0.02, 0.1, 0.6, 20.0, 0.0, 1, 1, 1, 1, 1, 0, 0.1, 0.11, 0.12]
+    a = O.rk4_step(F, S16, 0.0, 0.02, order=4, ko_sigma=0.1, ng=3)
+    b = O.rk4_step_ld(F, S16, 0.0, 0.02, order=4, ko_sigma=0.1, ng=3)
+    inc = np.abs(a - F).reshape(24, -1).max(axis=1)
+    err = np.abs(a - b.astype(np.float64)).reshape(24, -1).max(axis=1)
+    assert np.all(err <= 1e-12 * np.maximum(inc, 1e-12)), err / inc
+
+
+def _uniform_lapse_state(alpha0: float, shape=(9, 9, 9)) -> np.ndarray:
+    """A spatially uniform state: chi = 1, gt = delta, alpha = alpha0, everything else 0. All
+    derivatives vanish, K, At, beta, B and Gt stay 0, and the lapse obeys the slice-locking ODE
+    d(alpha - 1)/dt = -h_ssl (alpha - 1) exp(-t^2 / (2 sig^2)) alone."""
+    F = np.zeros((24,) + shape)
+    F[0] = alpha0
+    F[1] = 1.0
+    F[[3, 6, 8]] = 1.0
+    return F
+
+
+def ssl_exact(alpha0: float, t0: float, t1: float, h: float, sig: float) -> float:
+    """alpha(t1) of the uniform-state lapse ODE from alpha(t0) = alpha0, in closed form."""
+    from math import erf, exp, pi, sqrt
+    integral = sig * sqrt(pi / 2.0) * (erf(t1 / (sig * sqrt(2.0))) - erf(t0 / (sig * sqrt(2.0))))
+    return 1.0 + (alpha0 - 1.0) * exp(-h * integral)
+
+
+def test_rk4_stage_times_pinned_by_exact_lapse_solution():
+    """The oracle's RK4 stage times (t, t + dt/2, t + dt/2, t + dt) enter through the SSL
+    factor. With sig_ssl = 0.4 the factor changes by ~20 % over a step, so a stage evaluated at
+    a wrong time misses the closed-form solution by ~1e-5; RK4 itself misses it by O(dt^5)."""
+    h, sig, t0, dt, a0 = 0.6, 0.4, 0.35, 0.02, 1.3
+    F = _uniform_lapse_state(a0)
+    S16 = [2.0, 0.06, dt, 0.1, h, sig, t0, 1, 1, 1, 1, 1, 0, 0.1, 0.1, 0.1]
+    U = O.rk4_step(F, S16, t0, dt, order=4, ko_sigma=0.1, ng=3)
+    exact = ssl_exact(a0, t0, t0 + dt, h, sig)
+    assert np.all(U[0] == U[0, 4, 4, 4])
+    err = abs(U[0, 4, 4, 4] - exact) / (a0 - 1.0)
+    assert err < 1e-9, err
+    # the same step with every stage at the step's start time misses by far more
+    wrong = O.stage_rhs(F, S16, t0, order=4, ko_sigma=0.1, ng=3)[0, 4, 4, 4]
+    assert abs((a0 + dt * wrong) - exact) / (a0 - 1.0) > 1e-5
+    # the other fields stay where they were (to round-off)
+    assert np.max(np.abs(U[2])) < 1e-13 and np.max(np.abs(U[9:12])) < 1e-13
+    assert np.max(np.abs(U[1] - 1.0)) < 1e-13
