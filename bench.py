@@ -24,7 +24,7 @@ Secondary forms (extra keys, same metric family):
               step of all blocks (interiors advanced, ghosts held from the initial data),
               launch shape autotuned, eager and replayed from a captured graph; Mpts/s over
               interior points;
-  longrun     the apples gate (fused and INT8-split schemes against the two-pass scheme on
+  longrun     the tracking gate (fused and INT8-split schemes against the two-pass scheme on
               the reference's three N=24 configurations) with its verdicts and wall time, and
               a 128^3 long run with the reference's carried ghost values; the carried-ghost
               RK4 step at 256^3 is reported beside the default one (carried_ghost_step_ms),
@@ -254,35 +254,34 @@ def bench_batched(reps: int) -> dict:
 
 
 def bench_longrun() -> dict:
-    """The apples gate on the reference's suite (robust / gauge / hambump, N=24, 2 crossing
+    """The tracking gate on the reference's suite (robust / gauge / hambump, N=24, 2 crossing
     times, 8 samples each; fused against the two-pass scheme at FD order 6 and the INT8-split
     scheme at FD order 4), and a long run at 128^3 (robust noise 1e-8, 1 crossing time = 512
     RK4 steps, 4 samples) with the reference's carried ghosts."""
     import torch
-    from bssn3d_hip import apples_gate as ag
-    from bssn3d_hip.longrun import run
+    from bssn3d_hip import tracking_gate as tg
+    from bssn3d_hip.series import SeriesSpec, record
 
     out = {}
     for name, scheme, order in (("gate_fused_vs_twopass_fd6", "fused", 6),
                                 ("gate_int8_vs_twopass_fd4", "int8", 4)):
-        res = []
         w0 = time.perf_counter()
-        ok = ag.gate(scheme, order=order, verbose=False, results=res, device="cuda:0")
+        rep = tg.run_gate(scheme, order=order, device="cuda:0")
         torch.cuda.synchronize()
-        out[name] = {"pass": bool(ok), "wall_s": round(time.perf_counter() - w0, 3),
-                     "worst_dev": float("%.3e" % max(max(r["dev"].values()) for r in res))}
+        out[name] = {"pass": bool(rep.passed), "wall_s": round(time.perf_counter() - w0, 3),
+                     "worst_dev": float("%.3e" % rep.worst)}
     N = 128
-    run("robust", N=N, crossings=0.05, samples=1, ko_sigma=0.1, amp=1e-8, device="cuda:0")
+    record(SeriesSpec("robust", N, 0.05, 1, 0.1, 1e-8), device="cuda:0")
     torch.cuda.synchronize()
     w0 = time.perf_counter()
-    rows = run("robust", N=N, crossings=1.0, samples=4, ko_sigma=0.1, amp=1e-8, device="cuda:0")
+    s = record(SeriesSpec("robust", N, 1.0, 4, 0.1, 1e-8), device="cuda:0")
     torch.cuda.synchronize()
     wall = time.perf_counter() - w0
-    steps = rows[-1]["steps"]
+    steps = int(s.step[-1])
     out["longrun_robust_128^3"] = {
-        "steps": steps, "samples": len(rows), "wall_s": round(wall, 3),
+        "steps": steps, "samples": len(s), "wall_s": round(wall, 3),
         "steps_per_s": round(steps / wall, 2), "mpts_s": round(steps * N ** 3 / wall / 1e6, 2),
-        "final_H": float("%.4e" % rows[-1]["H"]), "finite": all(r["finite"] for r in rows)}
+        "final_H": float("%.4e" % s.H[-1]), "finite": s.all_finite}
     return out
 
 

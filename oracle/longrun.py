@@ -1,6 +1,6 @@
 """Long-run constraint series on the oracle.
 
-TEST INFRASTRUCTURE ONLY: the device long-run harness (bssn3d_hip.longrun) is checked against
+TEST INFRASTRUCTURE ONLY: the device long-run series (bssn3d_hip.series) are checked against
 this. Nothing in the product imports it.
 
 Restates the sampling loop of 3D/src/bssn3d/longrun_stability.py:124-188 on the C oracle: the

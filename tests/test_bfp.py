@@ -9,7 +9,7 @@ Bars:
     dequantized bundle bit for bit;
   * on the reference's Gowdy gate data (N=16, FD order 6) the BFP32 RHS error against the
     fp64 RHS is the restatement's (7.33e-10, relative L2, within 2 %) and BFP48 is below
-    1e-13; the composed BFP RK4 step tracks the fused step; the apples gate passes for the
+    1e-13; the composed BFP RK4 step tracks the fused step; the tracking gate passes for the
     BFP48 and BFP32 schemes.
 """
 from __future__ import annotations
@@ -25,7 +25,7 @@ import oracle as O
 from conftest import gpu_available, normwise
 
 from bssn3d_hip import _lib
-from bssn3d_hip import apples_gate as ag
+from bssn3d_hip import tracking_gate as tg
 from bssn3d_hip import initial_data as idata
 from bssn3d_hip.evolve import BSSNEvolution
 from bssn3d_hip.grid import Grid
@@ -164,10 +164,8 @@ def test_bfp_composed_step_tracks_fused(k, tol):
 
 
 @pytest.mark.parametrize("scheme", ["bfp48", "bfp32"])
-def test_apples_gate_bfp(scheme):
-    say(f"apples gate {scheme}")
-    res = []
-    ok = ag.gate(scheme, order=6, results=res, device=DEV)
-    for r in res:
-        print(r["id"], {q: f"{v:.2e}" for q, v in r["dev"].items()}, r["verdict_match"])
-    assert ok
+def test_tracking_gate_bfp(scheme):
+    say(f"tracking gate {scheme}")
+    rep = tg.run_gate(scheme, order=6, device=DEV)
+    print(rep.table())
+    assert rep.passed

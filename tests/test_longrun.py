@@ -8,8 +8,9 @@ Bars:
   * the device long-run series (fused) against the oracle's series sample by sample (relative
     1e-9) and against the reference's calibrated CAHD decay rates (3D/tests/validation/
     test_bssn_cahd_rate.py:16-18) to their printed digits;
-  * the apples gate (3D/src/bssn3d/apples_gate.py) on the reference's suite: fused against the
-    two-pass (verbatim) scheme at FD order 6, and the INT8-split scheme at FD order 4.
+  * the tracking gate (the reference's apples gate, 3D/src/bssn3d/apples_gate.py) on the
+    reference's suite: fused against the two-pass (verbatim) scheme at FD order 6, and the
+    INT8-split scheme at FD order 4.
 """
 from __future__ import annotations
 
@@ -21,11 +22,11 @@ import longrun as OL
 import oracle as O
 from conftest import gpu_available, normwise
 
-from bssn3d_hip import apples_gate as ag
+from bssn3d_hip import tracking_gate as tg
 from bssn3d_hip import initial_data as idata
 from bssn3d_hip.evolve import BSSNEvolution
 from bssn3d_hip.grid import Grid
-from bssn3d_hip.longrun import chunking, fit_rates, run
+from bssn3d_hip.series import SeriesSpec, record, secular_rates
 from bssn3d_hip.state import PhysicsParams, scalars
 
 pytestmark = [pytest.mark.gpu,
@@ -98,51 +99,50 @@ def test_composed_carried_matches_oracle():
 
 
 def test_longrun_rows_match_oracle_series():
-    say("long-run rows vs oracle (hambump N=12)")
-    rows = run("hambump", N=12, crossings=1.0, samples=4, ko_sigma=0.02, amp=0.005,
-               cahd_c=0.06, device=DEV)
+    say("long-run series vs oracle (hambump N=12)")
+    spec = SeriesSpec("hambump", 12, 1.0, 4, 0.02, 0.005, cahd_c=0.06)
+    got = record(spec, device=DEV)
     g = Grid.from_domain(12, order=6)
     p = PhysicsParams(cahd_c=0.06)
-    dt, total, chunk = chunking(12, 1.0, 4)
+    dt, total, chunk = spec.cadence()
     S16 = scalars(p, g.dx, g.dy, g.dz, dt, 0.0)
     want, _ = OL.series(OL.hambump(g.shape, g.ng, g.dx, 0.005), S16, dt, g.dx, g.ng, total,
                         chunk, 4, 0.02)
-    assert len(rows) == len(want) == 5
-    for a, b in zip(rows, want):
-        assert a["steps"] == b["steps"] and a["finite"] and b["finite"]
-        for m in ("H", "M", "max_alpha", "max_dev"):
-            assert ag._rel(a[m], b[m], 1e-12) <= 1e-9, (m, a[m], b[m])
+    assert len(got) == len(want) == 5 and got.all_finite
+    assert got.step.tolist() == [w["steps"] for w in want]
+    for mine, theirs in (("H", "H"), ("M", "M"), ("alpha_max", "max_alpha"),
+                         ("dev_max", "max_dev")):
+        b = np.array([w[theirs] for w in want])
+        rel = np.abs(got.column(mine) - b) / (np.abs(b) + 1e-12)
+        assert rel.max() <= 1e-9, (mine, rel)
 
 
 @pytest.mark.parametrize("cahd_c,golden,tol", CAHD_GOLDEN)
 def test_longrun_cahd_rates_match_reference(cahd_c, golden, tol):
     say(f"CAHD rate cahd_c={cahd_c}")
-    rows = run("hambump", N=12, crossings=2.0, samples=8, ko_sigma=0.02, amp=0.005,
-               cahd_c=cahd_c, device=DEV)
-    rate = fit_rates(rows)["H_exp"]
-    print(f"H_exp {rate:+.4f} (recorded {golden:+.3f})")
-    assert len(rows) == 9 and abs(rate - golden) <= tol
+    s = record(SeriesSpec("hambump", 12, 2.0, 8, 0.02, 0.005, cahd_c=cahd_c), device=DEV)
+    rate = secular_rates(s)["H_efold"]
+    print(f"H e-folds {rate:+.4f} (recorded {golden:+.3f})")
+    assert len(s) == 9 and abs(rate - golden) <= tol
 
 
 def test_longrun_reference_assertions():
     """The reference's long-run assertions (test_bssn_constraint_growth.py, test_bssn_longrun.py)
     on the device series."""
     say("robust / gauge long runs N=12")
-    rows = run("robust", N=12, crossings=2.0, samples=8, ko_sigma=0.1, amp=1e-8, device=DEV)
-    r = fit_rates(rows)
-    assert all(x["finite"] for x in rows) and r["H_exp"] < 0.2 and rows[-1]["H"] <= rows[0]["H"]
-    assert max(x["max_dev"] for x in rows) < 1e-3
-    rows = run("gauge", N=12, crossings=2.0, samples=8, ko_sigma=0.1, amp=0.01, device=DEV)
-    r = fit_rates(rows)
-    assert all(x["finite"] for x in rows) and r["M_lin"] > 0.0 and rows[-1]["M"] < 0.5
-    assert r["M_lin_vs_exp_resid"] < 3.0 and max(x["max_alpha"] for x in rows) < 2.0
+    s = record(SeriesSpec("robust", 12, 2.0, 8, 0.1, 1e-8), device=DEV)
+    r = secular_rates(s)
+    assert s.all_finite and r["H_efold"] < 0.2 and s.H[-1] <= s.H[0]
+    assert s.dev_max.max() < 1e-3
+    s = record(SeriesSpec("gauge", 12, 2.0, 8, 0.1, 0.01), device=DEV)
+    r = secular_rates(s)
+    assert s.all_finite and r["M_slope"] > 0.0 and s.M[-1] < 0.5
+    assert r["M_line_vs_exp"] < 3.0 and s.alpha_max.max() < 2.0
 
 
 @pytest.mark.parametrize("scheme,order", [("fused", 6), ("int8", 4)])
-def test_apples_gate(scheme, order):
-    say(f"apples gate {scheme} order {order}")
-    res = []
-    ok = ag.gate(scheme, order=order, results=res, device=DEV)
-    for r in res:
-        print(r["id"], {k: f"{v:.2e}" for k, v in r["dev"].items()}, r["verdict_match"])
-    assert ok
+def test_tracking_gate(scheme, order):
+    say(f"tracking gate {scheme} order {order}")
+    rep = tg.run_gate(scheme, order=order, device=DEV)
+    print(rep.table())
+    assert rep.passed

@@ -1,16 +1,16 @@
-"""CPU: the long-run series and the apples gate -- oracle pins and host logic.
+"""CPU: the long-run series and the tracking gate -- oracle pins and host logic.
 
 * The oracle's long-run series (oracle/longrun.py) against the calibrated Hamiltonian decay
   rates the reference records for its CAHD certificate (3D/tests/validation/
   test_bssn_cahd_rate.py:16-18: hambump, N=12, 2 crossings, 8 samples, KO 0.02, amp 0.005,
-  cahd_c = 0 / 0.03 / 0.06 -> H_exp = +0.001 / -2.14 / -3.05 e-folds per crossing). The same
-  series with the ghosts synced before sampling gives -2.21 / -4.41 and fails the pin: the
+  cahd_c = 0 / 0.03 / 0.06 -> H e-folding rate = +0.001 / -2.14 / -3.05 per crossing). The
+  same series with the ghosts synced before sampling gives -2.21 / -4.41 and fails the pin: the
   reference's constraint norms read the ghost values its step carries.
 * The reference's qualitative long-run assertions on robust noise and the gauge wave
   (3D/tests/validation/test_bssn_constraint_growth.py, 3D/tests/integration/
   test_bssn_longrun.py), on the oracle at N=12.
-* fit_rates on closed forms, the initial data, the chunking, and the gate's comparison logic
-  with synthetic trajectories (3D/tests/unit/test_apples_gate.py).
+* secular_rates on closed forms, the initial data, the sampling cadence, and the gate's
+  comparison logic on synthetic series (the cases of 3D/tests/unit/test_apples_gate.py).
 """
 from __future__ import annotations
 
@@ -22,11 +22,26 @@ import pytest
 import longrun as OL
 import oracle as O
 
-from bssn3d_hip import apples_gate as ag
 from bssn3d_hip import initial_data as idata
+from bssn3d_hip import tracking_gate as tg
 from bssn3d_hip.grid import Grid
-from bssn3d_hip.longrun import chunking, fit_rates
+from bssn3d_hip.series import Series, SeriesSpec, secular_rates
 from bssn3d_hip.state import PhysicsParams, scalars
+
+
+def as_series(rows) -> Series:
+    """The oracle's sample rows as a Series."""
+    s = Series.empty()
+    for r in rows:
+        s.append(t=r["ncross"], step=r.get("steps", 0), H=r["H"], M=r["M"],
+                 alpha_max=r.get("max_alpha", 1.0), dev_max=r.get("max_dev", 0.0),
+                 finite=r.get("finite", True))
+    return s
+
+
+def cadence(N, crossings, samples):
+    return SeriesSpec("hambump", N, crossings, samples, 0.0, 0.0).cadence()
+
 
 # (cahd_c, recorded H_exp, half of the last printed digit)
 CAHD_GOLDEN = [(0.0, 0.001, 0.0005), (0.03, -2.14, 0.005), (0.06, -3.05, 0.005)]
@@ -35,7 +50,7 @@ CAHD_GOLDEN = [(0.0, 0.001, 0.0005), (0.03, -2.14, 0.005), (0.06, -3.05, 0.005)]
 def oracle_run(U0, N, crossings, samples, ko_sigma, cahd_c=None, order=6, sync_before=False):
     g = Grid.from_domain(N, order=order)
     p = PhysicsParams() if cahd_c is None else PhysicsParams(cahd_c=cahd_c)
-    dt, total, chunk = chunking(N, crossings, samples)
+    dt, total, chunk = cadence(N, crossings, samples)
     S16 = scalars(p, g.dx, g.dy, g.dz, dt, 0.0)
     rows, U = OL.series(U0, S16, dt, g.dx, g.ng, total, chunk, samples, ko_sigma, order,
                         chi_floor=p.chi_floor, alpha_floor=p.alpha_floor)
@@ -50,14 +65,14 @@ def oracle_run(U0, N, crossings, samples, ko_sigma, cahd_c=None, order=6, sync_b
                 t += chunk * dt
             H, M = O.constraint_l2(O.periodic_sync(Ut, g.ng), (g.dx,) * 3, order, g.ng)
             rows2.append(dict(ncross=t, H=H, M=M))
-        return rows2
-    return rows
+        return as_series(rows2)
+    return as_series(rows)
 
 
-def test_chunking_matches_reference_suite():
-    assert chunking(24, 2.0, 8) == (1.0 / 96, 192, 24)
-    assert chunking(12, 2.0, 8) == (1.0 / 48, 96, 12)
-    dt, total, chunk = chunking(64, 20.0, 40)
+def test_cadence_matches_reference_suite():
+    assert cadence(24, 2.0, 8) == (1.0 / 96, 192, 24)
+    assert cadence(12, 2.0, 8) == (1.0 / 48, 96, 12)
+    dt, total, chunk = cadence(64, 20.0, 40)
     assert total == 5120 and chunk == 128
 
 
@@ -66,16 +81,16 @@ def test_oracle_cahd_rates_match_reference_calibration(cahd_c, golden, tol):
     g = Grid.from_domain(12, order=6)
     U0 = OL.hambump(g.shape, g.ng, g.dx, 0.005)
     rows = oracle_run(U0, 12, 2.0, 8, 0.02, cahd_c)
-    assert len(rows) == 9 and all(r["finite"] for r in rows)
-    rate = fit_rates(rows)["H_exp"]
-    print(f"cahd_c {cahd_c}: H_exp {rate:+.4f} (recorded {golden:+.3f})")
+    assert len(rows) == 9 and rows.all_finite
+    rate = secular_rates(rows)["H_efold"]
+    print(f"cahd_c {cahd_c}: H e-folds {rate:+.4f} (recorded {golden:+.3f})")
     assert abs(rate - golden) <= tol
 
 
 def test_synced_ghost_sampling_fails_the_pin():
     g = Grid.from_domain(12, order=6)
     U0 = OL.hambump(g.shape, g.ng, g.dx, 0.005)
-    rate = fit_rates(oracle_run(U0, 12, 2.0, 8, 0.02, 0.06, sync_before=True))["H_exp"]
+    rate = secular_rates(oracle_run(U0, 12, 2.0, 8, 0.02, 0.06, sync_before=True))["H_efold"]
     assert abs(rate - (-3.05)) > 0.5
 
 
@@ -84,11 +99,11 @@ def test_oracle_robust_noise_bounded_and_h_damped():
     g = Grid.from_domain(12, order=6)
     U0 = idata.robust_stability(g, amp=1e-8, seed=1).numpy()
     rows = oracle_run(U0, 12, 2.0, 8, 0.1)
-    r = fit_rates(rows)
-    assert all(x["finite"] for x in rows)
-    assert r["H_exp"] < 0.2 and rows[-1]["H"] <= rows[0]["H"]
-    assert max(x["max_dev"] for x in rows) < 1e-3
-    assert max(x["max_alpha"] for x in rows) < 2.0
+    r = secular_rates(rows)
+    assert rows.all_finite
+    assert r["H_efold"] < 0.2 and rows.H[-1] <= rows.H[0]
+    assert rows.dev_max.max() < 1e-3
+    assert rows.alpha_max.max() < 2.0
 
 
 def test_oracle_gauge_wave_momentum_drift_subexponential():
@@ -96,12 +111,12 @@ def test_oracle_gauge_wave_momentum_drift_subexponential():
     g = Grid.from_domain(12, order=6)
     U0 = idata.gauge_wave(g, amplitude=0.01, wavelength=1.0).numpy()
     rows = oracle_run(U0, 12, 2.0, 8, 0.1)
-    r = fit_rates(rows)
-    assert all(x["finite"] for x in rows)
-    assert r["M_lin"] > 0.0 and rows[-1]["M"] < 0.5
-    assert r["M_lin_vs_exp_resid"] < 3.0
-    assert max(x["max_alpha"] for x in rows) < 2.0
-    assert max(x["H"] for x in rows) < 1.0 and max(x["M"] for x in rows) < 2.0
+    r = secular_rates(rows)
+    assert rows.all_finite
+    assert r["M_slope"] > 0.0 and rows.M[-1] < 0.5
+    assert r["M_line_vs_exp"] < 3.0
+    assert rows.alpha_max.max() < 2.0
+    assert rows.H.max() < 1.0 and rows.M.max() < 2.0
 
 
 def test_initial_data_hambump_and_robust():
@@ -123,76 +138,88 @@ def test_initial_data_hambump_and_robust():
     assert not np.array_equal(r1, idata.robust_stability(g, amp=1e-8, seed=2).numpy())
 
 
-def test_fit_rates_closed_forms():
+def test_secular_rates_closed_forms():
     t = np.linspace(0.0, 2.0, 9)
-    lin = [dict(ncross=x, H=1.0 + 0.5 * x, M=2.0 + 3.0 * x) for x in t]
-    r = fit_rates(lin)
-    assert r["H_lin"] == pytest.approx(0.5, rel=1e-12)
-    assert r["M_lin"] == pytest.approx(3.0, rel=1e-12)
-    assert r["M_lin_vs_exp_resid"] < 1e-9                     # a line fits a line exactly
-    ex = [dict(ncross=x, H=1e-3 * math.exp(-2.5 * x), M=1e-4 * math.exp(0.7 * x)) for x in t]
-    r = fit_rates(ex)
-    assert r["H_exp"] == pytest.approx(-2.5, rel=1e-12)
-    assert r["M_exp"] == pytest.approx(0.7, rel=1e-12)
-    assert r["M_lin_vs_exp_resid"] > 1e6                      # exponential fits exactly
+    lin = as_series([dict(ncross=x, H=1.0 + 0.5 * x, M=2.0 + 3.0 * x) for x in t])
+    r = secular_rates(lin)
+    assert r["H_slope"] == pytest.approx(0.5, rel=1e-12)
+    assert r["M_slope"] == pytest.approx(3.0, rel=1e-12)
+    assert r["M_line_vs_exp"] < 1e-9                          # a line fits a line exactly
+    ex = as_series([dict(ncross=x, H=1e-3 * math.exp(-2.5 * x), M=1e-4 * math.exp(0.7 * x))
+                    for x in t])
+    r = secular_rates(ex)
+    assert r["H_efold"] == pytest.approx(-2.5, rel=1e-12)
+    assert r["M_efold"] == pytest.approx(0.7, rel=1e-12)
+    assert r["M_line_vs_exp"] > 1e6                           # exponential fits exactly
+    short = as_series([dict(ncross=x, H=1.0, M=1.0) for x in t[:2]])
     with pytest.raises(ValueError):
-        fit_rates(lin[:2])
+        secular_rates(short)
 
 
-# --- gate logic on synthetic trajectories (3D/tests/unit/test_apples_gate.py) ---------------
-
-def _rows(H, M=1e-6, a=1.0, dev=1e-7, finite=True, n=4):
-    return [dict(H=H, M=M, max_alpha=a, max_dev=dev, finite=finite) for _ in range(n)]
-
-
-def _patch(monkeypatch, ref_rows, cand_rows):
-    def fake_run(id_name, **kw):
-        return ref_rows if kw.get("scheme") == "twopass" else cand_rows
-    monkeypatch.setattr(ag, "run", fake_run)
+def test_series_columns_and_save(tmp_path):
+    s = as_series([dict(ncross=0.5 * i, steps=12 * i, H=1e-3 / (i + 1), M=2e-4, finite=i < 3)
+                   for i in range(4)])
+    assert len(s) == 4 and s.step.tolist() == [0, 12, 24, 36] and not s.all_finite
+    s.save(str(tmp_path / "s.txt"))
+    back = np.loadtxt(tmp_path / "s.txt")
+    assert back.shape == (4, 7) and np.array_equal(back[:, 2], s.H)
 
 
-def test_rel_is_floored():
-    assert ag._rel(0.0, 0.0, 1e-12) == 0.0
-    assert ag._rel(1.0 + 1e-9, 1.0, 1e-12) == pytest.approx(1e-9, rel=1e-3)
+# --- gate logic on synthetic series (the cases of 3D/tests/unit/test_apples_gate.py) ---------
+
+def _series(H, M=1e-6, a=1.0, dev=1e-7, finite=True, n=4):
+    return as_series([dict(ncross=0.25 * i, H=H, M=M, max_alpha=a, max_dev=dev, finite=finite)
+                      for i in range(n)])
 
 
-def test_identical_trajectories_pass(monkeypatch):
-    rows = _rows(7.1e-6)
-    _patch(monkeypatch, rows, rows)
-    assert ag.gate("fused", tol=1e-6, quick=True, verbose=False) is True
+def _recorder(ref, cand, seen=None):
+    def rec(spec):
+        if seen is not None:
+            seen.append((spec.data, spec.scheme, spec.n, spec.order))
+        return ref if spec.scheme == "twopass" else cand
+    return rec
 
 
-def test_roundoff_drift_within_tol_passes(monkeypatch):
-    _patch(monkeypatch, _rows(7.1e-6), _rows(7.1e-6 * (1 + 1e-9)))
-    assert ag.gate("fused", tol=1e-6, quick=True, verbose=False) is True
+def test_deviation_is_floored():
+    z = _series(0.0, M=0.0, a=0.0, dev=0.0)
+    assert max(tg.deviation(z, z, 1e-12).values()) == 0.0
+    a, b = _series(1.0 + 1e-9), _series(1.0)
+    assert tg.deviation(a, b, 1e-12)["H"] == pytest.approx(1e-9, rel=1e-3)
 
 
-def test_drift_beyond_tol_fails(monkeypatch):
-    _patch(monkeypatch, _rows(7.1e-6), _rows(7.1e-6 * (1 + 1e-3)))
-    assert ag.gate("fused", tol=1e-6, quick=True, verbose=False) is False
+def test_identical_series_pass():
+    s = _series(7.1e-6)
+    assert tg.run_gate("fused", suite=tg.SUITE[:1], recorder=_recorder(s, s)).passed
 
 
-def test_blowup_point_mismatch_fails(monkeypatch):
-    _patch(monkeypatch, _rows(7.1e-6, n=8), _rows(7.1e-6, n=4))
-    assert ag.gate("fused", tol=1e-6, quick=True, verbose=False) is False
+def test_roundoff_drift_within_tol_passes():
+    rec = _recorder(_series(7.1e-6), _series(7.1e-6 * (1 + 1e-9)))
+    assert tg.run_gate("fused", tol=1e-6, suite=tg.SUITE[:1], recorder=rec).passed
 
 
-def test_finiteness_disagreement_fails(monkeypatch):
-    ref = _rows(7.1e-6, n=4)
-    cand = _rows(7.1e-6, n=4)
-    cand[-1] = dict(cand[-1], finite=False)
-    _patch(monkeypatch, ref, cand)
-    assert ag.gate("fused", tol=1e-6, quick=True, verbose=False) is False
+def test_drift_beyond_tol_fails():
+    rec = _recorder(_series(7.1e-6), _series(7.1e-6 * (1 + 1e-3)))
+    assert not tg.run_gate("fused", tol=1e-6, suite=tg.SUITE[:1], recorder=rec).passed
 
 
-def test_gate_runs_the_whole_suite_and_reports(monkeypatch):
+def test_blowup_point_mismatch_fails():
+    rep = tg.run_gate("fused", suite=tg.SUITE[:1],
+                      recorder=_recorder(_series(7.1e-6, n=8), _series(7.1e-6, n=4)))
+    assert not rep.passed and rep.cases[0].samples == (8, 4)
+    assert "8 vs 4" in rep.table()
+
+
+def test_finiteness_disagreement_fails():
+    ref = _series(7.1e-6, n=4)
+    cand = _series(7.1e-6, n=4)
+    cand.finite[-1] = False
+    assert not tg.run_gate("fused", suite=tg.SUITE[:1], recorder=_recorder(ref, cand)).passed
+
+
+def test_gate_runs_the_whole_suite_and_reports():
     seen = []
-
-    def fake_run(id_name, **kw):
-        seen.append((id_name, kw["scheme"], kw["N"], kw["order"]))
-        return _rows(1e-5)
-    monkeypatch.setattr(ag, "run", fake_run)
-    res = []
-    assert ag.gate("int8", order=4, verbose=False, results=res) is True
-    assert [r["id"] for r in res] == ["robust", "gauge", "hambump"]
-    assert seen == [(c.id_name, s, 24, 4) for c in ag.GATE_SUITE for s in ("twopass", "int8")]
+    s = _series(1e-5)
+    rep = tg.run_gate("int8", order=4, recorder=_recorder(s, s, seen))
+    assert rep.passed and [c.case.data for c in rep.cases] == ["robust", "gauge", "hambump"]
+    assert seen == [(c.data, sch, 24, 4) for c in tg.SUITE for sch in ("twopass", "int8")]
+    assert "overall: pass" in rep.table()
