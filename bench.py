@@ -19,9 +19,10 @@ Secondary forms (extra keys, same metric family):
               256^3 (h = 12/256), with its normwise error against the FP64 fused RHS, and the
               RK4 step composed from it (per stage: sync, bundle, algebra, KO; full-array
               updates);
-  batched     640 cubic blocks of interior 8..32 on three levels (h = 1/4, 1/8, 1/16) with
-              masses 0.909 / 0.0909 4M apart: the RHS of all blocks in one launch and the RK4
-              step of all blocks (interiors advanced, ghosts held from the initial data),
+  batched     640 uneven blocks (extents 8..32 per axis) on three nested levels (h = 1/4,
+              1/8, 1/16) around the binary 0.909 / 0.0909 4M apart: the RHS of all blocks in
+              one launch and the hierarchy RK4 step (one batched launch per stage plus the
+              ghost fill program: periodic root, same-level copies, parent prolongation),
               launch shape autotuned, eager and replayed from a captured graph; Mpts/s over
               interior points;
   longrun     the tracking gate (fused and INT8-split schemes against the two-pass scheme on
@@ -70,32 +71,86 @@ BYTES_STAGES = (576, 960, 960, 768)
 BYTES_RHS = 384
 
 
+def _cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(seconds: float = 12.0):
-    """The oracle's RK4 step (C, OpenMP on all host cores) on the 48^3 single-puncture
-    workload, repeated for about ``seconds``."""
+    """The oracle (oracle/bssn_oracle.c, its -O3 AVX2/FMA timing build liboracle_perf.so) on this
+    host's cores, bounded samples of the workload:
+      value  RK4 steps of the 48^3 single-puncture workload (FD4/KO6) on all host cores, repeated
+             for about ``seconds``;
+      runs   the same step on one core, the RHS (+KO) on one core and on all cores at 48^3, and
+             one RHS evaluation of the 256^3 binary (262^3 padded) on all cores.
+    Mpts/s counts interior points for steps and padded points for RHS evaluations."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as O
     from bssn3d_hip import initial_data as idata
     from bssn3d_hip.state import PhysicsParams, scalars
 
+    O.use_build("perf")
+    ncores = O.num_threads()
     grid, st, _ = idata.config_single_puncture(seed=0, N=48)
     F = st.numpy()
     dt = 0.25 * grid.dx
-    S16 = scalars(PhysicsParams(), grid.dx, grid.dy, grid.dz, dt, 0.0)
-    O.rk4_step(F, S16, 0.0, dt, order=4, ko_sigma=0.1, ng=3)      # warm (page-in, thread pool)
-    n, t0 = 0, time.perf_counter()
-    U = F
-    while True:
-        U = O.rk4_step(U, S16, n * dt, dt, order=4, ko_sigma=0.1, ng=3)
-        n += 1
-        el = time.perf_counter() - t0
-        if el >= seconds:
-            break
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or O.num_threads()
-    return {"value": round(48 ** 3 * n / el / 1e6, 4), "unit": "Mpts/s (RK4 step)",
-            "cores": threads, "kind": "port",
-            "sample": f"{n} RK4 steps of the 48^3 single-puncture workload (54^3 padded), "
-                      f"FD4/KO6, oracle/bssn_oracle.c, {el:.1f} s"}
+    p = PhysicsParams()
+    S16 = scalars(p, grid.dx, grid.dy, grid.dz, dt, 0.0)
+    pad3 = grid.shape[0] ** 3
+
+    def time_loop(fn, budget, min_reps=1):
+        fn()                                   # warm (page-in, thread pool)
+        n, t0 = 0, time.perf_counter()
+        while True:
+            fn()
+            n += 1
+            el = time.perf_counter() - t0
+            if el >= budget and n >= min_reps:
+                return n, el
+
+    def step():
+        return O.rk4_step(F, S16, 0.0, dt, order=4, ko_sigma=0.1, ng=3)
+
+    def rhs():
+        return O.rhs(F, S16, 4, ko_sigma=0.1)
+
+    runs = []
+    O.set_threads(ncores)
+    n, el = time_loop(step, seconds)
+    value = 48 ** 3 * n / el / 1e6
+    sample = (f"{n} RK4 steps of the 48^3 single-puncture workload (54^3 padded), FD4/KO6, "
+              f"{el:.1f} s")
+    n, el = time_loop(rhs, 2.0)
+    runs.append({"what": "rhs+ko 48^3", "cores": ncores, "mpts_s": round(pad3 * n / el / 1e6, 4),
+                 "reps": n, "s": round(el, 2)})
+    O.set_threads(1)
+    n, el = time_loop(step, 3.0)
+    runs.append({"what": "rk4 step 48^3", "cores": 1,
+                 "mpts_s": round(48 ** 3 * n / el / 1e6, 4), "reps": n, "s": round(el, 2)})
+    n, el = time_loop(rhs, 2.0)
+    runs.append({"what": "rhs+ko 48^3", "cores": 1, "mpts_s": round(pad3 * n / el / 1e6, 4),
+                 "reps": n, "s": round(el, 2)})
+    O.set_threads(ncores)
+    del F, st
+    g2, st2, _ = idata.config_binary(seed=0, N=256)
+    F2 = st2.numpy()
+    del st2
+    S2 = scalars(p, g2.dx, g2.dy, g2.dz, 0.25 * g2.dx, 0.0)
+    t0 = time.perf_counter()
+    O.rhs(F2, S2, 4, ko_sigma=0.1)
+    el = time.perf_counter() - t0
+    runs.append({"what": "rhs+ko 256^3 binary (262^3 padded)", "cores": ncores,
+                 "mpts_s": round(F2[0].size / el / 1e6, 4), "reps": 1, "s": round(el, 2)})
+    del F2
+    O.use_build("check")
+    return {"value": round(value, 4), "unit": "Mpts/s (RK4 step)", "cores": ncores,
+            "kind": "port", "sample": sample, "build": "oracle/liboracle_perf.so (-O3 "
+            "-march=x86-64-v3, FMA contraction on)", "cpu_model": _cpu_model(), "runs": runs}
 
 
 def measured_traffic(workload: str):
@@ -199,55 +254,64 @@ def bench_int8(N: int, reps: int) -> dict:
 
 
 def bench_batched(reps: int) -> dict:
-    """640 uneven blocks: batched RHS and RK4 step (blocks periodic on themselves)."""
+    """The batched block-AMR workload: 640 uneven blocks (interior extents per axis from
+    {8, 12, 16, 24, 32}) on three nested levels (h = 1/4, 1/8, 1/16) around the 1:10 binary
+    (m = 0.909, 0.0909, 4M apart); per RK stage one batched launch over every block, then the
+    ghost fill program (root periodic, same-level copies, prolongation from the parent level).
+    The launch shape is autotuned; steps are timed eagerly and replayed from a captured graph."""
     import torch
-    from bssn3d_hip.batch import BlockBatch, config_blocks, fill_punctures, timed
+    from bssn3d_hip.batch import timed
+    from bssn3d_hip.patches import config_hierarchy, fill_binary
     from bssn3d_hip.state import PhysicsParams
 
     dev = torch.device("cuda:0")
-    blocks = config_blocks(seed=0)
-    batch = BlockBatch(blocks, ng=3, device=dev)
-    U0 = batch.empty_pool()
-    fill_punctures(batch, U0, seed=0)
+    w0 = time.perf_counter()
+    H, _ = config_hierarchy(seed=0, device=dev)
+    prog = H.program
+    build_s = time.perf_counter() - w0
+    U0 = fill_binary(H, H.empty_pool())
     p = PhysicsParams()
-    dt = 0.25 * 0.0625
-    # Blocks have no neighbour map here: each step holds the ghosts of the initial data
-    # (closed-form Brill-Lindquist values) and advances the interiors; every timed phase
-    # restarts from the initial data.
-    nrep = reps
-    tune = batch.autotune(U0, dt, p, bc="halo", candidates=[0, 1024, 2048, 4096], reps=2)
+    dt = 0.25 * H.boxes[-1].h
+    kw = dict(params=p, order=4, ko_sigma=0.1)
+    tune = H.tune_launch(U0, dt, candidates=[0, 1024, 2048, 4096], reps=2, **kw)
+    batch = H.batch
     U = U0.clone()
-    OUT = batch.empty_pool()
+    OUT = H.empty_pool()
     batch.rhs_into(U, OUT, p, dt, 0.0, 4, 0.0)
     torch.cuda.synchronize()
     ms_rhs = timed(lambda: batch.rhs_into(U, OUT, p, dt, 0.0, 4, 0.0), reps, dev)
     del OUT
+    ms_sync = timed(lambda: H.sync_(U), reps, dev)
     t = [0.0]
 
     def step():
-        batch.step_(U, t[0], dt, p, 4, 0.1, "halo")
+        H.step_(U, t[0], dt, **kw)
         t[0] += dt
     step()
     torch.cuda.synchronize()
     U.copy_(U0)
-    ms_step = timed(step, nrep, dev)
-    assert bool(torch.isfinite(U).all()), "non-finite batched state"
+    ms_step = timed(step, reps, dev)
+    assert bool(torch.isfinite(U).all()), "non-finite hierarchy state"
     U.copy_(U0)
     tdev = torch.full((1,), 0.0, dtype=torch.float64, device=dev)
-    g = batch.capture_steps(U, tdev, dt, nsteps=1, params=p, bc="halo")
+    g = H.capture_steps(U, tdev, dt, 1, **kw)
     g.replay()
     torch.cuda.synchronize()
     U.copy_(U0)
-    ms_graph = timed(g.replay, nrep, dev)
-    assert bool(torch.isfinite(U).all()), "non-finite batched state"
+    ms_graph = timed(g.replay, reps, dev)
+    assert bool(torch.isfinite(U).all()), "non-finite hierarchy state"
     pts = batch.interior_points
-    return {"workload": "640_blocks_8-32_levels_1/4_1/8_1/16_masses_0.909_0.0909_sep4",
-            "blocks": len(blocks), "interior_points": pts, "padded_points": batch.padded_points,
+    return {"workload": "640_blocks_3_nested_levels_h1/4_1/8_1/16_sizes_8-32_masses_0.909_0.0909"
+                        "_sep4",
+            "blocks": len(H), "blocks_per_level": [b.nblocks for b in H.boxes],
+            "interior_points": pts, "padded_points": batch.padded_points,
+            "ghost_program": {"copies": prog.n_copy, "prolongations": prog.n_prolong,
+                              "fill_ms": round(ms_sync, 4)},
+            "host_build_s": round(build_s, 2),
             "launch_persistent_waves": batch.persistent_waves,
             "autotune_step_ms": {str(k): round(v, 4) for k, v in tune.items()},
             "rhs_ms": round(ms_rhs, 4), "rhs_mpts_s": round(pts / (ms_rhs * 1e-3) / 1e6, 2),
-            "step_bc": "ghosts held from the initial data (no neighbour map)",
-            "steps_timed": nrep,
+            "steps_timed": reps,
             "step_ms": round(ms_step, 4), "step_mpts_s": round(pts / (ms_step * 1e-3) / 1e6, 2),
             "graph_step_ms": round(ms_graph, 4),
             "graph_step_mpts_s": round(pts / (ms_graph * 1e-3) / 1e6, 2)}
@@ -293,7 +357,7 @@ def bench_amr(reps: int) -> dict:
     pass (indicator, flags, transfers, new plan and program)."""
     import torch
     from bssn3d_hip import initial_data as idata
-    from bssn3d_hip.amr import AMRHierarchy
+    from bssn3d_hip.amr import AMRHierarchy, RegridPolicy
     from bssn3d_hip.batch import timed
     from bssn3d_hip.state import CHI, PhysicsParams
 
@@ -303,11 +367,10 @@ def bench_amr(reps: int) -> dict:
     pos = [(-3.0 + 0.01, 0.013, 0.007), (3.0 + 0.01, 0.013, 0.007)]
     fn = lambda g: idata.punctures(g, [0.5, 0.5], pos, amp=0.0, device=dev)
     U = H.fill(H.empty_pool(), fn)
-    kw = dict(refine_threshold=0.02, coarsen_threshold=1e-6, hysteresis_K=1, n_buffer=0,
-              fields=[CHI])
+    pol = RegridPolicy(refine_above=0.02, coarsen_below=1e-6, patience=1, buffer=0)
     for _ in range(2):
-        U = H.regrid(U, **kw)
-    counts = [H.topo.n_active(L) for L in range(3)]
+        U = H.regrid(U, pol, fields=[CHI])
+    counts = [H.tree.count(L) for L in range(3)]
     p = PhysicsParams()
     dt = 0.25 * H.spacing(2)
     t = [0.0]
@@ -330,7 +393,7 @@ def bench_amr(reps: int) -> dict:
     V = H2.fill(H2.empty_pool(), fn)
     torch.cuda.synchronize()
     w0 = time.perf_counter()
-    V = H2.regrid(V, **kw)
+    V = H2.regrid(V, pol, fields=[CHI])
     H2.batch, H2.program
     torch.cuda.synchronize()
     ms_regrid = (time.perf_counter() - w0) * 1e3
@@ -404,6 +467,59 @@ def bench_bfp(N: int, reps: int) -> dict:
     return out
 
 
+def bench_fd6(sizes, reps: int) -> dict:
+    """FD order 6 with KO order 8 (4 ghosts), the reference's production pairing and the
+    configuration of its published fused-kernel figure (gen_fused.py:63-66 at 128^3): the same
+    equal-mass binary construction at 128^3 and 256^3 (h = 16/N), RK4 step and bare RHS."""
+    import torch
+    from bssn3d_hip import initial_data as idata
+    from bssn3d_hip.evolve import BSSNEvolution
+    from bssn3d_hip.rhs import rhs_into
+    from bssn3d_hip.state import PhysicsParams, scalars
+
+    dev = torch.device("cuda:0")
+    p = PhysicsParams()
+    out = {}
+    for N in sizes:
+        grid, U, _ = idata.config_binary(seed=0, N=N, device=dev, ng=4)
+        dt = 0.25 * grid.dx
+        evo = BSSNEvolution(grid, p, order=6, ko_sigma=0.1, bc="periodic", integrator="rk4",
+                            ko_order=8)
+        t = 0.0
+        for _ in range(2):
+            evo.step_(U, t, dt)
+            t += dt
+        a, b = _events()
+        torch.cuda.synchronize()
+        a.record()
+        for _ in range(reps):
+            evo.step_(U, t, dt)
+            t += dt
+        b.record()
+        torch.cuda.synchronize()
+        ms_step = a.elapsed_time(b) / reps
+        finite = bool(torch.isfinite(U).all())
+        S16 = scalars(p, grid.dx, grid.dy, grid.dz, dt, t)
+        o = torch.empty_like(U)
+        rhs_into(U, o, S16, 6)
+        a.record()
+        for _ in range(reps):
+            rhs_into(U, o, S16, 6)
+        b.record()
+        torch.cuda.synchronize()
+        ms_rhs = a.elapsed_time(b) / reps
+        n3 = N ** 3
+        out[f"{N}^3"] = {"step_ms": round(ms_step, 4),
+                         "step_mpts_s": round(n3 / (ms_step * 1e-3) / 1e6, 2),
+                         "rhs_ms": round(ms_rhs, 4),
+                         "rhs_mpts_s": round(grid.shape[0] ** 3 / (ms_rhs * 1e-3) / 1e6, 2),
+                         "finite": finite}
+        del U, o, evo
+        torch.cuda.empty_cache()
+    out["workload"] = "bl_binary_fd6_ko8_ng4_h16/N"
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -417,6 +533,7 @@ def main() -> int:
     ap.add_argument("--no-longrun", action="store_true")
     ap.add_argument("--no-amr", action="store_true")
     ap.add_argument("--no-bfp", action="store_true")
+    ap.add_argument("--no-fd6", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="also run one plain device copy of this many GiB (profiler calibration)")
@@ -430,6 +547,7 @@ def main() -> int:
     import torch
     from bssn3d_hip import initial_data as idata
     from bssn3d_hip import _lib
+    from bssn3d_hip.constraints import ConstraintSolver
     from bssn3d_hip.evolve import BSSNEvolution
     from bssn3d_hip.rhs import rhs_into, rhs_twopass_into, stream_handle
     from bssn3d_hip.state import PhysicsParams, scalars
@@ -448,6 +566,12 @@ def main() -> int:
     def ev():
         return torch.cuda.Event(enable_timing=True)
 
+    # Hamiltonian / momentum constraint norms of the workload before and after the timed
+    # steps (interior RMS, FD order 4 like the evolution; ConstraintSolver.l2 is the
+    # reference's constraints.py:53-62)
+    cons = ConstraintSolver(grid, order=4)
+    h0, m0 = cons.l2(U)
+
     # ---------------------------------------------------------------- RK4 step (the metric)
     t = 0.0
     for _ in range(args.warmup):
@@ -465,6 +589,8 @@ def main() -> int:
     wall = time.perf_counter() - w0
     ms_step = e0.elapsed_time(e1) / args.steps
     assert bool(torch.isfinite(U).all()), "non-finite state after the timed steps"
+    h1, m1 = cons.l2(U)
+    nsteps_done = args.warmup + args.steps
 
     # periodic refill alone, to separate the stage kernels inside the step
     reps = max(args.steps, 5)
@@ -589,6 +715,10 @@ def main() -> int:
         "sommerfeld_step_mpts_s": round(npts / (ms_sommerfeld * 1e-3) / 1e6, 2),
         "sommerfeld_finite": sommerfeld_finite,
         "roofline": roof,
+        "constraints": {"order": 4, "H_l2_initial": float("%.6e" % h0),
+                        "M_l2_initial": float("%.6e" % m0),
+                        "H_l2_final": float("%.6e" % h1), "M_l2_final": float("%.6e" % m1),
+                        "steps": nsteps_done, "t_final": round(t, 6)},
     }
     if ms_twopass is not None:
         res["twopass_rhs_ms"] = round(ms_twopass, 4)
@@ -596,6 +726,9 @@ def main() -> int:
         res["fused_vs_twopass_rhs"] = round(ms_twopass / ms_rhs, 3)
     del out, U
     torch.cuda.empty_cache()
+    if not args.no_fd6:
+        res["fd6_ko8"] = bench_fd6((128, N), reps)
+        torch.cuda.empty_cache()
     if not args.no_int8:
         res["int8_split"] = bench_int8(N, reps)
         torch.cuda.empty_cache()

@@ -262,14 +262,39 @@ long long bssn_batch_points(const bssn_batch* b, int interior);
 int bssn_batch_rhs_f64(const bssn_batch* b, const double* F, const double* S16, double* OUT,
                        int fd_order, double ko_sigma, void* stream);
 
+/* Doubles of a batch's derivative bundle: block b's 138 arrays (the single-grid bundle layout
+ * of bssn_deriv_bundle_f64 over its padded points) follow block b-1's, each block starting on a
+ * 64-double boundary, in the order the blocks were given to bssn_batch_create; -1 for a null
+ * handle. */
+long long bssn_batch_bundle_doubles(const bssn_batch* b);
+
+/* The INT8-split bundle (bssn_deriv_bundle_i8_f64) of every block of a batch in five launches
+ * (one per plane orientation and field group; wave tiles of all blocks in one grid). Each
+ * block's arrays equal bssn_deriv_bundle_i8_f64 on that block alone bit for bit. F: the state
+ * pool; D: bssn_batch_bundle_doubles(b) doubles. The precision form of the derivatives under
+ * the batched-block schedule: the reference's INT8 contraction (2D/src/mcs2d/schemes/
+ * ozaki.py:311-377) over its slot-batched AMR step (2D/src/mcs2d/amr/evolve.py:659-775). */
+int bssn_batch_deriv_bundle_i8_f64(const bssn_batch* b, const double* F, double* D, int fd_order,
+                                   int slices, void* stream);
+
 /* Workspace of a batched step: integrator 0 = RK4 (3 pools). */
 size_t bssn_batch_step_workspace_bytes(const bssn_batch* b, int integrator);
 
 /* Each block periodic on its own interior: ghost refill of every block. */
 int bssn_batch_periodic_sync_f64(const bssn_batch* b, double* U, void* stream);
 
-/* Apply n halo copies (device array) to a pool. */
-int bssn_batch_halo_f64(double* pool, const bssn_halo_copy* map, long long n, void* stream);
+/* Apply n halo copies (device array) to a pool of pool_doubles doubles. An entry that would
+ * read or write outside the pool is skipped (bssn_count_skipped_entries finds such entries on the
+ * host). */
+int bssn_batch_halo_f64(double* pool, long long pool_doubles, const bssn_halo_copy* map,
+                        long long n, void* stream);
+
+/* How many of n entries (a host array; kind 0 = bssn_halo_copy, 1 = bssn_prolong_point,
+ * 2 = bssn_prolong_block) the device kernels would skip on a pool of pool_doubles doubles because
+ * they reach outside it (for a block job also: padded extent above 24 or an octant index other
+ * than 0 or 1); -1 for invalid arguments. The kernels run the same test per entry. */
+long long bssn_count_skipped_entries(int kind, const void* entries, long long n,
+                                     long long pool_doubles);
 
 /* One RK4 step of every block of the pool U in place (the reference's per-stage pattern:
  * RHS of all blocks, interior update, ghost sync; 2D/src/mcs2d/amr/evolve.py:745-771).
@@ -308,17 +333,19 @@ int bssn_algebra_bfp_f64(const double* F, const void* Q, int mant_bits, const do
 
 /* Prolongations of n points (device array) into a pool: the reference's prolongate / the halo
  * fill of sync_across_levels (3D/src/multipatch/amr/kernels.py:66-108, sync.py:84-97). */
-int bssn_prolong_f64(double* pool, const bssn_prolong_point* pts, long long n, void* stream);
+int bssn_prolong_f64(double* pool, long long pool_doubles, const bssn_prolong_point* pts,
+                     long long n, void* stream);
 
 /* n whole-block prolongation jobs (device array). */
-int bssn_prolong_blocks_f64(double* pool, const bssn_prolong_block* jobs, long long n,
-                            void* stream);
+int bssn_prolong_blocks_f64(double* pool, long long pool_doubles, const bssn_prolong_block* jobs,
+                            long long n, void* stream);
 
 /* Run a ghost fill program on a pool: the hierarchy sync of the reference's N-level step,
  * root level periodic, then each finer level from its parent (prolongation) and from
  * same-level face neighbours (2D/src/mcs2d/amr/evolve.py:719-735,
  * 3D/src/multipatch/amr/sync.py:84-122). */
-int bssn_ghost_fill_f64(double* pool, const bssn_ghost_program* prog, void* stream);
+int bssn_ghost_fill_f64(double* pool, long long pool_doubles, const bssn_ghost_program* prog,
+                        void* stream);
 
 /* bssn_batch_rk4_step_f64 with bc = 2 whose ghost refill after every stage and at the end is a
  * ghost fill program: the reference's shared-dt N-level RK4 step over a block hierarchy

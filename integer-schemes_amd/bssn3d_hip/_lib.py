@@ -73,15 +73,19 @@ SIGNATURES = {
     "bssn_batch_points": (_c_ll, [_c_void_p, _c_int]),
     "bssn_batch_rhs_f64": (_c_int, [_c_void_p, _c_void_p, _PD, _c_void_p, _c_int, _c_double,
                                     _c_void_p]),
+    "bssn_batch_bundle_doubles": (_c_ll, [_c_void_p]),
+    "bssn_batch_deriv_bundle_i8_f64": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_int, _c_int,
+                                                _c_void_p]),
     "bssn_batch_step_workspace_bytes": (_c_size_t, [_c_void_p, _c_int]),
     "bssn_batch_periodic_sync_f64": (_c_int, [_c_void_p, _c_void_p, _c_void_p]),
-    "bssn_batch_halo_f64": (_c_int, [_c_void_p, _c_void_p, _c_ll, _c_void_p]),
+    "bssn_batch_halo_f64": (_c_int, [_c_void_p, _c_ll, _c_void_p, _c_ll, _c_void_p]),
+    "bssn_count_skipped_entries": (_c_ll, [_c_int, _c_void_p, _c_ll, _c_ll]),
     "bssn_batch_rk4_step_f64": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _PD, _c_int, _c_double,
                                          _c_int, _c_void_p, _c_ll, _c_double, _c_double,
                                          _c_double, _c_double, _c_void_p, _c_void_p]),
-    "bssn_prolong_f64": (_c_int, [_c_void_p, _c_void_p, _c_ll, _c_void_p]),
-    "bssn_ghost_fill_f64": (_c_int, [_c_void_p, _c_void_p, _c_void_p]),
-    "bssn_prolong_blocks_f64": (_c_int, [_c_void_p, _c_void_p, _c_ll, _c_void_p]),
+    "bssn_prolong_f64": (_c_int, [_c_void_p, _c_ll, _c_void_p, _c_ll, _c_void_p]),
+    "bssn_ghost_fill_f64": (_c_int, [_c_void_p, _c_ll, _c_void_p, _c_void_p]),
+    "bssn_prolong_blocks_f64": (_c_int, [_c_void_p, _c_ll, _c_void_p, _c_ll, _c_void_p]),
     "bssn_batch_rk4_step_prog_f64": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _PD, _c_int,
                                               _c_double, _c_void_p, _c_double, _c_double,
                                               _c_double, _c_double, _c_void_p, _c_void_p]),

@@ -1,36 +1,37 @@
-"""Block-structured AMR for the BSSN state on the device: a live, node-centred hierarchy.
+"""Block-structured AMR for the BSSN state on the device: a live octree of equal blocks.
 
-Reference patterns (the reference has no BSSN AMR; its block AMR serves the MCS and wave
-systems):
-  * node-centred 2:1 transfers and the indicator: 3D/src/multipatch/amr/kernels.py:53-158
-    (prolongation = tensor product of fine[2k] = coarse[k], fine[2k+1] = 6-point Lagrange
-    midpoint; restriction = injection; indicator = largest undivided difference);
-  * topology bookkeeping: 3D/src/multipatch/amr/state.py:109-275 (bbox in level node indices,
-    parent/children, 6-face neighbours by bbox adjacency with no wrap on fine levels, proper
-    nesting);
-  * ghost sync of the shared-dt N-level step: 2D/src/mcs2d/amr/evolve.py:719-735 and
-    3D/src/multipatch/amr/sync.py:84-122 (root level periodic; each finer level, coarse to
-    fine, from a prolongation of its parent, then same-level face slabs over the interior
-    extent from neighbours);
-  * regrid: 2D/src/mcs2d/amr/regrid.py:113-486 (hysteresis flags, no-orphan nesting and
-    Chebyshev buffer dilation, coarsen finest-first by restriction, refine coarse-first by
-    prolongation, slot capacities that grow on demand), in 3D (8 octants).
+The hierarchy is an octree forest over a periodic root tiling. A block is named by its tile
+``(level, i, j, k)``: block (i, j, k) of its level's lattice, whose first interior node sits at
+level node index BS (i, j, k). Everything else is arithmetic on the tile -- the parent is
+(level - 1, i >> 1, j >> 1, k >> 1), the octant inside it (i & 1, j & 1, k & 1), the face
+neighbours are the tiles one lattice step away -- so the host keeps only which tiles exist,
+the pool slot each one occupies, and a hysteresis counter per tile.
 
-Device side: every block lives in a fixed slot of one float64 pool (level L's slots follow
-level L-1's); the active blocks form a BlockBatch (its plan lists only active blocks, so idle
-slots cost nothing); the ghost sync is a ghost fill program (bssn_ghost_fill_f64: one phase per
-level, prolongations and copies) run after every RK stage by bssn_batch_rk4_step_prog_f64.
-Prolongations run as whole-block jobs (bssn_prolong_blocks_f64: separable x, y, z passes
-through LDS, padded blocks up to 24 per axis) or point lists (bssn_prolong_f64); restrictions
-as copy lists (bssn_batch_halo_f64); the indicator is bssn_batch_indicator_f64.
+What a hierarchy holds and how its ghosts are filled follows the reference's node-centred
+block AMR (it has none for BSSN; its block AMR serves the MCS and wave systems):
+  * 2:1 transfers: fine[2k] = coarse[k], fine[2k+1] = 6-point Lagrange midpoint, tensor
+    product x, y, z; restriction = injection (3D/src/multipatch/amr/kernels.py:53-140);
+  * the hierarchy sync of its shared-dt step: root level periodic, then each finer level
+    coarse to fine from its parent's prolongation, with same-level face slabs over the
+    interior extent taken from face neighbours (2D/src/mcs2d/amr/evolve.py:719-735,
+    3D/src/multipatch/amr/sync.py:84-122);
+  * regrid: an indicator per block with a hysteresis of K consecutive votes, no coarsening of
+    a block that still has children, refinement spread to nearby blocks of the same level,
+    coarsening finest level first and refinement coarsest first
+    (2D/src/mcs2d/amr/regrid.py:113-538), in 3D with 8 octants.
 
-One deliberate difference: a slot's hysteresis streak restarts at 0 when a block is created in
-it (the reference keeps the streak of the block that last held the slot).
+Device side: every block lives in a fixed slot of one float64 pool (level L's slots follow level
+L-1's); the existing tiles form a BlockBatch (its plan lists only them); the ghost sync is a
+ghost fill program (bssn_ghost_fill_f64: one phase per level) run after every RK stage by
+bssn_batch_rk4_step_prog_f64. Prolongations run as whole-block jobs (bssn_prolong_blocks_f64,
+padded blocks up to 24 per axis) or point lists (bssn_prolong_f64); restrictions as copy lists
+(bssn_batch_halo_f64); the indicator is bssn_batch_indicator_f64.
 """
 from __future__ import annotations
 
 import ctypes
-from dataclasses import dataclass, field
+import itertools
+from dataclasses import dataclass
 from typing import Callable, Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
@@ -43,8 +44,7 @@ from .rhs import stream_handle
 from .state import NUM_VARS, PhysicsParams
 
 REFINE, KEEP, COARSEN = 1, 0, -1
-RATIO = 2
-NFACE = 6
+OCTANTS = tuple(itertools.product((0, 1), repeat=3))
 
 PROLONG_DTYPE = np.dtype([("dst", "<i8"), ("parent", "<i8"), ("dst_stride", "<i8"),
                           ("fine", "<i4", (3,)), ("pS", "<i4", (3,))])
@@ -56,190 +56,245 @@ assert BLOCK_DTYPE.itemsize == 40
 BLOCK_WMAX = 24
 SKIP_INTERIOR_WRITE = 64
 
+Tile = Tuple[int, int, int, int]
 
-# --- topology (host) --------------------------------------------------------------------------
+
+# --- tiles -------------------------------------------------------------------------------------
+
+def parent_tile(t: Tile) -> Tile:
+    return (t[0] - 1, t[1] >> 1, t[2] >> 1, t[3] >> 1)
+
+
+def octant(t: Tile) -> Tuple[int, int, int]:
+    """Which of its parent's eight octants tile t covers."""
+    return (t[1] & 1, t[2] & 1, t[3] & 1)
+
+
+def child_tiles(t: Tile) -> List[Tile]:
+    L, i, j, k = t
+    return [(L + 1, 2 * i + a, 2 * j + b, 2 * k + c) for a, b, c in OCTANTS]
+
+
+def face_step(t: Tile, face: int) -> Tile:
+    """The tile across face ``face`` (0..5 = -x, +x, -y, +y, -z, +z), present or not."""
+    axis, sign = face >> 1, 1 if face & 1 else -1
+    c = list(t)
+    c[1 + axis] += sign
+    return tuple(c)
+
+
+class SlotArena:
+    """The pool slots of one level: a tile takes the lowest free slot."""
+
+    def __init__(self, capacity: int):
+        self.owner: List[Optional[Tile]] = [None] * int(capacity)
+
+    @property
+    def capacity(self) -> int:
+        return len(self.owner)
+
+    @property
+    def used(self) -> int:
+        return sum(o is not None for o in self.owner)
+
+    def full(self) -> bool:
+        return all(o is not None for o in self.owner)
+
+    def take(self, tile: Tile) -> int:
+        for s, o in enumerate(self.owner):
+            if o is None:
+                self.owner[s] = tile
+                return s
+        raise RuntimeError(f"no free slot on level {tile[0]} (capacity {self.capacity})")
+
+    def release(self, slot: int) -> None:
+        self.owner[slot] = None
+
+    def widen(self, capacity: int) -> None:
+        if capacity < self.capacity:
+            raise ValueError("a level's capacity never shrinks")
+        self.owner.extend([None] * (capacity - self.capacity))
+
+
+class Octree:
+    """Which tiles exist, their slots, and their regrid vote counters.
+
+    roots: lattice extent of level 0 (every root tile exists); depth: number of levels;
+    capacities: slots per level; ceiling: the largest capacity a level may grow to."""
+
+    def __init__(self, roots: Sequence[int], depth: int, capacities: Sequence[int],
+                 ceiling: int):
+        self.roots = tuple(int(v) for v in roots)
+        self.depth = int(depth)
+        self.ceiling = int(ceiling)
+        if len(capacities) != self.depth:
+            raise ValueError("one capacity per level")
+        if capacities[0] < int(np.prod(self.roots)) or max(capacities) > self.ceiling:
+            raise ValueError("capacities: level 0 holds every root tile; none above the ceiling")
+        self.arenas = [SlotArena(c) for c in capacities]
+        self.slot_of: Dict[Tile, int] = {}
+        self.votes: Dict[Tile, int] = {}
+        self.high_water = [0] * self.depth
+        self.widenings: List[Tuple[int, int, int]] = []     # (level, old, new capacity)
+        for i, j, k in itertools.product(*[range(n) for n in self.roots]):
+            self.plant((0, i, j, k))
+        self.note_occupancy()
+
+    # membership ----------------------------------------------------------------------------
+    def __contains__(self, t) -> bool:
+        return t in self.slot_of
+
+    def plant(self, t: Tile) -> int:
+        """Create tile t (its parent must exist unless it is a root); returns its slot."""
+        t = tuple(int(v) for v in t)
+        if t in self.slot_of:
+            raise ValueError(f"tile {t} exists")
+        if not 0 <= t[0] < self.depth:
+            raise ValueError(f"tile {t}: level outside 0..{self.depth - 1}")
+        if t[0] == 0 and not all(0 <= t[1 + a] < self.roots[a] for a in range(3)):
+            raise ValueError(f"root tile {t} outside the root lattice {self.roots}")
+        s = self.arenas[t[0]].take(t)
+        self.slot_of[t] = s
+        self.votes[t] = 0
+        return s
+
+    def prune(self, t: Tile) -> None:
+        self.arenas[t[0]].release(self.slot_of.pop(t))
+        self.votes.pop(t, None)
+
+    def capacity(self, level: int) -> int:
+        return self.arenas[level].capacity
+
+    def widen(self, level: int) -> Tuple[int, int]:
+        """Double a level's slot capacity (up to the ceiling); returns (old, new)."""
+        old = self.capacity(level)
+        new = min(2 * old, self.ceiling)
+        if new <= old:
+            raise RuntimeError(f"level {level} is at its slot ceiling ({self.ceiling})")
+        self.arenas[level].widen(new)
+        self.widenings.append((level, old, new))
+        return old, new
+
+    # queries -------------------------------------------------------------------------------
+    def on_level(self, level: int) -> List[Tile]:
+        """The level's tiles in slot order."""
+        return [o for o in self.arenas[level].owner if o is not None]
+
+    def all_tiles(self) -> List[Tile]:
+        """Every tile, level by level in slot order (the batch order)."""
+        return [t for L in range(self.depth) for t in self.on_level(L)]
+
+    def count(self, level: int) -> int:
+        return self.arenas[level].used
+
+    def kids(self, t: Tile) -> List[Tile]:
+        return [c for c in child_tiles(t) if c in self.slot_of]
+
+    def neighbour(self, t: Tile, face: int) -> Optional[Tile]:
+        """The same-level tile across a face if it exists (no wrap: the root level's periodic
+        images are the ghost program's business)."""
+        n = face_step(t, face)
+        return n if n in self.slot_of else None
+
+    def nesting_errors(self) -> List[Tuple[Tile, str]]:
+        """Tiles of level >= 1 whose parent tile does not exist (a 2:1 child always lies in its
+        parent's footprint by construction)."""
+        return [(t, "parent missing") for t in self.slot_of
+                if t[0] > 0 and parent_tile(t) not in self.slot_of]
+
+    def note_occupancy(self) -> None:
+        for L in range(self.depth):
+            self.high_water[L] = max(self.high_water[L], self.count(L))
+
+
+# --- regrid decisions (host) -------------------------------------------------------------------
 
 @dataclass
-class AMRTopology:
-    """Host bookkeeping of a hierarchy (3D/src/multipatch/amr/state.py:109-275).
-    bbox_ijk[(L, s)] = level-L node index of block (L, s)'s first interior node."""
-    levels: int
-    BS: int
-    caps: List[int]
-    max_blocks: int
-    active: np.ndarray = None
-    streaks: np.ndarray = None
-    parent: Dict = field(default_factory=dict)
-    children: Dict = field(default_factory=dict)
-    bbox_ijk: Dict = field(default_factory=dict)
-    neighbors: Dict = field(default_factory=dict)
-    peak_active: List[int] = None
-    grow_history: List = field(default_factory=list)
+class RegridPolicy:
+    """When blocks split and merge.
+
+    A tile's counter moves one step toward +patience while its indicator exceeds
+    ``refine_above``, one step toward -patience while it is below ``coarsen_below``, and one
+    step back toward 0 otherwise; the tile refines when the counter sits at +patience and
+    coarsens at -patience. Root tiles never coarsen, tiles on ``max_level`` (default: the
+    deepest level) never refine, a tile with children never coarsens (so a hierarchy thins one
+    level per pass, finest first), and refinement spreads to every tile of the same level
+    within ``buffer`` lattice steps in each axis."""
+    refine_above: float
+    coarsen_below: float
+    patience: int = 3
+    buffer: int = 1
+    max_level: Optional[int] = None
 
     def __post_init__(self):
-        if self.active is None:
-            self.active = np.zeros((self.levels, self.max_blocks), bool)
-        if self.streaks is None:
-            self.streaks = np.zeros((self.levels, self.max_blocks), np.int32)
-        if self.peak_active is None:
-            self.peak_active = [0] * self.levels
+        if not self.refine_above > self.coarsen_below:
+            raise ValueError("refine_above must exceed coarsen_below")
+        if self.patience < 1:
+            raise ValueError("patience must be at least 1")
 
-    def n_active(self, level: int) -> int:
-        return int(self.active[level].sum())
+    def vote(self, tree: Octree, score: Dict[Tile, float]) -> Dict[Tile, int]:
+        """Advance every tile's counter by its score (missing = 0) and return the raw action
+        of each tile (before the nesting and buffer rules)."""
+        P = self.patience
+        top = tree.depth - 1 if self.max_level is None else min(self.max_level, tree.depth - 1)
+        out = {}
+        for t in tree.all_tiles():
+            v = float(score.get(t, 0.0))
+            c = tree.votes[t]
+            if v > self.refine_above:
+                c = min(c + 1, P)
+            elif v < self.coarsen_below:
+                c = max(c - 1, -P)
+            else:
+                c -= (c > 0) - (c < 0)
+            tree.votes[t] = c
+            act = REFINE if c == P else (COARSEN if c == -P else KEEP)
+            if (act == COARSEN and t[0] == 0) or (act == REFINE and t[0] >= top):
+                act = KEEP
+            out[t] = act
+        return out
 
-    def record_occupancy(self):
-        for L in range(self.levels):
-            self.peak_active[L] = max(self.peak_active[L], self.n_active(L))
+    def settle(self, tree: Octree, acts: Dict[Tile, int]) -> Dict[Tile, int]:
+        """Apply the nesting and buffer rules to raw actions."""
+        acts = dict(acts)
+        for t, a in acts.items():
+            if a == COARSEN and tree.kids(t):
+                acts[t] = KEEP
+        if self.buffer > 0:
+            grow = [t for t, a in acts.items() if a == REFINE]
+            span = range(-self.buffer, self.buffer + 1)
+            for L, i, j, k in grow:
+                for di, dj, dk in itertools.product(span, span, span):
+                    n = (L, i + di, j + dj, k + dk)
+                    if n in acts:
+                        acts[n] = REFINE
+        return acts
 
-    def recommended_caps(self, margin: float = 1.5) -> tuple:
-        rec = []
-        for L in range(self.levels):
-            want = self.peak_active[L] if L == 0 else int(np.ceil(self.peak_active[L] * margin))
-            rec.append(min(max(want, 1), self.max_blocks))
-        return tuple(rec)
-
-    def free_slots(self, level: int) -> np.ndarray:
-        return np.flatnonzero(~self.active[level, :self.caps[level]])
-
-    def add_block(self, level: int, slot: int, bbox: tuple, parent: Optional[tuple] = None):
-        self.active[level, slot] = True
-        self.streaks[level, slot] = 0
-        self.bbox_ijk[(level, slot)] = tuple(int(v) for v in bbox)
-        if parent is not None:
-            self.parent[(level, slot)] = parent
-            self.children.setdefault(parent, []).append((level, slot))
-
-    def remove_block(self, level: int, slot: int):
-        key = (level, slot)
-        self.active[level, slot] = False
-        par = self.parent.pop(key, None)
-        if par is not None and par in self.children:
-            self.children[par] = [c for c in self.children[par] if c != key]
-        self.children.pop(key, None)
-        self.bbox_ijk.pop(key, None)
-        self.neighbors = {k: v for k, v in self.neighbors.items() if k[:2] != key and v != key}
-
-    def blocks(self, level: int | None = None) -> List[Tuple[int, int]]:
-        levels = range(self.levels) if level is None else [level]
-        return [(L, int(s)) for L in levels for s in np.flatnonzero(self.active[L])]
-
-    def child_corner(self, key) -> tuple:
-        """Octant (cx, cy, cz) of block ``key`` inside its parent."""
-        L, _ = key
-        cb = self.bbox_ijk[key]
-        pb = self.bbox_ijk[self.parent[key]]
-        half = self.BS // RATIO
-        return tuple((cb[a] // RATIO - pb[a]) // half for a in range(3))
-
-    def rebuild_neighbors(self):
-        """Same-level 6-face neighbours from bbox adjacency (no wrap): faces 0..5 =
-        (-x, +x, -y, +y, -z, +z) (state.py:167-190)."""
-        self.neighbors = {}
-        BS = self.BS
-        for L in range(self.levels):
-            pos = {self.bbox_ijk[(L, s)]: s for (_, s) in self.blocks(L)}
-            for (i0, j0, k0), s in pos.items():
-                cand = ((i0 - BS, j0, k0), (i0 + BS, j0, k0), (i0, j0 - BS, k0),
-                        (i0, j0 + BS, k0), (i0, j0, k0 - BS), (i0, j0, k0 + BS))
-                for f, p in enumerate(cand):
-                    nb = pos.get(p)
-                    self.neighbors[(L, s, f)] = (L, nb) if nb is not None else None
-
-    def check_proper_nesting(self) -> list:
-        """Every active L >= 1 block has an active parent whose interior contains its footprint
-        (state.py:192-216)."""
-        bad = []
-        half = self.BS // RATIO
-        for L in range(1, self.levels):
-            for (_, s) in self.blocks(L):
-                par = self.parent.get((L, s))
-                if par is None:
-                    bad.append((L, s, "no parent"))
-                    continue
-                if not self.active[par]:
-                    bad.append((L, s, "parent inactive"))
-                    continue
-                cb, pb = self.bbox_ijk[(L, s)], self.bbox_ijk[par]
-                f0 = [cb[a] // RATIO for a in range(3)]
-                if not all(pb[a] <= f0[a] and f0[a] + half <= pb[a] + self.BS for a in range(3)):
-                    bad.append((L, s, "footprint outside parent"))
-        return bad
+    def decide(self, tree: Octree, score: Dict[Tile, float]) -> Dict[Tile, int]:
+        return self.settle(tree, self.vote(tree, score))
 
 
-# --- regrid decisions (host) ------------------------------------------------------------------
+# --- transfer entries (host) -------------------------------------------------------------------
 
-def compute_flags(indicators_per_level: Sequence[np.ndarray], topo: AMRTopology, *,
-                  refine_threshold: float, coarsen_threshold: float, hysteresis_K: int = 3,
-                  max_level: Optional[int] = None) -> np.ndarray:
-    """Indicator -> REFINE / KEEP / COARSEN per (level, slot) with a streak hysteresis
-    (regrid.py:113-190): streak +1 above refine_threshold (cap +K), -1 below coarsen_threshold
-    (floor -K), else decays by 1 toward 0; REFINE at +K, COARSEN at -K; level 0 never
-    coarsens; no block at or below the deepest refinable level refines. Updates topo.streaks."""
-    if not refine_threshold > coarsen_threshold:
-        raise ValueError("refine_threshold must be > coarsen_threshold")
-    deepest = topo.levels - 1 if max_level is None else min(max_level, topo.levels - 1)
-    flags = np.zeros((topo.levels, topo.max_blocks), dtype=np.int32)
-    K = hysteresis_K
-    for L in range(topo.levels):
-        ind = np.asarray(indicators_per_level[L], dtype=np.float64)
-        act = topo.active[L]
-        s = topo.streaks[L]
-        above = (ind > refine_threshold) & act
-        below = (ind < coarsen_threshold) & act
-        new = np.where(above, np.minimum(s + 1, K),
-                       np.where(below, np.maximum(s - 1, -K),
-                                np.where(s > 0, s - 1, np.where(s < 0, s + 1, 0))))
-        topo.streaks[L] = new
-        fl = np.where(new >= K, REFINE, np.where(new <= -K, COARSEN, KEEP))
-        if L == 0:
-            fl = np.where(fl == COARSEN, KEEP, fl)
-        if L >= deepest:
-            fl = np.where(fl == REFINE, KEEP, fl)
-        flags[L] = fl
-    return flags
-
-
-def enforce_nesting_buffer(flags: np.ndarray, topo: AMRTopology, *, n_buffer: int = 1) -> np.ndarray:
-    """No coarsening of a block with active children; REFINE spreads to same-level active
-    blocks within n_buffer blocks in Chebyshev distance (regrid.py:195-250, in 3D)."""
-    flags = flags.copy()
-    for key, kids in list(topo.children.items()):
-        if topo.active[key] and any(topo.active[k] for k in kids) and flags[key] == COARSEN:
-            flags[key] = KEEP
-    reach = n_buffer * topo.BS
-    for L in range(topo.levels):
-        ref = [topo.bbox_ijk[(L, s)] for (_, s) in topo.blocks(L) if flags[L, s] == REFINE]
-        if not ref:
-            continue
-        R = np.asarray(ref)
-        for (_, s) in topo.blocks(L):
-            if flags[L, s] == REFINE:
-                continue
-            b = np.asarray(topo.bbox_ijk[(L, s)])
-            if np.any(np.all(np.abs(R - b) <= reach, axis=1)):
-                flags[L, s] = REFINE
-    return flags
-
-
-# --- transfer entries (host) ------------------------------------------------------------------
-
-def _grid_points(W: int) -> np.ndarray:
+def _cube(W: int) -> np.ndarray:
+    """All points of a W^3 padded block, x slowest."""
     return np.stack(np.meshgrid(np.arange(W), np.arange(W), np.arange(W), indexing="ij"),
                     -1).reshape(-1, 3)
 
 
-def _lin(c: np.ndarray, W: int) -> np.ndarray:
+def _flat(c: np.ndarray, W: int) -> np.ndarray:
     return (c[:, 0] * W + c[:, 1]) * W + c[:, 2]
 
 
 def prolong_entries(points: np.ndarray, dst_off: int, parent_off: int, corner, BS: int,
                     ng: int) -> np.ndarray:
-    """Prolongation entries for the given points (full-block indices) of a child block at
-    octant ``corner`` of its parent: the child point ff sits at index ng + corner BS + ff of
-    the 2:1 refinement of the parent's padded block (kernels.py:78-85)."""
+    """bssn_prolong_point entries for the given padded-block points of a child in octant
+    ``corner`` of its parent: child point ff is index ng + corner BS + ff of the 2:1 refinement
+    of the parent's padded block (the child's first interior node coincides with parent node
+    ng + corner BS / 2)."""
     W = BS + 2 * ng
     e = np.empty(points.shape[0], dtype=PROLONG_DTYPE)
-    e["dst"] = dst_off + _lin(points, W)
+    e["dst"] = dst_off + _flat(points, W)
     e["parent"] = parent_off
     e["dst_stride"] = W ** 3
     e["fine"] = ng + np.asarray(corner, dtype=np.int64)[None, :] * BS + points
@@ -248,44 +303,25 @@ def prolong_entries(points: np.ndarray, dst_off: int, parent_off: int, corner, B
 
 
 def by_parity(e: np.ndarray) -> np.ndarray:
-    """Prolongation entries ordered by the parity class of their fine indices (stable), so that
-    the device's lanes in one wave evaluate the same number of taps."""
+    """Prolongation entries ordered (stably) by the parity class of their fine indices, so that
+    the lanes of one wave evaluate the same number of taps."""
     f = e["fine"]
     cls = (f[:, 0] & 1) * 4 + (f[:, 1] & 1) * 2 + (f[:, 2] & 1)
     return e[np.argsort(cls, kind="stable")]
 
 
 def restrict_entries(parent_off: int, child_off: int, corner, BS: int, ng: int) -> np.ndarray:
-    """Injection of a child's coincident interior nodes into its parent's octant
-    (kernels.py:113-140): parent ng + corner BS/2 + a <- child ng + 2 a, a in [0, BS/2)^3."""
+    """Injection copies of a child's even interior nodes onto the parent nodes they coincide
+    with: parent ng + corner BS/2 + a <- child ng + 2 a, a in [0, BS/2)^3."""
     W = BS + 2 * ng
-    half = BS // RATIO
-    a = _grid_points(half)
-    p = ng + np.asarray(corner)[None, :] * half + a
-    c = ng + RATIO * a
+    half = BS // 2
+    a = _cube(half)
     e = np.empty(a.shape[0], dtype=HALO_DTYPE)
-    e["dst"] = parent_off + _lin(p, W)
-    e["src"] = child_off + _lin(c, W)
+    e["dst"] = parent_off + _flat(ng + np.asarray(corner)[None, :] * half + a, W)
+    e["src"] = child_off + _flat(ng + 2 * a, W)
     e["dst_stride"] = W ** 3
     e["src_stride"] = W ** 3
     return e
-
-
-def _face_slab(f: int, BS: int, ng: int):
-    """Destination and source full-block index ranges of face f's ghost slab (sync.py:100-108):
-    my ghost slab along the face axis, interior extent on the other two."""
-    ax, hi = f // 2, f % 2
-    rng = [(ng, ng + BS)] * 3
-    dst, src = list(rng), list(rng)
-    if hi == 0:
-        dst[ax], src[ax] = (0, ng), (BS, BS + ng)
-    else:
-        dst[ax], src[ax] = (ng + BS, 2 * ng + BS), (ng, 2 * ng)
-    return dst, src
-
-
-def _box(r):
-    return np.stack(np.meshgrid(*[np.arange(a, b) for a, b in r], indexing="ij"), -1).reshape(-1, 3)
 
 
 def block_job(dst_off: int, parent_off: int, corner, BS: int, ng: int, skip: int) -> np.ndarray:
@@ -295,120 +331,123 @@ def block_job(dst_off: int, parent_off: int, corner, BS: int, ng: int, skip: int
     return e
 
 
-def build_ghost_program(topo: AMRTopology, nb_root, ng: int, slot_offset: Callable,
+def _slab(face: int, BS: int, ng: int):
+    """(destination, source) boxes of a face copy: the ghost slab beyond ``face`` over the
+    interior extent of the other two axes, and the neighbour's interior layers it mirrors."""
+    axis, upper = face >> 1, face & 1
+    dst = [(ng, ng + BS)] * 3
+    src = [(ng, ng + BS)] * 3
+    dst[axis] = (ng + BS, BS + 2 * ng) if upper else (0, ng)
+    src[axis] = (ng, 2 * ng) if upper else (BS, BS + ng)
+    grid = lambda r: np.stack(np.meshgrid(*[np.arange(a, b) for a, b in r], indexing="ij"),
+                              -1).reshape(-1, 3)
+    return grid(dst), grid(src)
+
+
+def _copies(dst_off: int, dst_pts: np.ndarray, src_off, src_pts: np.ndarray, W: int):
+    e = np.empty(dst_pts.shape[0], dtype=HALO_DTYPE)
+    e["dst"] = dst_off + _flat(dst_pts, W)
+    e["src"] = src_off + _flat(src_pts, W)
+    e["dst_stride"] = W ** 3
+    e["src_stride"] = W ** 3
+    return e
+
+
+def build_ghost_program(tree: Octree, BS: int, ng: int, offset_of: Callable[[Tile], int],
                         use_blocks: bool | None = None):
-    """Entries of the hierarchy sync, phase by phase: phase 0 = periodic copies for every ghost
-    point of every root block; phase L >= 1 = for every active level-L block, face copies from
-    its same-level neighbours and prolongations from its parent for all its other ghost points
-    -- one whole-block job per block when the padded extent allows (use_blocks, default when
-    W <= BLOCK_WMAX), else one entry per point.
-    Returns (prolongs, blocks, copies, prolong_end, block_end, copy_end)."""
-    BS = topo.BS
+    """The hierarchy sync as ghost fill program entries, one phase per level.
+    Phase 0: every ghost point of every root tile is copied from the root tile that owns its
+    periodic image. Phase L >= 1, per tile: face slabs copied from face neighbours that exist;
+    every other ghost point prolongated from the parent -- one whole-block job per tile when
+    the padded extent allows (use_blocks, default when W <= BLOCK_WMAX), else point entries
+    grouped by parity.
+    Returns (prolongs, jobs, copies, prolong_end, job_end, copy_end) (ends cumulative)."""
     W = BS + 2 * ng
-    P = W ** 3
     if use_blocks is None:
         use_blocks = W <= BLOCK_WMAX
     if use_blocks and W > BLOCK_WMAX:
         raise ValueError(f"whole-block prolongation needs BS + 2 ng <= {BLOCK_WMAX}")
-    pts = _grid_points(W)
-    ghost = pts[np.any((pts < ng) | (pts >= ng + BS), axis=1)]
-    nb = np.asarray(nb_root, dtype=np.int64)
-    N = nb * BS
-    root_off = -np.ones(tuple(nb), dtype=np.int64)
-    for k in topo.blocks(0):
-        b = np.asarray(topo.bbox_ijk[k]) // BS
-        root_off[tuple(b)] = slot_offset(*k)
-    if np.any(root_off < 0) or len(topo.blocks(0)) != int(np.prod(nb)):
+    roots = np.asarray(tree.roots, dtype=np.int64)
+    if tree.count(0) != int(np.prod(roots)):
         raise ValueError("the root level must tile the box")
-    prolongs, jobs, copies, pe, be, ce = [], [], [], [], [], []
-    # phase 0: root periodic
-    for key in topo.blocks(0):
-        b0 = np.asarray(topo.bbox_ijk[key])
-        g = (b0[None, :] + ghost - ng) % N[None, :]
-        blk = g // BS
-        loc = g - blk * BS + ng
-        e = np.empty(ghost.shape[0], dtype=HALO_DTYPE)
-        e["dst"] = slot_offset(0, key[1]) + _lin(ghost, W)
-        e["src"] = root_off[blk[:, 0], blk[:, 1], blk[:, 2]] + _lin(loc, W)
-        e["dst_stride"] = P
-        e["src_stride"] = P
-        copies.append(e)
-    pe.append(sum(len(x) for x in prolongs))
-    be.append(sum(len(x) for x in jobs))
-    ce.append(sum(len(x) for x in copies))
-    topo.rebuild_neighbors()
-    for L in range(1, topo.levels):
-        level_pro = []
-        for key in topo.blocks(L):
-            s = key[1]
-            done = np.zeros((W, W, W), bool)
-            done[ng:ng + BS, ng:ng + BS, ng:ng + BS] = True
+    pts = _cube(W)
+    halo = pts[np.any((pts < ng) | (pts >= ng + BS), axis=1)]
+    root_offset = np.empty(tuple(roots), dtype=np.int64)
+    for t in tree.on_level(0):
+        root_offset[t[1], t[2], t[3]] = offset_of(t)
+    pro, job, cop = [], [], []
+    ends = ([], [], [])
+
+    def close_phase():
+        for lst, acc in zip((pro, job, cop), ends):
+            acc.append(sum(len(x) for x in lst))
+
+    span = roots * BS                       # root nodes per axis (periodic)
+    for t in tree.on_level(0):
+        node = np.asarray(t[1:]) * BS + halo - ng      # level-0 node index of each ghost
+        node %= span[None, :]
+        own = node // BS
+        cop.append(_copies(offset_of(t), halo, root_offset[own[:, 0], own[:, 1], own[:, 2]],
+                           node - own * BS + ng, W))
+    close_phase()
+    for L in range(1, tree.depth):
+        point_lists = []
+        for t in tree.on_level(L):
+            covered = np.zeros((W, W, W), bool)
+            covered[ng:ng + BS, ng:ng + BS, ng:ng + BS] = True
             skip = 0
-            for f in range(NFACE):
-                nb = topo.neighbors.get((L, s, f))
-                if nb is None:
+            for face in range(6):
+                n = tree.neighbour(t, face)
+                if n is None:
                     continue
-                skip |= 1 << f
-                dr, sr = _face_slab(f, BS, ng)
-                d, sp = _box(dr), _box(sr)
-                e = np.empty(d.shape[0], dtype=HALO_DTYPE)
-                e["dst"] = slot_offset(L, s) + _lin(d, W)
-                e["src"] = slot_offset(L, nb[1]) + _lin(sp, W)
-                e["dst_stride"] = P
-                e["src_stride"] = P
-                copies.append(e)
-                done[d[:, 0], d[:, 1], d[:, 2]] = True
-            par = topo.parent[key]
+                d, s = _slab(face, BS, ng)
+                cop.append(_copies(offset_of(t), d, offset_of(n), s, W))
+                covered[d[:, 0], d[:, 1], d[:, 2]] = True
+                skip |= 1 << face
+            par = parent_tile(t)
             if use_blocks:
-                jobs.append(block_job(slot_offset(L, s), slot_offset(*par),
-                                      topo.child_corner(key), BS, ng, skip))
+                job.append(block_job(offset_of(t), offset_of(par), octant(t), BS, ng, skip))
             else:
-                rest = pts[~done.reshape(-1)]
-                level_pro.append(prolong_entries(rest, slot_offset(L, s), slot_offset(*par),
-                                                 topo.child_corner(key), BS, ng))
-        if level_pro:
-            prolongs.append(by_parity(np.concatenate(level_pro)))
-        pe.append(sum(len(x) for x in prolongs))
-        be.append(sum(len(x) for x in jobs))
-        ce.append(sum(len(x) for x in copies))
+                rest = pts[~covered.reshape(-1)]
+                point_lists.append(prolong_entries(rest, offset_of(t), offset_of(par),
+                                                   octant(t), BS, ng))
+        if point_lists:
+            pro.append(by_parity(np.concatenate(point_lists)))
+        close_phase()
     cat = lambda xs, dt: np.concatenate(xs) if xs else np.empty(0, dtype=dt)
-    return (cat(prolongs, PROLONG_DTYPE), cat(jobs, BLOCK_DTYPE), cat(copies, HALO_DTYPE),
-            pe, be, ce)
+    return (cat(pro, PROLONG_DTYPE), cat(job, BLOCK_DTYPE), cat(cop, HALO_DTYPE), *ends)
 
 
-class _DeviceProgram:
-    """A ghost fill program uploaded to the device (keeps the arrays alive)."""
+class DeviceProgram:
+    """A ghost fill program in device memory (keeps the entry arrays alive)."""
 
     def __init__(self, prolongs: np.ndarray, jobs: np.ndarray, copies: np.ndarray, pe, be, ce,
                  device):
         self.n_prolong, self.n_block, self.n_copy = len(prolongs), len(jobs), len(copies)
-        up = lambda a: _upload(a, device) if len(a) else None
-        self._p, self._b, self._c = up(prolongs), up(jobs), up(copies)
-        self._pe = (ctypes.c_longlong * len(pe))(*pe)
-        self._be = (ctypes.c_longlong * len(be))(*be)
-        self._ce = (ctypes.c_longlong * len(ce))(*ce)
-        ptr = lambda t: t.data_ptr() if t is not None else None
-        self.struct = _lib.GhostProgram(len(pe), self._pe, self._be, self._ce, ptr(self._p),
-                                        ptr(self._b), ptr(self._c))
+        put = lambda a: upload(a, device) if len(a) else None
+        self._p, self._b, self._c = put(prolongs), put(jobs), put(copies)
+        self._ends = [(ctypes.c_longlong * len(x))(*x) for x in (pe, be, ce)]
+        addr = lambda t: t.data_ptr() if t is not None else None
+        self.struct = _lib.GhostProgram(len(pe), *self._ends, addr(self._p), addr(self._b),
+                                        addr(self._c))
 
     @property
     def ptr(self):
         return ctypes.byref(self.struct)
 
 
-def _upload(entries: np.ndarray, device) -> torch.Tensor:
+def upload(entries: np.ndarray, device) -> torch.Tensor:
     return torch.from_numpy(np.ascontiguousarray(entries).view(np.uint8).copy()).to(device)
 
 
-# --- the hierarchy ----------------------------------------------------------------------------
+# --- the hierarchy -----------------------------------------------------------------------------
 
 class AMRHierarchy:
-    """A node-centred block hierarchy on a periodic box from lo: a root tiling of nb_root blocks
-    of BS^3 nodes at one spacing h0 = L / (nb_x BS) (L = the box's x extent), refined 2:1 by
-    octants up to
-    ``levels`` levels; every block holds BS^3 interior nodes and ng ghost layers. The state is
-    one float64 pool (``empty_pool``); block (L, s) is ``view(U, L, s)``, a (24, W, W, W) state
-    (W = BS + 2 ng). Steps are shared-dt RK4 over all active blocks."""
+    """An octree hierarchy of BS^3-node blocks over a periodic box starting at ``lo``: a root
+    tiling of nb_root blocks at spacing h0 = L / (nb_root[0] BS) (L is the box's x extent),
+    refined 2:1 up to ``levels`` levels; every block has ng ghost layers. The state is one
+    float64 pool (``empty_pool``); tile t's block is ``view(U, t)``, a (24, W, W, W) state
+    (W = BS + 2 ng). Steps are shared-dt RK4 over all blocks."""
 
     def __init__(self, nb_root=(2, 2, 2), BS: int = 16, ng: int = 3, levels: int = 3,
                  caps: Sequence[int] | None = None, max_blocks: int = 512, L: float = 1.0,
@@ -426,105 +465,98 @@ class AMRHierarchy:
         nroot = int(np.prod(self.nb_root))
         if caps is None:
             caps = [nroot] + [min(max_blocks, 8 * nroot)] * (levels - 1)
-        caps = [int(c) for c in caps]
-        if caps[0] < nroot or max(caps) > max_blocks or len(caps) != levels:
-            raise ValueError("caps: one per level, caps[0] >= root blocks, each <= max_blocks")
-        self.h0 = self.L / (self.nb_root[0] * self.BS)       # L is the box's x extent
-        self.box = tuple(n * self.BS * self.h0 for n in self.nb_root)
-        self.topo = AMRTopology(levels, self.BS, caps, int(max_blocks))
-        nbx, nby, nbz = self.nb_root
-        for bi in range(nbx):
-            for bj in range(nby):
-                for bk in range(nbz):
-                    self.topo.add_block(0, (bi * nby + bj) * nbz + bk,
-                                        (bi * self.BS, bj * self.BS, bk * self.BS))
-        self.topo.record_occupancy()
-        self._layout()
+        self.tree = Octree(self.nb_root, levels, [int(c) for c in caps], int(max_blocks))
+        self.h0 = self.L / (self.nb_root[0] * self.BS)
+        self._place()
         self._batch = None
         self._prog = None
 
-    # --- pool layout ----------------------------------------------------------------------
-    def _layout(self):
+    # pool layout ---------------------------------------------------------------------------
+    def _place(self):
         self.slot_doubles = (NUM_VARS * self.W ** 3 + 63) // 64 * 64
-        self.level_base = [0]
-        for c in self.topo.caps[:-1]:
-            self.level_base.append(self.level_base[-1] + c * self.slot_doubles)
-        self.pool_doubles = self.level_base[-1] + self.topo.caps[-1] * self.slot_doubles
+        self.level_base, pos = [], 0
+        for L in range(self.levels):
+            self.level_base.append(pos)
+            pos += self.tree.capacity(L) * self.slot_doubles
+        self.pool_doubles = pos
 
-    def slot_offset(self, L: int, s: int) -> int:
-        if not 0 <= s < self.topo.caps[L]:
-            raise IndexError(f"slot {s} outside level {L}'s capacity {self.topo.caps[L]}")
-        return self.level_base[L] + s * self.slot_doubles
+    def slot_offset(self, t: Tile) -> int:
+        """Pool offset of tile t's block."""
+        return self.level_base[t[0]] + self.tree.slot_of[tuple(t)] * self.slot_doubles
 
     def empty_pool(self) -> torch.Tensor:
         return torch.zeros(self.pool_doubles, dtype=torch.float64, device=self.device)
 
-    def view(self, U, L: int, s: int):
-        """Block (L, s) of a pool (torch tensor or numpy array) as a (24, W, W, W) view."""
-        o = self.slot_offset(L, s)
+    def view(self, U, t: Tile):
+        """Tile t's block of a pool (torch tensor or numpy array) as a (24, W, W, W) view."""
+        o = self.slot_offset(t)
         blk = U[o:o + NUM_VARS * self.W ** 3]
         shape = (NUM_VARS, self.W, self.W, self.W)
         return blk.view(*shape) if isinstance(blk, torch.Tensor) else blk.reshape(shape)
 
-    def spacing(self, L: int) -> float:
-        return self.h0 / (RATIO ** L)
+    def spacing(self, level: int) -> float:
+        return self.h0 / 2 ** level
 
-    def block(self, key) -> Block:
-        L, _ = key
-        h = self.spacing(L)
-        b = self.topo.bbox_ijk[key]
+    def origin(self, t: Tile) -> Tuple[int, int, int]:
+        """Level node index of tile t's first interior node."""
+        return tuple(self.BS * v for v in t[1:])
+
+    def block(self, t: Tile) -> Block:
+        h = self.spacing(t[0])
+        o = self.origin(t)
         return Block(n=(self.BS,) * 3, h=(h,) * 3,
-                     lo=tuple(self.lo[a] + b[a] * h for a in range(3)), level=L)
+                     lo=tuple(self.lo[a] + o[a] * h for a in range(3)), level=t[0])
 
-    def grid(self, key) -> Grid:
-        return self.block(key).grid(self.ng)
+    def grid(self, t: Tile) -> Grid:
+        return self.block(t).grid(self.ng)
 
-    # --- plans ----------------------------------------------------------------------------
-    def _invalidate(self):
+    # plans ---------------------------------------------------------------------------------
+    def _drop_plans(self):
         if self._batch is not None:
             self._batch.close()
         self._batch = None
         self._prog = None
 
     @property
-    def keys(self) -> List[Tuple[int, int]]:
-        """Active blocks in batch order (level, then slot)."""
-        return self.topo.blocks()
+    def keys(self) -> List[Tile]:
+        """Every tile in batch order (level, then slot)."""
+        return self.tree.all_tiles()
 
     @property
     def batch(self) -> BlockBatch:
         if self._batch is None:
-            keys = self.keys
-            self._batch = BlockBatch([self.block(k) for k in keys], ng=self.ng,
+            tiles = self.keys
+            self._batch = BlockBatch([self.block(t) for t in tiles], ng=self.ng,
                                      device=self.device,
-                                     offsets=[self.slot_offset(*k) for k in keys],
+                                     offsets=[self.slot_offset(t) for t in tiles],
                                      pool_doubles=self.pool_doubles)
         return self._batch
 
-    def program_entries(self):
-        return build_ghost_program(self.topo, self.nb_root, self.ng, self.slot_offset)
+    def program_entries(self, use_blocks: bool | None = None):
+        return build_ghost_program(self.tree, self.BS, self.ng, self.slot_offset, use_blocks)
 
     @property
-    def program(self) -> _DeviceProgram:
+    def program(self) -> DeviceProgram:
         if self._prog is None:
             pr, jb, cp, pe, be, ce = self.program_entries()
-            self._check_entries(pr, cp, jb)
-            self._prog = _DeviceProgram(pr, jb, cp, pe, be, ce, self.device)
+            self._validate(pr, cp, jb)
+            self._prog = DeviceProgram(pr, jb, cp, pe, be, ce, self.device)
         return self._prog
 
-    def _check_entries(self, prolongs: np.ndarray, copies: np.ndarray, jobs=None):
-        span = (NUM_VARS - 1) * self.W ** 3
-        hi = self.pool_doubles - span
-        blk_hi = self.pool_doubles - NUM_VARS * self.W ** 3
-        for name, arr, top in (("dst", prolongs["dst"], hi), ("parent", prolongs["parent"], hi),
-                               ("dst", copies["dst"], hi), ("src", copies["src"], hi)) + \
-                (() if jobs is None else (("job dst", jobs["dst"], blk_hi + 1),
-                                          ("job parent", jobs["parent"], blk_hi + 1))):
+    def _validate(self, prolongs: np.ndarray, copies: np.ndarray, jobs: np.ndarray):
+        """Every entry must address points of blocks inside this pool (the device kernels do
+        not bound-check offsets)."""
+        blk = NUM_VARS * self.W ** 3
+        point_top = self.pool_doubles - (NUM_VARS - 1) * self.W ** 3
+        checks = [(prolongs["dst"], point_top), (prolongs["parent"], self.pool_doubles - blk + 1),
+                  (copies["dst"], point_top), (copies["src"], point_top),
+                  (jobs["dst"], self.pool_doubles - blk + 1),
+                  (jobs["parent"], self.pool_doubles - blk + 1)]
+        for arr, top in checks:
             if len(arr) and (arr.min() < 0 or arr.max() >= top):
-                raise RuntimeError(f"ghost program {name} offset outside the pool")
-        if jobs is not None and len(jobs) and (self.W > BLOCK_WMAX or
-                                               np.any(jobs["BS"] != self.BS) or
-                                               np.any(jobs["ng"] != self.ng)):
+                raise RuntimeError("ghost program entry outside the pool")
+        if len(jobs) and (self.W > BLOCK_WMAX or np.any(jobs["BS"] != self.BS) or
+                          np.any(jobs["ng"] != self.ng)):
             raise RuntimeError("whole-block jobs need this hierarchy's BS, ng and W <= 24")
         if len(prolongs) and (prolongs["fine"].min() < 0 or
                               prolongs["fine"].max() >= 2 * self.W):
@@ -535,45 +567,45 @@ class AMRHierarchy:
                 U.numel() != self.pool_doubles or not U.is_contiguous() or U.device != self.device:
             raise ValueError("U must be this hierarchy's contiguous float64 pool")
 
-    # --- data -----------------------------------------------------------------------------
+    # data ----------------------------------------------------------------------------------
     def fill(self, U: torch.Tensor, fn: Callable[[Grid], torch.Tensor], sync: bool = True):
-        """Every active block <- fn(its Grid) (a (24, W, W, W) state at the block's nodes)."""
+        """Every block <- fn(its Grid) (a (24, W, W, W) state at the block's nodes)."""
         self._check_pool(U)
-        for k in self.keys:
-            self.view(U, *k).copy_(fn(self.grid(k)))
+        for t in self.keys:
+            self.view(U, t).copy_(fn(self.grid(t)))
         if sync:
             self.sync_(U)
         return U
 
+    def _root_slices(self, t: Tile, g: int):
+        o = self.origin(t)
+        return (slice(None),) + tuple(slice(g + o[a], g + o[a] + self.BS) for a in range(3))
+
     def from_global(self, U: torch.Tensor, F: torch.Tensor, sync: bool = True):
-        """Root interiors <- the interior of a single-grid (24, N+2g, N+2g, N+2g) state (any
-        ghost width g), N = nb BS; then the hierarchy is synced."""
+        """Root interiors <- the interior of a single-grid (24, N+2g, ...) state (any ghost
+        width g), N = nb_root BS per axis; then the hierarchy is synced."""
         self._check_pool(U)
         g = (F.shape[1] - self.nb_root[0] * self.BS) // 2
-        BS, ng = self.BS, self.ng
-        for k in self.topo.blocks(0):
-            b = self.topo.bbox_ijk[k]
-            self.view(U, *k)[:, ng:ng + BS, ng:ng + BS, ng:ng + BS].copy_(
-                F[:, g + b[0]:g + b[0] + BS, g + b[1]:g + b[1] + BS, g + b[2]:g + b[2] + BS])
+        inner = (slice(None),) + (slice(self.ng, self.ng + self.BS),) * 3
+        for t in self.tree.on_level(0):
+            self.view(U, t)[inner].copy_(F[self._root_slices(t, g)])
         if sync:
             self.sync_(U)
         return U
 
     def to_global(self, U: torch.Tensor) -> torch.Tensor:
         """The root level's interiors as one (24, N0, N1, N2) array."""
-        BS, ng = self.BS, self.ng
-        N = [n * BS for n in self.nb_root]
-        out = torch.empty((NUM_VARS, *N), dtype=U.dtype, device=U.device)
-        for k in self.topo.blocks(0):
-            b = self.topo.bbox_ijk[k]
-            out[:, b[0]:b[0] + BS, b[1]:b[1] + BS, b[2]:b[2] + BS] = \
-                self.view(U, *k)[:, ng:ng + BS, ng:ng + BS, ng:ng + BS]
+        out = torch.empty((NUM_VARS, *[n * self.BS for n in self.nb_root]), dtype=U.dtype,
+                          device=U.device)
+        inner = (slice(None),) + (slice(self.ng, self.ng + self.BS),) * 3
+        for t in self.tree.on_level(0):
+            out[self._root_slices(t, 0)] = self.view(U, t)[inner]
         return out
 
-    # --- operators ------------------------------------------------------------------------
+    # operators -----------------------------------------------------------------------------
     def sync_(self, U: torch.Tensor) -> torch.Tensor:
         self._check_pool(U)
-        _lib.check(_lib.load().bssn_ghost_fill_f64(U.data_ptr(), self.program.ptr,
+        _lib.check(_lib.load().bssn_ghost_fill_f64(U.data_ptr(), U.numel(), self.program.ptr,
                                                    stream_handle(self.device)))
         return U
 
@@ -582,8 +614,7 @@ class AMRHierarchy:
 
     def step_(self, U: torch.Tensor, t: float, dt: float, params: PhysicsParams | None = None,
               order: int = 4, ko_sigma: float = 0.1, tdev: torch.Tensor | None = None):
-        """One shared-dt RK4 step of every active block in place (U synced on entry and on
-        exit)."""
+        """One shared-dt RK4 step of every block in place (U synced on entry and on exit)."""
         params = params or PhysicsParams()
         self._check_pool(U)
         b = self.batch
@@ -602,13 +633,13 @@ class AMRHierarchy:
     def capture_steps(self, U: torch.Tensor, tdev: torch.Tensor, dt: float, nsteps: int = 1,
                       **kw) -> torch.cuda.CUDAGraph:
         """Capture ``nsteps`` hierarchy steps of U into a HIP graph (time in tdev, advanced by
-        each replay). The topology must not change while the graph is in use."""
+        each replay). The tiles must not change while the graph is in use."""
         self.workspace()
         prog = self.program
-        warm = U.clone()                  # first launches (code loading) outside the capture
-        self.step_(warm, 0.0, dt, **kw)
+        scratch = U.clone()               # first launches (code loading) outside the capture
+        self.step_(scratch, 0.0, dt, **kw)
         torch.cuda.synchronize(self.device)
-        del warm
+        del scratch
         g = torch.cuda.CUDAGraph()
         s = torch.cuda.Stream(device=self.device)
         s.wait_stream(torch.cuda.current_stream(self.device))
@@ -626,142 +657,118 @@ class AMRHierarchy:
             t = t + dt
         return t
 
-    def indicators(self, U: torch.Tensor, fields: Sequence[int] | None = None) -> List[np.ndarray]:
-        """Per level, per slot: the largest undivided difference over the chosen fields (all by
-        default) of each active block, 0 for idle slots."""
+    def indicators(self, U: torch.Tensor,
+                   fields: Sequence[int] | None = None) -> Dict[Tile, float]:
+        """Per tile: the largest undivided difference over the chosen fields (all by default)."""
         self._check_pool(U)
-        keys = self.keys
-        out = torch.empty(len(keys) * NUM_VARS, dtype=torch.float64, device=self.device)
+        tiles = self.keys
+        out = torch.empty(len(tiles) * NUM_VARS, dtype=torch.float64, device=self.device)
         _lib.check(_lib.load().bssn_batch_indicator_f64(self.batch._handle, U.data_ptr(),
                                                         out.data_ptr(),
                                                         stream_handle(self.device)))
-        v = out.view(len(keys), NUM_VARS)
+        v = out.view(len(tiles), NUM_VARS)
         if fields is not None:
             v = v[:, list(fields)]
-        v = v.max(dim=1).values.cpu().numpy()
-        res = [np.zeros(self.topo.max_blocks) for _ in range(self.levels)]
-        for (L, s), x in zip(keys, v):
-            res[L][s] = x
-        return res
+        return dict(zip(tiles, v.max(dim=1).values.cpu().numpy().tolist()))
 
-    # --- regrid ---------------------------------------------------------------------------
-    def _grow(self, U: torch.Tensor, L: int) -> torch.Tensor:
-        """Enlarge level L's slot capacity (regrid.py:283-308): a new pool, levels copied."""
-        old = self.topo.caps[L]
-        new = min(max(old + max(8, old // 2), old + 1), self.topo.max_blocks)
-        if new <= old:
-            raise RuntimeError(f"AMR slot ceiling reached at level {L}: {old} == max_blocks")
-        old_base, old_caps = list(self.level_base), list(self.topo.caps)
-        self.topo.caps[L] = new
-        self.topo.grow_history.append((L, old, new))
-        self._layout()
+    # regrid --------------------------------------------------------------------------------
+    def _widen(self, U: torch.Tensor, level: int) -> torch.Tensor:
+        """Double a level's slot capacity: a new pool, every level copied to its new base."""
+        caps = [self.tree.capacity(L) for L in range(self.levels)]
+        base = list(self.level_base)
+        self.tree.widen(level)
+        self._place()
         V = self.empty_pool()
-        for l in range(self.levels):
-            n = old_caps[l] * self.slot_doubles
-            V[self.level_base[l]:self.level_base[l] + n] = U[old_base[l]:old_base[l] + n]
-        self._invalidate()
+        for L in range(self.levels):
+            n = caps[L] * self.slot_doubles
+            V[self.level_base[L]:self.level_base[L] + n] = U[base[L]:base[L] + n]
+        self._drop_plans()
         return V
 
-    def apply_flags(self, U: torch.Tensor, flags: np.ndarray, auto_grow: bool = True) -> torch.Tensor:
-        """Coarsen (finest level first: restrict into the parent, free the slot), then refine
-        (coarsest first: up to 8 children per flagged block, each prolongated from its parent
-        over the whole padded block) (regrid.py:255-420). Returns the pool (a new one if a
-        level grew); the hierarchy is synced."""
-        topo, BS, ng = self.topo, self.BS, self.ng
+    def _inject(self, U: torch.Tensor, tiles: List[Tile]) -> None:
+        """Copy each tile's coincident interior nodes into its parent (one launch)."""
+        if not tiles:
+            return
+        e = np.concatenate([restrict_entries(self.slot_offset(parent_tile(t)),
+                                             self.slot_offset(t), octant(t), self.BS, self.ng)
+                            for t in tiles])
+        _lib.check(_lib.load().bssn_batch_halo_f64(U.data_ptr(), U.numel(),
+                                                   upload(e, self.device).data_ptr(), len(e),
+                                                   stream_handle(self.device)))
+
+    def _interpolate(self, U: torch.Tensor, tiles: List[Tile]) -> None:
+        """Fill each new tile's whole padded block from its parent (one launch)."""
+        if not tiles:
+            return
         lib = _lib.load()
         st = stream_handle(self.device)
+        BS, ng = self.BS, self.ng
+        if self.W <= BLOCK_WMAX:
+            e = np.concatenate([block_job(self.slot_offset(t), self.slot_offset(parent_tile(t)),
+                                          octant(t), BS, ng, SKIP_INTERIOR_WRITE) for t in tiles])
+            _lib.check(lib.bssn_prolong_blocks_f64(U.data_ptr(), U.numel(),
+                                                   upload(e, self.device).data_ptr(), len(e), st))
+        else:
+            pts = _cube(self.W)
+            e = by_parity(np.concatenate([prolong_entries(pts, self.slot_offset(t),
+                                                          self.slot_offset(parent_tile(t)),
+                                                          octant(t), BS, ng) for t in tiles]))
+            _lib.check(lib.bssn_prolong_f64(U.data_ptr(), U.numel(),
+                                            upload(e, self.device).data_ptr(), len(e), st))
+
+    def apply(self, U: torch.Tensor, acts: Dict[Tile, int], grow: bool = True) -> torch.Tensor:
+        """Carry out regrid actions. Coarsening runs finest level first (each tile's nodes are
+        injected into its parent, then the tile is removed); refinement coarsest level first
+        (a refining tile gets all eight children, each interpolated from it over its whole
+        padded block; a full level doubles its slots when ``grow``). Returns the pool (a new one
+        if a level grew); the hierarchy is synced if anything changed."""
+        tree = self.tree
         changed = False
         for L in range(self.levels - 1, 0, -1):
-            ents = []                     # this level's restrictions: one copy launch
-            for (_, s) in topo.blocks(L):
-                key = (L, s)
-                if flags[L, s] != COARSEN or key not in topo.parent:
-                    continue
-                par = topo.parent[key]
-                ents.append(restrict_entries(self.slot_offset(*par), self.slot_offset(L, s),
-                                             topo.child_corner(key), BS, ng))
-                topo.remove_block(L, s)
-                changed = True
-            if ents:
-                e = np.concatenate(ents)
-                d = _upload(e, self.device)
-                _lib.check(lib.bssn_batch_halo_f64(U.data_ptr(), d.data_ptr(), len(e), st))
-        pts = _grid_points(self.W)
+            gone = [t for t in tree.on_level(L) if acts.get(t) == COARSEN]
+            self._inject(U, gone)
+            for t in gone:
+                tree.prune(t)
+            changed |= bool(gone)
         for L in range(self.levels - 1):
-            cl = L + 1
-            new = []                      # this level's new children: one prolongation launch
-            for (_, s) in topo.blocks(L):
-                if flags[L, s] != REFINE or not topo.active[L, s]:
+            born = []
+            for t in tree.on_level(L):
+                if acts.get(t) != REFINE:
                     continue
-                pb = topo.bbox_ijk[(L, s)]
-                existing = {topo.bbox_ijk[k] for k in topo.blocks(cl)}
-                for cx in (0, 1):
-                    for cy in (0, 1):
-                        for cz in (0, 1):
-                            c = (cx, cy, cz)
-                            cb = tuple(pb[a] * RATIO + c[a] * BS for a in range(3))
-                            if cb in existing:
-                                continue
-                            free = topo.free_slots(cl)
-                            if len(free) == 0:
-                                if not auto_grow:
-                                    raise RuntimeError(f"AMR slots exhausted at level {cl}")
-                                torch.cuda.synchronize(self.device)
-                                U = self._grow(U, cl)
-                                free = topo.free_slots(cl)
-                            slot = int(free[0])
-                            topo.add_block(cl, slot, cb, parent=(L, s))
-                            new.append((slot, s, c))
-                            changed = True
-            if new:        # offsets after any growth; parents are level L, children level L+1
-                if self.W <= BLOCK_WMAX:
-                    e = np.concatenate([block_job(self.slot_offset(cl, slot),
-                                                  self.slot_offset(L, ps), c, BS, ng,
-                                                  SKIP_INTERIOR_WRITE) for slot, ps, c in new])
-                    d = _upload(e, self.device)
-                    _lib.check(lib.bssn_prolong_blocks_f64(U.data_ptr(), d.data_ptr(), len(e),
-                                                           st))
-                else:
-                    e = by_parity(np.concatenate([
-                        prolong_entries(pts, self.slot_offset(cl, slot), self.slot_offset(L, ps),
-                                        c, BS, ng) for slot, ps, c in new]))
-                    d = _upload(e, self.device)
-                    _lib.check(lib.bssn_prolong_f64(U.data_ptr(), d.data_ptr(), len(e), st))
+                for c in child_tiles(t):
+                    if c in tree:
+                        continue
+                    if tree.arenas[L + 1].full():
+                        if not grow:
+                            raise RuntimeError(f"level {L + 1} has no free slot")
+                        torch.cuda.synchronize(self.device)
+                        U = self._widen(U, L + 1)
+                    tree.plant(c)
+                    born.append(c)
+            self._interpolate(U, born)      # offsets after any widening
+            changed |= bool(born)
         if changed:
-            self._invalidate()
-            topo.rebuild_neighbors()
+            self._drop_plans()
             self.sync_(U)
-        topo.record_occupancy()
+        tree.note_occupancy()
         return U
 
-    def regrid(self, U: torch.Tensor, *, refine_threshold: float, coarsen_threshold: float,
-               hysteresis_K: int = 3, n_buffer: int = 1, max_level: Optional[int] = None,
+    def regrid(self, U: torch.Tensor, policy: RegridPolicy,
                fields: Sequence[int] | None = None) -> torch.Tensor:
-        """One regrid pass: indicators -> flags -> nesting/buffer -> apply (regrid.py:425-460).
-        Returns the pool (a new one if a level grew)."""
-        ind = self.indicators(U, fields)
-        flags = compute_flags(ind, self.topo, refine_threshold=refine_threshold,
-                              coarsen_threshold=coarsen_threshold, hysteresis_K=hysteresis_K,
-                              max_level=max_level)
-        flags = enforce_nesting_buffer(flags, self.topo, n_buffer=n_buffer)
-        return self.apply_flags(U, flags)
+        """Indicators, the policy's decisions, and their application. Returns the pool."""
+        return self.apply(U, policy.decide(self.tree, self.indicators(U, fields)))
 
-    def evolve_with_regrid(self, U: torch.Tensor, t: float, dt: float, n_steps: int,
-                           regrid_every: int, *, refine_threshold: float,
-                           coarsen_threshold: float, hysteresis_K: int = 3, n_buffer: int = 1,
-                           max_level: Optional[int] = None, fields: Sequence[int] | None = None,
-                           on_regrid=None, **step_kw):
-        """n_steps shared-dt steps with a regrid every regrid_every steps (none after the last
-        chunk) (regrid.py:463-538). Returns (U, t)."""
+    def run(self, U: torch.Tensor, t: float, dt: float, nsteps: int, policy: RegridPolicy,
+            every: int, fields: Sequence[int] | None = None, on_regrid=None, **step_kw):
+        """nsteps shared-dt steps with a regrid after every ``every`` steps (not after the last
+        one). on_regrid(steps_done, U, self) is called after each regrid. Returns (U, t)."""
         done = 0
-        while done < n_steps:
-            chunk = min(regrid_every, n_steps - done)
-            t = self.evolve(U, t, dt, chunk, **step_kw)
-            done += chunk
-            if done < n_steps:
-                U = self.regrid(U, refine_threshold=refine_threshold,
-                                coarsen_threshold=coarsen_threshold, hysteresis_K=hysteresis_K,
-                                n_buffer=n_buffer, max_level=max_level, fields=fields)
+        while done < nsteps:
+            n = min(every, nsteps - done)
+            t = self.evolve(U, t, dt, n, **step_kw)
+            done += n
+            if done < nsteps:
+                U = self.regrid(U, policy, fields)
                 if on_regrid is not None:
                     on_regrid(done, U, self)
         return U, t

@@ -122,6 +122,7 @@ class BlockBatch(BlockLayout):
                                          stream_handle(self.device)))
         self._handle = handle
         self._work = None
+        self._int8_buf = None
         self.persistent_waves = 0
 
     def close(self):
@@ -213,8 +214,96 @@ class BlockBatch(BlockLayout):
 
     def halo_(self, U: torch.Tensor, halo: "HaloMap") -> torch.Tensor:
         self._check_pool(U)
-        _lib.check(_lib.load().bssn_batch_halo_f64(U.data_ptr(), halo.data.data_ptr(), halo.n,
-                                                   stream_handle(self.device)))
+        _lib.check(_lib.load().bssn_batch_halo_f64(U.data_ptr(), U.numel(), halo.data.data_ptr(),
+                                                   halo.n, stream_handle(self.device)))
+        return U
+
+    # --- the precision form over the batch ----------------------------------------------
+    @property
+    def bundle_doubles(self) -> int:
+        return int(_lib.load().bssn_batch_bundle_doubles(self._handle))
+
+    def bundle_offsets(self):
+        """Offset of each block's 138 derivative arrays in a batch bundle (the library's
+        layout: block after block, 64-double aligned)."""
+        out, pos = [], 0
+        for P in self.P:
+            out.append(pos)
+            pos += (138 * P + 63) // 64 * 64
+        return out
+
+    def bundle_view(self, D: torch.Tensor, b: int) -> torch.Tensor:
+        o = self.bundle_offsets()[b]
+        return D[o:o + 138 * self.P[b]].view(138, *self.S[b])
+
+    def deriv_bundle_i8(self, F: torch.Tensor, D: torch.Tensor, slices: int = 7) -> None:
+        """D <- the INT8-split derivative bundle of every block (one launch per pass)."""
+        self._check_pool(F)
+        if D.dtype != torch.float64 or D.numel() < self.bundle_doubles or D.device != self.device:
+            raise ValueError("D must hold bundle_doubles float64 values on the batch's device")
+        _lib.check(_lib.load().bssn_batch_deriv_bundle_i8_f64(
+            self._handle, F.data_ptr(), D.data_ptr(), 4, int(slices), stream_handle(self.device)))
+
+    def _int8_stage(self, Y, out, D, t, dt, params, halo, ko_sigma, slices):
+        """out <- f(Y) of every block: ghosts of Y from the halo map, the batched INT8 bundle,
+        then per block the FP64 algebra and the Kreiss-Oliger term (ghosts read as data)."""
+        if halo is not None:
+            self.halo_(Y, halo)
+        self.deriv_bundle_i8(Y, D, slices)
+        lib = _lib.load()
+        st = stream_handle(self.device)
+        offs = self.bundle_offsets()
+        for b, blk in enumerate(self.blocks):
+            Sx, Sy, Sz = self.S[b]
+            P = self.P[b]
+            Fb, Ob = self.view(Y, b), self.view(out, b)
+            Db = D[offs[b]:offs[b] + 138 * P]
+            _lib.check(lib.bssn_algebra_f64(Fb.data_ptr(), Db.data_ptr(),
+                                            _lib.dbl(scalars(params, *blk.h, dt, t)),
+                                            Ob.data_ptr(), P, st))
+            if ko_sigma > 0.0:
+                _lib.check(lib.bssn_ko_add_f64(Fb.data_ptr(), Ob.data_ptr(), Sx, Sy, Sz, 4,
+                                               float(ko_sigma), _lib.dbl(blk.h), 0, self.ng, st))
+
+    def int8_step_(self, U: torch.Tensor, t: float, dt: float,
+                   params: PhysicsParams | None = None, halo: "HaloMap | None" = None,
+                   ko_sigma: float = 0.1, slices: int = 7) -> torch.Tensor:
+        """One RK4 step of every block with the INT8-split derivatives (FD order 4), composed
+        as the single-grid INT8 step is (per stage: ghost refill, bundle, algebra, KO; RK4
+        combinations over the whole pool; enforce; a final ghost refill), the ghosts of each
+        stage input taken from other blocks by the halo map (U synced on entry)."""
+        params = params or PhysicsParams()
+        self._check_pool(U)
+        if self._int8_buf is None or self._int8_buf[0].numel() < self.pool_doubles:
+            n = self.pool_doubles
+            self._int8_buf = (torch.empty(n, dtype=torch.float64, device=self.device),
+                              torch.empty(n, dtype=torch.float64, device=self.device),
+                              torch.empty(n, dtype=torch.float64, device=self.device),
+                              torch.empty(self.bundle_doubles, dtype=torch.float64,
+                                          device=self.device))
+        k, acc, Y, D = self._int8_buf
+        Un = U[:self.pool_doubles]
+        h = 0.5 * dt
+        self._int8_stage(U, k, D, t, dt, params, None, ko_sigma, slices)
+        acc.copy_(k)
+        torch.add(Un, k, alpha=h, out=Y)
+        self._int8_stage(Y, k, D, t + h, dt, params, halo, ko_sigma, slices)
+        acc.add_(k, alpha=2.0)
+        torch.add(Un, k, alpha=h, out=Y)
+        self._int8_stage(Y, k, D, t + h, dt, params, halo, ko_sigma, slices)
+        acc.add_(k, alpha=2.0)
+        torch.add(Un, k, alpha=dt, out=Y)
+        self._int8_stage(Y, k, D, t + dt, dt, params, halo, ko_sigma, slices)
+        acc.add_(k)
+        Un.add_(acc, alpha=dt / 6.0)
+        lib = _lib.load()
+        st = stream_handle(self.device)
+        for b in range(len(self.blocks)):
+            v = self.view(U, b)
+            _lib.check(lib.bssn_enforce_f64(v.data_ptr(), self.P[b], params.chi_floor,
+                                            params.alpha_floor, st))
+        if halo is not None:
+            self.halo_(U, halo)
         return U
 
     # --- tuning and capture ---------------------------------------------------------------

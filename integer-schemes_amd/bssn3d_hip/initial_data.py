@@ -163,10 +163,13 @@ class Perturbation:
         return Perturbation(k, phi)
 
 
-def _mode_sum(grid: Grid, pert: Perturbation, f: int, device):
+def _mode_sum(grid: Grid, pert: Perturbation, f: int, device, domain=None):
     X, Y, Z = _coords(grid, device)
-    L = (grid.Nx * grid.dx, grid.Ny * grid.dy, grid.Nz * grid.dz)
-    lo = (grid.xmin, grid.ymin, grid.zmin)
+    if domain is None:
+        L = (grid.Nx * grid.dx, grid.Ny * grid.dy, grid.Nz * grid.dz)
+        lo = (grid.xmin, grid.ymin, grid.zmin)
+    else:
+        lo, L = domain
     acc = torch.zeros(grid.shape, dtype=torch.float64, device=device)
     for n in range(pert.k.shape[1]):
         kx, ky, kz = (int(v) for v in pert.k[f, n])
@@ -177,8 +180,10 @@ def _mode_sum(grid: Grid, pert: Perturbation, f: int, device):
 
 
 def punctures(grid: Grid, masses: Sequence[float], positions: Sequence[Sequence[float]],
-              amp: float = 1e-4, seed: int = 0, device="cpu") -> torch.Tensor:
-    """Brill-Lindquist punctures + seeded perturbations (see the module docstring)."""
+              amp: float = 1e-4, seed: int = 0, device="cpu", domain=None) -> torch.Tensor:
+    """Brill-Lindquist punctures + seeded perturbations (see the module docstring). ``domain``
+    = (lo, L) (3 coordinates each) sets the box the modes are periodic on (default: the grid's
+    own interior), so that blocks of one hierarchy sample one smooth function."""
     X, Y, Z = _coords(grid, device)
     h = min(grid.dx, grid.dy, grid.dz)
     psi = torch.ones(grid.shape, dtype=torch.float64, device=device)
@@ -194,7 +199,7 @@ def punctures(grid: Grid, masses: Sequence[float], positions: Sequence[Sequence[
     if amp != 0.0:
         pert = Perturbation.draw(np.random.default_rng(seed))
         for f in range(NUM_VARS):
-            d[f] += amp * _mode_sum(grid, pert, f, device)
+            d[f] += amp * _mode_sum(grid, pert, f, device, domain)
     return d
 
 
@@ -217,13 +222,13 @@ def config_single_puncture(seed: int = 0, N: int = 48, device="cpu", ng: int = 3
 
 
 def config_binary(seed: int = 0, N: int = 256, L: float = 16.0, masses=(0.5, 0.5),
-                  separation: float = 6.0, device="cpu"):
-    """N^3 interior, 3 ghosts, h = L/N on [-L/2, L/2): two punctures separated by ``separation``
-    along x, shifted together off the grid planes by a seeded sub-h offset, 1e-4 perturbations.
-    The default is the 256^3, h = 1/16 equal-mass binary."""
+                  separation: float = 6.0, device="cpu", ng: int = 3):
+    """N^3 interior, ng ghosts (3 by default), h = L/N on [-L/2, L/2): two punctures separated
+    by ``separation`` along x, shifted together off the grid planes by a seeded sub-h offset,
+    1e-4 perturbations. The default is the 256^3, h = 1/16 equal-mass binary."""
     h = L / N
     lo = -0.5 * L
-    grid = Grid(N, N, N, h, h, h, ng=3, xmin=lo, ymin=lo, zmin=lo)
+    grid = Grid(N, N, N, h, h, h, ng=ng, xmin=lo, ymin=lo, zmin=lo)
     rng = np.random.default_rng(seed)
     s = offgrid_shift(rng, h)
     p1 = np.array([-0.5 * separation, 0.0, 0.0]) + s

@@ -12,9 +12,11 @@ Schemes run one after the other on the same initial data:
            INT8 matrix cores) and the FP64 algebra (the reference's per-stage composition);
   "batched" the same grid tiled into equal blocks, advanced by the batched fused kernel with
            halo copies after every stage (bit-identical to "fused"; the block schedule's
-           throughput at this size).
-The INT8-split derivatives exist for the single-grid composed step only, so "int8" and
-"batched" are measured separately against "fused" rather than combined.
+           throughput at this size);
+  "int8_batched"
+           the two combined: the same block tiling, every stage's derivatives from the batched
+           INT8-split bundle of all blocks (bssn_batch_deriv_bundle_i8_f64), the FP64 algebra
+           and KO per block, halo copies after every stage (BlockBatch.int8_step_).
 
 The Hamiltonian and momentum constraint norms (interior RMS, FD order 4) are sampled every
 ``sample_every`` steps; the step time excludes the sampling. The INT8 run is compared with the
@@ -63,7 +65,7 @@ def run_scheme(scheme: str, N: int = 384, steps: int = 200, sample_every: int = 
     dt = 0.25 * grid.dx
     cs = ConstraintSolver(grid, order=4)
     npts = N ** 3
-    if scheme == "batched":
+    if scheme in ("batched", "int8_batched"):
         if N % tiles:
             raise ValueError("N must be a multiple of the tile count")
         n = N // tiles
@@ -89,9 +91,14 @@ def run_scheme(scheme: str, N: int = 384, steps: int = 200, sample_every: int = 
             BSSNEvolution(grid, p, order=4, bc="periodic").periodic_sync_(U)
             return U
 
-        def advance(k, t):
-            for i in range(k):
-                batch.step_(pool, t + i * dt, dt, p, 4, 0.1, "halo", halo)
+        if scheme == "batched":
+            def advance(k, t):
+                for i in range(k):
+                    batch.step_(pool, t + i * dt, dt, p, 4, 0.1, "halo", halo)
+        else:
+            def advance(k, t):
+                for i in range(k):
+                    batch.int8_step_(pool, t + i * dt, dt, p, halo, ko_sigma=0.1)
         state = to_grid
     else:
         ev = BSSNEvolution(grid, p, order=4, ko_sigma=0.1, bc="periodic", scheme=scheme)
@@ -109,7 +116,7 @@ def run_scheme(scheme: str, N: int = 384, steps: int = 200, sample_every: int = 
     torch.cuda.synchronize(dev)
     U.copy_(U0)
     del U0
-    if scheme == "batched":
+    if scheme in ("batched", "int8_batched"):
         for b, o in enumerate(tl):
             batch.view(pool, b).copy_(U[:, o[0]:o[0] + n + 2 * ng, o[1]:o[1] + n + 2 * ng,
                                         o[2]:o[2] + n + 2 * ng])
@@ -148,7 +155,8 @@ def compare(ref: Dict, cand: Dict, floor: float = 1e-12) -> Dict:
 
 
 def stress(N: int = 384, steps: int = 200, sample_every: int = 20, seed: int = 0,
-           schemes=("fused", "int8", "batched"), device="cuda", verbose: bool = False) -> Dict:
+           schemes=("fused", "int8", "batched", "int8_batched"), device="cuda",
+           verbose: bool = False) -> Dict:
     out = {}
     for s in schemes:
         out[s] = run_scheme(s, N, steps, sample_every, seed, device, verbose=verbose)
@@ -165,7 +173,7 @@ def main(argv=None) -> int:
     ap.add_argument("--N", type=int, default=384)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--sample-every", type=int, default=20)
-    ap.add_argument("--schemes", default="fused,int8,batched")
+    ap.add_argument("--schemes", default="fused,int8,batched,int8_batched")
     ap.add_argument("--out", default=None, help="write the results as JSON here")
     a = ap.parse_args(argv)
     res = stress(a.N, a.steps, a.sample_every, schemes=tuple(a.schemes.split(",")), verbose=True)

@@ -1,6 +1,7 @@
 // C ABI of the BSSN HIP library: argument checks, scalar unpacking and kernel sequencing.
 // Declarations and their reference counterparts: include/bssn_hip.h.
 #include "bssn_hip.h"
+#include "bssn_entries.h"
 
 #include <hip/hip_runtime.h>
 
@@ -616,6 +617,9 @@ struct bssn_batch {
   int grid_full = 0, grid_inner = 0, grid_ghost = 0;   // one workgroup per brick
   int* ctr = nullptr;                // device queue counters
   int persistent = 0;                // 0, or the persistent wave count
+  const long long* i8pre = nullptr;  // device: 5 prefix arrays of the INT8 bundle passes
+  long long i8tot[5] = {0, 0, 0, 0, 0};
+  long long bundle_doubles = 0;      // 138 arrays per block, block by block (BlockDesc::doff)
 };
 
 namespace {
@@ -780,6 +784,7 @@ size_t bssn_batch_plan_bytes(int nblk, const int* S, int ng) {
     if (S[b] - 2 * ng < 1) return 0;
   size_t n = align_up(sizeof(BlockDesc) * (size_t)nblk);
   for (int w = 0; w < 3; ++w) n += align_up(sizeof(BrickItem) * (size_t)list_count(nblk, S, ng, w));
+  n += align_up(sizeof(long long) * 5 * ((size_t)nblk + 1));   // INT8 bundle pass prefixes
   return n + align_up(sizeof(int) * BATCH_CHUNKS);
 }
 
@@ -794,11 +799,16 @@ int bssn_batch_create(bssn_batch** out, void* plan, size_t plan_bytes, int nblk,
     return fail(BSSN_E_ARG, "plan buffer too small: need " + std::to_string(need) + " bytes");
   std::vector<BlockDesc> desc((size_t)nblk);
   std::vector<std::vector<BrickItem>> pf((size_t)nblk), pi((size_t)nblk), pg((size_t)nblk);
-  long long interior = 0, padded = 0;
+  std::vector<long long> pre(5 * ((size_t)nblk + 1), 0);
+  long long interior = 0, padded = 0, doff = 0;
   for (int b = 0; b < nblk; ++b) {
     const int* s = S + 3 * b;
     BlockDesc& d = desc[(size_t)b];
     d.off = (long)off[b];
+    d.doff = (long)doff;
+    doff += (((long long)BSSN_ND * s[0] * s[1] * s[2]) + 63) / 64 * 64;
+    for (int k = 0; k < 5; ++k)
+      pre[(size_t)k * (nblk + 1) + b + 1] = pre[(size_t)k * (nblk + 1) + b] + i8_pass_tiles(k, s);
     d.ng = ng;
     for (int a = 0; a < 3; ++a) {
       d.S[a] = s[a];
@@ -826,6 +836,7 @@ int bssn_batch_create(bssn_batch** out, void* plan, size_t plan_bytes, int nblk,
   const size_t o_full = put(L.full.data(), sizeof(BrickItem) * L.full.size());
   const size_t o_inner = put(L.inner.data(), sizeof(BrickItem) * L.inner.size());
   const size_t o_ghost = put(L.ghost.data(), sizeof(BrickItem) * L.ghost.size());
+  const size_t o_pre = put(pre.data(), sizeof(long long) * pre.size());
   const size_t o_ctr = pos;
   hipStream_t st = (hipStream_t)stream;
   TRY(check_hip(hipMemcpyAsync(plan, img.data(), need, hipMemcpyHostToDevice, st), "plan upload"));
@@ -851,6 +862,9 @@ int bssn_batch_create(bssn_batch** out, void* plan, size_t plan_bytes, int nblk,
   bt->grid_inner = flat_grid(L.cb_inner);
   bt->grid_ghost = flat_grid(L.cb_ghost);
   bt->ctr = (int*)(base + o_ctr);
+  bt->i8pre = (const long long*)(base + o_pre);
+  for (int k = 0; k < 5; ++k) bt->i8tot[k] = pre[(size_t)k * (nblk + 1) + nblk];
+  bt->bundle_doubles = doff;
   *out = bt;
   return BSSN_OK;
 }
@@ -885,6 +899,21 @@ int bssn_batch_rhs_f64(const bssn_batch* b, const double* F, const double* S16, 
   return run_batch(b, s, a, fd_order, ko_sigma > 0.0, CLAMP, (hipStream_t)stream);
 }
 
+long long bssn_batch_bundle_doubles(const bssn_batch* b) { return b ? b->bundle_doubles : -1; }
+
+int bssn_batch_deriv_bundle_i8_f64(const bssn_batch* b, const double* F, double* D, int fd_order,
+                                   int slices, void* stream) {
+  TRY(check_batch(b));
+  if (!F || !D) return fail(BSSN_E_ARG, "null pointer");
+  if (fd_order != 4)
+    return fail(BSSN_E_ARG, "the INT8-split bundle supports fd_order 4 only (order-6 second "
+                            "derivative numerators exceed int8)");
+  if (slices < 1 || slices > 7) return fail(BSSN_E_ARG, "slices must be 1..7");
+  return check_hip(launch_batch_bundle_i8(F, D, b->blk, b->i8pre, b->i8tot, b->nblk, slices,
+                                          (hipStream_t)stream),
+                   "batched INT8 bundle launch");
+}
+
 size_t bssn_batch_step_workspace_bytes(const bssn_batch* b, int integrator) {
   if (!b) return 0;
   const size_t pool = (size_t)b->pool_doubles * sizeof(double);
@@ -899,10 +928,26 @@ int bssn_batch_periodic_sync_f64(const bssn_batch* b, double* U, void* stream) {
                    "batched periodic sync");
 }
 
-int bssn_batch_halo_f64(double* pool, const bssn_halo_copy* map, long long n, void* stream) {
-  if (!pool) return fail(BSSN_E_ARG, "null pool pointer");
+int bssn_batch_halo_f64(double* pool, long long pool_doubles, const bssn_halo_copy* map,
+                        long long n, void* stream) {
+  if (!pool || pool_doubles <= 0) return fail(BSSN_E_ARG, "null or empty pool");
   if (n < 0 || (n > 0 && !map)) return fail(BSSN_E_ARG, "bad halo map");
-  return check_hip(launch_halo_copy(pool, map, (long)n, (hipStream_t)stream), "halo copy");
+  return check_hip(launch_halo_copy(pool, pool_doubles, map, (long)n, (hipStream_t)stream),
+                   "halo copy");
+}
+
+long long bssn_count_skipped_entries(int kind, const void* entries, long long n,
+                                     long long pool_doubles) {
+  if (n < 0 || (n > 0 && !entries) || pool_doubles <= 0 || kind < 0 || kind > 2) return -1;
+  long long bad = 0;
+  for (long long i = 0; i < n; ++i) {
+    bool ok;
+    if (kind == 0) ok = halo_entry_ok(((const bssn_halo_copy*)entries)[i], pool_doubles);
+    else if (kind == 1) ok = prolong_entry_ok(((const bssn_prolong_point*)entries)[i], pool_doubles);
+    else ok = block_job_ok(((const bssn_prolong_block*)entries)[i], pool_doubles);
+    bad += !ok;
+  }
+  return bad;
 }
 
 }  // extern "C"
@@ -980,17 +1025,19 @@ int check_program(const bssn_ghost_program* p) {
   return BSSN_OK;
 }
 
-int run_program(double* pool, const bssn_ghost_program* p, hipStream_t st) {
+int run_program(double* pool, long long pool_n, const bssn_ghost_program* p, hipStream_t st) {
   long long pb = 0, bb = 0, cb = 0;
   for (int i = 0; i < p->nphase; ++i) {
     const long long pe = p->prolong_end[i], be = p->block_end[i], ce = p->copy_end[i];
     if (pe > pb)
-      TRY(check_hip(launch_prolong(pool, p->prolongs + pb, (long)(pe - pb), st), "prolongation"));
+      TRY(check_hip(launch_prolong(pool, pool_n, p->prolongs + pb, (long)(pe - pb), st),
+                    "prolongation"));
     if (be > bb)
-      TRY(check_hip(launch_prolong_blocks(pool, p->blocks + bb, (long)(be - bb), st),
+      TRY(check_hip(launch_prolong_blocks(pool, pool_n, p->blocks + bb, (long)(be - bb), st),
                     "block prolongation"));
     if (ce > cb)
-      TRY(check_hip(launch_halo_copy(pool, p->copies + cb, (long)(ce - cb), st), "ghost copies"));
+      TRY(check_hip(launch_halo_copy(pool, pool_n, p->copies + cb, (long)(ce - cb), st),
+                    "ghost copies"));
     pb = pe;
     bb = be;
     cb = ce;
@@ -1011,30 +1058,33 @@ int bssn_batch_rk4_step_f64(const bssn_batch* b, double* U, double* work, const 
     return fail(BSSN_E_ARG, "bc = 2 needs a halo map");
   hipStream_t st = (hipStream_t)stream;
   auto sync = [&](double* pool) {
-    return check_hip(launch_halo_copy(pool, halo, (long)nhalo, st), "halo copy");
+    return check_hip(launch_halo_copy(pool, b->pool_doubles, halo, (long)nhalo, st), "halo copy");
   };
   return batch_rk4(b, U, work, S16, fd_order, ko_sigma, bc, sync, true, t, dt, chi_floor,
                    alpha_floor, tdev, st);
 }
 
-int bssn_prolong_f64(double* pool, const bssn_prolong_point* pts, long long n, void* stream) {
-  if (!pool) return fail(BSSN_E_ARG, "null pool pointer");
+int bssn_prolong_f64(double* pool, long long pool_doubles, const bssn_prolong_point* pts,
+                     long long n, void* stream) {
+  if (!pool || pool_doubles <= 0) return fail(BSSN_E_ARG, "null or empty pool");
   if (n < 0 || (n > 0 && !pts)) return fail(BSSN_E_ARG, "bad prolongation list");
-  return check_hip(launch_prolong(pool, pts, (long)n, (hipStream_t)stream), "prolongation");
+  return check_hip(launch_prolong(pool, pool_doubles, pts, (long)n, (hipStream_t)stream),
+                   "prolongation");
 }
 
-int bssn_prolong_blocks_f64(double* pool, const bssn_prolong_block* jobs, long long n,
-                            void* stream) {
-  if (!pool) return fail(BSSN_E_ARG, "null pool pointer");
+int bssn_prolong_blocks_f64(double* pool, long long pool_doubles, const bssn_prolong_block* jobs,
+                            long long n, void* stream) {
+  if (!pool || pool_doubles <= 0) return fail(BSSN_E_ARG, "null or empty pool");
   if (n < 0 || (n > 0 && !jobs)) return fail(BSSN_E_ARG, "bad prolongation job list");
-  return check_hip(launch_prolong_blocks(pool, jobs, (long)n, (hipStream_t)stream),
+  return check_hip(launch_prolong_blocks(pool, pool_doubles, jobs, (long)n, (hipStream_t)stream),
                    "block prolongation");
 }
 
-int bssn_ghost_fill_f64(double* pool, const bssn_ghost_program* prog, void* stream) {
-  if (!pool) return fail(BSSN_E_ARG, "null pool pointer");
+int bssn_ghost_fill_f64(double* pool, long long pool_doubles, const bssn_ghost_program* prog,
+                        void* stream) {
+  if (!pool || pool_doubles <= 0) return fail(BSSN_E_ARG, "null or empty pool");
   TRY(check_program(prog));
-  return run_program(pool, prog, (hipStream_t)stream);
+  return run_program(pool, pool_doubles, prog, (hipStream_t)stream);
 }
 
 int bssn_batch_rk4_step_prog_f64(const bssn_batch* b, double* U, double* work, const double* S16,
@@ -1043,7 +1093,7 @@ int bssn_batch_rk4_step_prog_f64(const bssn_batch* b, double* U, double* work, c
                                  double* tdev, void* stream) {
   TRY(check_program(prog));
   hipStream_t st = (hipStream_t)stream;
-  auto sync = [&](double* pool) { return run_program(pool, prog, st); };
+  auto sync = [&](double* pool) { return run_program(pool, b->pool_doubles, prog, st); };
   return batch_rk4(b, U, work, S16, fd_order, ko_sigma, 2, sync, false, t, dt, chi_floor,
                    alpha_floor, tdev, st);
 }

@@ -14,6 +14,7 @@
 // interior plus one ring, per (block, field), NaN-propagating like a max over an array.
 #include <hip/hip_runtime.h>
 
+#include "bssn_entries.h"
 #include "bssn_hip.h"
 #include "bssn_kernels.h"
 
@@ -85,13 +86,15 @@ __device__ __forceinline__ void prolong_point(double* __restrict__ pool,
   }
 }
 
-// One thread per entry. Entries sorted by parity class keep a wave on one branch.
+// One thread per entry. Entries sorted by parity class keep a wave on one branch. An entry
+// that would reach outside the pool (pool_n doubles) is skipped.
 __global__ __launch_bounds__(256) void k_prolong(double* __restrict__ pool,
                                                  const bssn_prolong_point* __restrict__ pts,
-                                                 long n) {
+                                                 long n, long long pool_n) {
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   const bssn_prolong_point m = pts[e];
+  if (!prolong_entry_ok(m, pool_n)) return;
   const int cls = (m.fine[0] & 1) << 2 | (m.fine[1] & 1) << 1 | (m.fine[2] & 1);
   switch (cls) {
     case 0: prolong_point<false, false, false>(pool, m); break;
@@ -105,11 +108,12 @@ __global__ __launch_bounds__(256) void k_prolong(double* __restrict__ pool,
   }
 }
 
-hipError_t launch_prolong(double* pool, const void* pts, long n, hipStream_t st) {
+hipError_t launch_prolong(double* pool, long long pool_n, const void* pts, long n,
+                          hipStream_t st) {
   if (n <= 0) return hipSuccess;
   const long blocks = (n + 255) / 256;
   hipLaunchKernelGGL(k_prolong, dim3((unsigned)blocks), dim3(256), 0, st, pool,
-                     (const bssn_prolong_point*)pts, n);
+                     (const bssn_prolong_point*)pts, n, pool_n);
   return hipGetLastError();
 }
 
@@ -117,16 +121,17 @@ hipError_t launch_prolong(double* pool, const void* pts, long n, hipStream_t st)
 // X[rx][ky][kz] (fine x index, coarse y and z window) from the parent, the y pass
 // Y[rx][ry][kz] from X, and the z sums are formed only at the points the job writes. Window
 // index k stands for parent node kmin + k, read clamped, so the taps of every sum are the
-// clamped nodes of the per-point kernel, summed in the same order.
-constexpr int PB_WMAX = 24;
-constexpr int PB_KMAX = PB_WMAX / 2 + 7;
-
+// clamped nodes of the per-point kernel, summed in the same order. A job that is not a valid
+// block of at most PB_WMAX padded nodes per axis inside the pool (pool_n doubles) is skipped
+// (the test is uniform across the workgroup, so no thread reaches a barrier alone).
 __global__ __launch_bounds__(1024) void k_prolong_block(double* __restrict__ pool,
-                                                       const bssn_prolong_block* __restrict__ jobs) {
+                                                       const bssn_prolong_block* __restrict__ jobs,
+                                                       long long pool_n) {
 #pragma clang fp contract(off)
   __shared__ double X[PB_WMAX * PB_KMAX * PB_KMAX];
   __shared__ double Y[PB_WMAX * PB_WMAX * PB_KMAX];
   const bssn_prolong_block jb = jobs[blockIdx.x / BSSN_NF];
+  if (!block_job_ok(jb, pool_n)) return;
   const int f = (int)(blockIdx.x % BSSN_NF);
   const int W = jb.BS + 2 * jb.ng;
   const long sy = W, sx = (long)W * W, P = sx * W;
@@ -198,10 +203,11 @@ __global__ __launch_bounds__(1024) void k_prolong_block(double* __restrict__ poo
   }
 }
 
-hipError_t launch_prolong_blocks(double* pool, const void* jobs, long n, hipStream_t st) {
+hipError_t launch_prolong_blocks(double* pool, long long pool_n, const void* jobs, long n,
+                                 hipStream_t st) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_prolong_block, dim3((unsigned)(n * BSSN_NF)), dim3(1024), 0, st, pool,
-                     (const bssn_prolong_block*)jobs);
+                     (const bssn_prolong_block*)jobs, pool_n);
   return hipGetLastError();
 }
 

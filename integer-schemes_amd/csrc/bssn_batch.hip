@@ -2,6 +2,7 @@
 // device time) and the dispatch of the batched stage kernels (bssn_batch.h).
 #include <hip/hip_runtime.h>
 
+#include "bssn_entries.h"
 #include "bssn_hip.h"
 #include "bssn_kernels.h"
 #include "bssn_params.h"
@@ -117,12 +118,14 @@ hipError_t launch_batch_hold(const double* U, double* Ya, double* Yb, const Bloc
 // Ghost exchange between blocks: entry e copies all 24 fields of one point,
 // pool[dst + f * dst_stride] = pool[src + f * src_stride]. Sources are interior points, which
 // no entry writes, so the order of the copies does not matter.
+// An entry that would reach outside the pool (pool_n doubles) is skipped.
 __global__ __launch_bounds__(256) void k_halo_copy(double* __restrict__ pool,
                                                    const bssn_halo_copy* __restrict__ map,
-                                                   long n) {
+                                                   long n, long long pool_n) {
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   const bssn_halo_copy m = map[e];
+  if (!halo_entry_ok(m, pool_n)) return;
   // all 24 loads before the stores (an entry's interior source is never a ghost destination),
   // so they are in flight together instead of each waiting behind the previous store
   double v[BSSN_NF];
@@ -132,11 +135,12 @@ __global__ __launch_bounds__(256) void k_halo_copy(double* __restrict__ pool,
   for (int f = 0; f < BSSN_NF; ++f) pool[m.dst + (long)f * m.dst_stride] = v[f];
 }
 
-hipError_t launch_halo_copy(double* pool, const void* map, long n, hipStream_t st) {
+hipError_t launch_halo_copy(double* pool, long long pool_n, const void* map, long n,
+                            hipStream_t st) {
   if (n <= 0) return hipSuccess;
   const long blocks = (n + 255) / 256;
   hipLaunchKernelGGL(k_halo_copy, dim3((unsigned)blocks), dim3(256), 0, st, pool,
-                     (const bssn_halo_copy*)map, n);
+                     (const bssn_halo_copy*)map, n, pool_n);
   return hipGetLastError();
 }
 

@@ -100,10 +100,22 @@ struct I8Args {
   int nslice;
 };
 
+// Launch geometry of one (ORIENT, TWO) pass over a grid of padded extents S: the number of
+// wave tiles (fields x tiles along the plane's two axes x planes).
+template <int ORIENT, bool TWO>
+__host__ __device__ inline long i8_tiles(const int* S) {
+  const int ax1 = ORIENT == 0 ? 1 : 0, ax2 = ORIENT == 2 ? 1 : 2;
+  const int axp = ORIENT == 0 ? 0 : (ORIENT == 1 ? 1 : 2);
+  const long nt1 = (S[ax1] + I8_TILE - 1) / I8_TILE, nt2 = (S[ax2] + I8_TILE - 1) / I8_TILE;
+  return (TWO ? 11L : 13L) * nt1 * nt2 * S[axp];
+}
+
 // ORIENT 0: (y, z) planes at each x; 1: (x, z) planes at each y; 2: (x, y) planes at each z.
 // TWO: the fields with second derivatives (else first derivatives only).
+// One wave: tile w (plane fastest, then tile along axis 2, along axis 1, field) of the grid a.
 template <int ORIENT, bool TWO>
-__global__ __launch_bounds__(64) void k_i8_plane(I8Args a) {
+__device__ __forceinline__ void i8_tile(const I8Args& a, long w, int lane, signed char* tq,
+                                        signed char* th, signed char* tl) {
   constexpr int AX1 = ORIENT == 0 ? 1 : 0;
   constexpr int AX2 = ORIENT == 2 ? 1 : 2;
   constexpr int AXP = ORIENT == 0 ? 0 : (ORIENT == 1 ? 1 : 2);
@@ -114,15 +126,9 @@ __global__ __launch_bounds__(64) void k_i8_plane(I8Args a) {
   constexpr bool MIX = TWO;
   constexpr int NF = TWO ? 11 : 13;
 
-  __shared__ __attribute__((aligned(16))) signed char tq[I8_BOX * I8_BOX];
-  __shared__ __attribute__((aligned(16))) signed char th[I8_BOX * I8_BOX];
-  __shared__ __attribute__((aligned(16))) signed char tl[I8_BOX * I8_BOX];
-
-  const int lane = (int)threadIdx.x;
   const int n1 = a.S[AX1], n2 = a.S[AX2], npl = a.S[AXP];
   const long st[3] = {(long)a.S[1] * a.S[2], (long)a.S[2], 1L};
   const int nt1 = (n1 + I8_TILE - 1) / I8_TILE, nt2 = (n2 + I8_TILE - 1) / I8_TILE;
-  long w = (long)blockIdx.x;
   const int pl = (int)(w % npl);
   w /= npl;
   const int t2 = (int)(w % nt2);
@@ -263,16 +269,97 @@ __global__ __launch_bounds__(64) void k_i8_plane(I8Args a) {
   }
 }
 
+#define BSSN_I8_LDS                                                          \
+  __shared__ __attribute__((aligned(16))) signed char tq[I8_BOX * I8_BOX];   \
+  __shared__ __attribute__((aligned(16))) signed char th[I8_BOX * I8_BOX];   \
+  __shared__ __attribute__((aligned(16))) signed char tl[I8_BOX * I8_BOX]
+
+template <int ORIENT, bool TWO>
+__global__ __launch_bounds__(64) void k_i8_plane(I8Args a) {
+  BSSN_I8_LDS;
+  i8_tile<ORIENT, TWO>(a, (long)blockIdx.x, (int)threadIdx.x, tq, th, tl);
+}
+
+// The same pass over every block of a batch: block b owns wave tiles [pre[b], pre[b + 1]) of
+// the launch; its bundle is the 138 arrays at D + blk[b].doff and its stencil scales are formed
+// from its spacings with the host's rounding, so each block's bundle equals the single-grid
+// bundle of that block bit for bit.
+template <int ORIENT, bool TWO>
+__global__ __launch_bounds__(64) void k_i8_plane_batch(const double* pool, double* D,
+                                                       const BlockDesc* __restrict__ blks,
+                                                       const long long* __restrict__ pre,
+                                                       int nblk, int nslice) {
+  BSSN_I8_LDS;
+  const long long w = (long long)blockIdx.x;
+  int lo = 0, hi = nblk;                 // pre[lo] <= w < pre[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (pre[mid] <= w) lo = mid;
+    else hi = mid;
+  }
+  const int b = __builtin_amdgcn_readfirstlane(lo);
+  const BlockDesc blk = blks[b];
+  I8Args a;
+  a.F = pool + blk.off;
+  a.D = D + blk.doff;
+  for (int i = 0; i < 3; ++i) a.S[i] = blk.S[i];
+  a.P = (long)blk.S[0] * blk.S[1] * blk.S[2];
+  const Scales sc = make_scales(blk.h, 4, 0.0);
+  for (int i = 0; i < 3; ++i) {
+    a.s1[i] = uniform(sc.s1[i]);
+    a.s2[i] = uniform(sc.s2[i]);
+    a.sm[i] = uniform(sc.sm[i]);
+  }
+  a.nslice = nslice;
+  i8_tile<ORIENT, TWO>(a, (long)(w - pre[b]), (int)threadIdx.x, tq, th, tl);
+}
+
 template <int ORIENT, bool TWO>
 static hipError_t launch_i8_t(const I8Args& a, hipStream_t st) {
-  constexpr int AX1 = ORIENT == 0 ? 1 : 0;
-  constexpr int AX2 = ORIENT == 2 ? 1 : 2;
-  constexpr int AXP = ORIENT == 0 ? 0 : (ORIENT == 1 ? 1 : 2);
-  const long nt1 = (a.S[AX1] + I8_TILE - 1) / I8_TILE, nt2 = (a.S[AX2] + I8_TILE - 1) / I8_TILE;
-  const long n = (TWO ? 11L : 13L) * nt1 * nt2 * a.S[AXP];
+  const long n = i8_tiles<ORIENT, TWO>(a.S);
   if (n > 0x7fffffffL) return hipErrorInvalidValue;
   hipLaunchKernelGGL((k_i8_plane<ORIENT, TWO>), dim3((unsigned)n), dim3(64), 0, st, a);
   return hipGetLastError();
+}
+
+// wave tiles of pass k (0: (y,z) planes, fields with second derivatives; 1: (y,z), the others;
+// 2, 3: the same on (x,z) planes; 4: (x,y) planes, mixed d_xy) on a grid of padded extents S
+long i8_pass_tiles(int k, const int* S) {
+  switch (k) {
+    case 0: return i8_tiles<0, true>(S);
+    case 1: return i8_tiles<0, false>(S);
+    case 2: return i8_tiles<1, true>(S);
+    case 3: return i8_tiles<1, false>(S);
+    default: return i8_tiles<2, true>(S);
+  }
+}
+
+template <int ORIENT, bool TWO>
+static hipError_t launch_i8_batch_t(const double* pool, double* D, const BlockDesc* blk,
+                                    const long long* pre, long long n, int nblk, int nslice,
+                                    hipStream_t st) {
+  if (n > 0x7fffffffLL) return hipErrorInvalidValue;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL((k_i8_plane_batch<ORIENT, TWO>), dim3((unsigned)n), dim3(64), 0, st, pool,
+                     D, blk, pre, nblk, nslice);
+  return hipGetLastError();
+}
+
+// the five passes over a batch; pre: 5 prefix arrays of nblk + 1 entries (device), total: the
+// five tile counts (host)
+hipError_t launch_batch_bundle_i8(const double* pool, double* D, const BlockDesc* blk,
+                                  const long long* pre, const long long* total, int nblk,
+                                  int nslice, hipStream_t st) {
+  const long long* p = pre;
+  const long long m = nblk + 1;
+  hipError_t e;
+  if ((e = launch_i8_batch_t<0, true>(pool, D, blk, p, total[0], nblk, nslice, st))) return e;
+  if ((e = launch_i8_batch_t<0, false>(pool, D, blk, p + m, total[1], nblk, nslice, st))) return e;
+  if ((e = launch_i8_batch_t<1, true>(pool, D, blk, p + 2 * m, total[2], nblk, nslice, st)))
+    return e;
+  if ((e = launch_i8_batch_t<1, false>(pool, D, blk, p + 3 * m, total[3], nblk, nslice, st)))
+    return e;
+  return launch_i8_batch_t<2, true>(pool, D, blk, p + 4 * m, total[4], nblk, nslice, st);
 }
 
 hipError_t launch_deriv_bundle_i8(const double* F, double* D, const Dims& d, const Scales& w,

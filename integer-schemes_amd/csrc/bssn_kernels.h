@@ -75,6 +75,7 @@ struct BlockDesc {
   int S[3];       // padded extents
   int ng;         // ghost width
   double h[3];    // spacings
+  long doff;      // offset of the block's 138 derivative arrays in a batch bundle, in doubles
 };
 
 // One 4x4x4 brick of one block: the block index and the brick origin in the block's padded
@@ -112,9 +113,16 @@ hipError_t launch_batch_o6(const StageArgs& a, const BatchArgs& b, int grid, boo
 hipError_t launch_batch_reset(int* ctr, hipStream_t st);
 hipError_t launch_batch_periodic(double* pool, const BlockDesc* blk, const BrickList& ghost,
                                  int grid, hipStream_t st);
-hipError_t launch_halo_copy(double* pool, const void* map, long n, hipStream_t st);
-hipError_t launch_prolong(double* pool, const void* pts, long n, hipStream_t st);
-hipError_t launch_prolong_blocks(double* pool, const void* jobs, long n, hipStream_t st);
+long i8_pass_tiles(int k, const int* S);
+hipError_t launch_batch_bundle_i8(const double* pool, double* D, const BlockDesc* blk,
+                                  const long long* pre, const long long* total, int nblk,
+                                  int nslice, hipStream_t st);
+hipError_t launch_halo_copy(double* pool, long long pool_n, const void* map, long n,
+                            hipStream_t st);
+hipError_t launch_prolong(double* pool, long long pool_n, const void* pts, long n,
+                          hipStream_t st);
+hipError_t launch_prolong_blocks(double* pool, long long pool_n, const void* jobs, long n,
+                                 hipStream_t st);
 hipError_t launch_indicator(const double* pool, const BlockDesc* blk, int nblk, double* out,
                             hipStream_t st);
 hipError_t launch_batch_hold(const double* U, double* Ya, double* Yb, const BlockDesc* blk,

@@ -103,8 +103,7 @@ def indicator(block: np.ndarray, ng: int) -> np.ndarray:
 # --- hierarchy -------------------------------------------------------------------------------
 
 def parent_of(key, hier, BS):
-    L, _ = key
-    cb = hier[key][1]
+    L, cb = hier[key]
     f0 = [cb[a] // 2 for a in range(3)]
     for k, (lv, pb) in hier.items():
         if lv == L - 1 and all(pb[a] <= f0[a] < pb[a] + BS for a in range(3)):

@@ -58,6 +58,16 @@ static const int PAIR_J[6] = {0, 1, 2, 1, 2, 2};
 
 int orc_version(void) { return 1; }
 
+/* OpenMP team size of the parallel loops below (timing runs on 1 core and on all cores). */
+#ifdef _OPENMP
+#include <omp.h>
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int orc_get_threads(void) { return omp_get_max_threads(); }
+#else
+void orc_set_threads(int n) { (void)n; }
+int orc_get_threads(void) { return 1; }
+#endif
+
 /* ---------------------------------------------------------------- stencils */
 
 int orc_stencil(int kind, int order, real* c) {

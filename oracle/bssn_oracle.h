@@ -30,6 +30,8 @@ extern "C" {
 #endif
 
 int orc_version(void);
+void orc_set_threads(int n);
+int orc_get_threads(void);
 
 /* Stencil coefficients (3D common/src/mcs_common/derivatives.py:33-49). kind 0 = first
  * derivative, 1 = second derivative, 2 = Kreiss-Oliger. Writes order+1 values, returns the

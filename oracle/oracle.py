@@ -53,6 +53,26 @@ def lib():
     return _lib
 
 
+PERF_PATH = HERE / "liboracle_perf.so"
+
+
+def use_build(kind: str) -> None:
+    """Select the build the functions of this module call: "check" (liboracle.so: -O2 with no
+    FMA contraction, the build every parity test uses) or "perf" (liboracle_perf.so: the same
+    source at -O3 with AVX2/FMA, used only to time the CPU baseline)."""
+    global _lib
+    path = {"check": LIB_PATH, "perf": PERF_PATH}[kind]
+    if not path.exists():
+        build()
+    _lib = ctypes.CDLL(str(path))
+
+
+def set_threads(n: int) -> int:
+    """OpenMP team size of the current build's parallel loops; returns the size in effect."""
+    lib().orc_set_threads(int(n))
+    return int(lib().orc_get_threads())
+
+
 def have_ref() -> bool:
     return REF_PATH.exists()
 

@@ -28,7 +28,8 @@ from conftest import gpu_available, normwise
 
 from bssn3d_hip import _lib
 from bssn3d_hip import initial_data as idata
-from bssn3d_hip.amr import AMRHierarchy, prolong_entries
+from bssn3d_hip.amr import (REFINE, AMRHierarchy, RegridPolicy, octant, parent_tile,
+                            prolong_entries)
 from bssn3d_hip.evolve import BSSNEvolution
 from bssn3d_hip.grid import Grid
 from bssn3d_hip.rhs import stream_handle
@@ -46,22 +47,19 @@ def say(msg):
 
 
 def three_level(BS=8, ng=3, device=DEV, L=1.0):
+    """A hierarchy on 2x2x2 root blocks: root tile (0,0,0) fully refined, 2 children of root
+    tile (0,0,1) (one a z face neighbour of a child of the first), 3 level-2 tiles under level-1
+    tile (1,1,1)."""
     H = AMRHierarchy(nb_root=(2, 2, 2), BS=BS, ng=ng, levels=3, device=device, L=L,
                      lo=(-L / 2,) * 3)
-    T = H.topo
-    s = 0
+    T = H.tree
     for c in CORNERS:
-        T.add_block(1, s, tuple(c[a] * BS for a in range(3)), parent=(0, 0))
-        s += 1
-    p1 = T.bbox_ijk[(0, 1)]
+        T.plant((1,) + tuple(c))
     for c in ((0, 0, 0), (1, 0, 0)):
-        T.add_block(1, s, tuple(2 * p1[a] + c[a] * BS for a in range(3)), parent=(0, 1))
-        s += 1
-    k = [key for key in T.blocks(1) if T.bbox_ijk[key] == (BS, BS, BS)][0]
-    pb = T.bbox_ijk[k]
-    for i, c in enumerate(((0, 0, 0), (1, 0, 0), (1, 1, 1))):
-        T.add_block(2, i, tuple(2 * pb[a] + c[a] * BS for a in range(3)), parent=k)
-    assert T.check_proper_nesting() == []
+        T.plant((1, c[0], c[1], 2 + c[2]))
+    for c in ((0, 0, 0), (1, 0, 0), (1, 1, 1)):
+        T.plant((2, 2 + c[0], 2 + c[1], 2 + c[2]))
+    assert T.nesting_errors() == []
     return H
 
 
@@ -88,11 +86,11 @@ def smooth_fn(seed=0, amp=1e-3, L=1.0, lo=-0.5):
 
 def host_blocks(H, U):
     P = U.cpu().numpy()
-    return {k: H.view(P, *k).copy() for k in H.keys}
+    return {k: H.view(P, k).copy() for k in H.keys}
 
 
 def hier_of(H):
-    return {k: (k[0], H.topo.bbox_ijk[k]) for k in H.keys}
+    return {t: (t[0], H.origin(t)) for t in H.keys}
 
 
 @pytest.mark.parametrize("BS,ng", [(8, 3), (16, 4)])
@@ -108,7 +106,7 @@ def test_prolong_kernel_bitwise(BS, ng):
     e = np.concatenate([prolong_entries(pts, (i + 1) * P, 0, c, BS, ng)
                         for i, c in enumerate(CORNERS)])
     d = torch.from_numpy(e.view(np.uint8).copy()).to(DEV)
-    _lib.check(_lib.load().bssn_prolong_f64(pool.data_ptr(), d.data_ptr(), len(e),
+    _lib.check(_lib.load().bssn_prolong_f64(pool.data_ptr(), pool.numel(), d.data_ptr(), len(e),
                                             stream_handle(pool.device)))
     got = pool.cpu().numpy()
     for i, c in enumerate(CORNERS):
@@ -134,7 +132,8 @@ def test_prolong_blocks_bitwise(BS, ng):
     e = np.concatenate([block_job((i + 1) * P, 0, c, BS, ng, sk) for i, (c, sk) in
                         enumerate(jobs)])
     d = torch.from_numpy(e.view(np.uint8).copy()).to(DEV)
-    _lib.check(_lib.load().bssn_prolong_blocks_f64(pool.data_ptr(), d.data_ptr(), len(e),
+    _lib.check(_lib.load().bssn_prolong_blocks_f64(pool.data_ptr(), pool.numel(), d.data_ptr(),
+                                                   len(e),
                                                    stream_handle(pool.device)))
     got = pool.cpu().numpy()
     for i, (c, sk) in enumerate(jobs):
@@ -153,7 +152,7 @@ def test_indicator_bitwise():
     U = torch.from_numpy(np.random.default_rng(3).standard_normal(H.pool_doubles)).to(DEV)
     keys = H.keys
     W = H.W
-    U[H.slot_offset(*keys[4]) + 5 * W ** 3 + (5 * W + 5) * W + 5] = float("nan")   # interior
+    U[H.slot_offset(keys[4]) + 5 * W ** 3 + (5 * W + 5) * W + 5] = float("nan")    # interior
     out = torch.empty(len(keys) * 24, dtype=torch.float64, device=DEV)
     _lib.check(_lib.load().bssn_batch_indicator_f64(H.batch._handle, U.data_ptr(),
                                                     out.data_ptr(), stream_handle(U.device)))
@@ -164,7 +163,7 @@ def test_indicator_bitwise():
         assert np.array_equal(got[i], want, equal_nan=True), k
     assert np.isnan(got[4, 5]) and not np.isnan(got[4, 4])
     lv = H.indicators(U, fields=[0, 1])
-    assert lv[keys[0][0]][keys[0][1]] == max(got[0, 0], got[0, 1])
+    assert lv[keys[0]] == max(got[0, 0], got[0, 1])
 
 
 def test_sync_program_bitwise():
@@ -314,21 +313,20 @@ def test_regrid_refine_then_coarsen():
     U = H.fill(H.empty_pool(), bump_fn())
     before = host_blocks(H, U)
     ind = H.indicators(U, fields=[CHI])
-    hot = [k for k in H.keys if ind[0][k[1]] > 1e-3]
+    hot = [t for t in H.keys if ind[t] > 1e-3]
     assert len(hot) >= 1
-    U = H.regrid(U, refine_threshold=1e-3, coarsen_threshold=1e-9, hysteresis_K=1, n_buffer=0)
-    T = H.topo
-    kids = T.blocks(1)
-    assert len(kids) == 8 * len(hot) and T.check_proper_nesting() == []
+    U = H.regrid(U, RegridPolicy(refine_above=1e-3, coarsen_below=1e-9, patience=1, buffer=0))
+    T = H.tree
+    kids = T.on_level(1)
+    assert len(kids) == 8 * len(hot) and T.nesting_errors() == []
     after = host_blocks(H, U)
     I = (slice(None),) + (slice(3, 11),) * 3
     for k in kids:
-        par = T.parent[k]
-        want = OA.prolongate(before[par], T.child_corner(k), 8, 3)
+        want = OA.prolongate(before[parent_tile(k)], octant(k), 8, 3)
         assert np.array_equal(after[k][I], want[I]), k
     # every block now reads low: the children coarsen back (injection restores the parent)
-    U = H.regrid(U, refine_threshold=1e9, coarsen_threshold=1e8, hysteresis_K=1)
-    assert T.blocks(1) == [] and T.check_proper_nesting() == []
+    U = H.regrid(U, RegridPolicy(refine_above=1e9, coarsen_below=1e8, patience=1))
+    assert T.on_level(1) == [] and T.nesting_errors() == []
     final = host_blocks(H, U)
     for k in H.keys:
         assert np.array_equal(final[k][I], before[k][I]), k
@@ -340,33 +338,30 @@ def test_capacity_growth_keeps_data():
                      device=DEV)
     U = H.fill(H.empty_pool(), bump_fn(center=(0.0, 0.0, 0.0)))
     before = host_blocks(H, U)
-    flags = np.zeros((2, 64), np.int32)
-    flags[0, 0] = 1
-    U2 = H.apply_flags(U, flags)
-    assert H.topo.caps[1] >= 8 and H.topo.grow_history and U2.numel() == H.pool_doubles
+    U2 = H.apply(U, {(0, 0, 0, 0): REFINE})
+    assert H.tree.capacity(1) >= 8 and H.tree.widenings and U2.numel() == H.pool_doubles
     after = host_blocks(H, U2)
     I = (slice(None),) + (slice(3, 11),) * 3
     for k in before:
         assert np.array_equal(after[k][I], before[k][I])
-    assert len(H.topo.blocks(1)) == 8
+    assert len(H.tree.on_level(1)) == 8
 
 
-def test_evolve_with_regrid_follows_bump():
+def test_run_with_regrids_follows_bump():
     say("evolve with regrids")
     H = AMRHierarchy(nb_root=(2, 2, 2), BS=8, ng=3, levels=3, device=DEV)
     U = H.fill(H.empty_pool(), bump_fn())
     dt = 0.25 * H.spacing(2)
     seen = []
-    U, t = H.evolve_with_regrid(U, 0.0, dt, 8, 2, refine_threshold=2e-3,
-                                coarsen_threshold=1e-6, hysteresis_K=1, n_buffer=0,
-                                fields=[CHI], on_regrid=lambda n, u, h: seen.append(
-                                    [h.topo.n_active(L) for L in range(3)]),
-                                params=PhysicsParams(), order=4, ko_sigma=0.1)
+    pol = RegridPolicy(refine_above=2e-3, coarsen_below=1e-6, patience=1, buffer=0)
+    U, t = H.run(U, 0.0, dt, 8, pol, every=2, fields=[CHI],
+                 on_regrid=lambda n, u, h: seen.append([h.tree.count(L) for L in range(3)]),
+                 params=PhysicsParams(), order=4, ko_sigma=0.1)
     assert abs(t - 8 * dt) < 1e-15 and len(seen) == 3
-    assert seen[0][1] > 0 and H.topo.check_proper_nesting() == []
+    assert seen[0][1] > 0 and H.tree.nesting_errors() == []
     assert bool(torch.isfinite(U).all())
     # the bump's centre lies inside a level-1 block
     def holds(k):
         b = H.block(k)
         return all(b.lo[a] <= 0.2 < b.lo[a] + 8 * b.h[a] for a in range(3))
-    assert any(holds(k) for k in H.topo.blocks(1))
+    assert any(holds(k) for k in H.tree.on_level(1))

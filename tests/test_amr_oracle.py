@@ -104,33 +104,27 @@ def test_indicator():
 # --- hierarchies ----------------------------------------------------------------------------
 
 def three_level(BS=8, ng=3):
-    """A hierarchy on 2x2x2 root blocks: root block 0 fully refined, 2 children of root
-    block 1 (one a z face neighbour of a root-0 child), 3 level-2 blocks under one level-1
-    block."""
+    """A hierarchy on 2x2x2 root blocks: root tile (0,0,0) fully refined, 2 children of root
+    tile (0,0,1) (one a z face neighbour of a child of the first), 3 level-2 tiles under level-1
+    tile (1,1,1)."""
     H = AMRHierarchy(nb_root=(2, 2, 2), BS=BS, ng=ng, levels=3, device="cpu")
-    T = H.topo
-    s = 0
+    T = H.tree
     for c in CORNERS:
-        T.add_block(1, s, tuple(c[a] * BS for a in range(3)), parent=(0, 0))
-        s += 1
-    p1 = T.bbox_ijk[(0, 1)]
+        T.plant((1,) + tuple(c))
     for c in ((0, 0, 0), (1, 0, 0)):
-        T.add_block(1, s, tuple(2 * p1[a] + c[a] * BS for a in range(3)), parent=(0, 1))
-        s += 1
-    k = [key for key in T.blocks(1) if T.bbox_ijk[key] == (BS, BS, BS)][0]
-    pb = T.bbox_ijk[k]
-    for i, c in enumerate(((0, 0, 0), (1, 0, 0), (1, 1, 1))):
-        T.add_block(2, i, tuple(2 * pb[a] + c[a] * BS for a in range(3)), parent=k)
-    assert T.check_proper_nesting() == []
+        T.plant((1, c[0], c[1], 2 + c[2]))
+    for c in ((0, 0, 0), (1, 0, 0), (1, 1, 1)):
+        T.plant((2, 2 + c[0], 2 + c[1], 2 + c[2]))
+    assert T.nesting_errors() == []
     return H
 
 
 def _hier(H):
-    return {k: (k[0], H.topo.bbox_ijk[k]) for k in H.keys}
+    return {t: (t[0], H.origin(t)) for t in H.keys}
 
 
 def _views(H, pool):
-    return {k: H.view(pool, *k) for k in H.keys}
+    return {k: H.view(pool, k) for k in H.keys}
 
 
 def test_program_emulation_equals_oracle_sync():
@@ -140,7 +134,7 @@ def test_program_emulation_equals_oracle_sync():
                        H.BS, H.ng)
     for use_blocks in (True, False):
         P = pool.copy()
-        pr, jb, cp, pe, be, ce = build_ghost_program(H.topo, H.nb_root, H.ng, H.slot_offset,
+        pr, jb, cp, pe, be, ce = build_ghost_program(H.tree, H.BS, H.ng, H.slot_offset,
                                                      use_blocks)
         assert len(pe) == 3 and pe[0] == 0 and ce[0] > 0
         assert (len(jb) > 0) == use_blocks and (len(pr) > 0) != use_blocks
@@ -160,7 +154,7 @@ def test_program_covers_every_ghost_once():
     # copies read interior points only
     src = cp["src"]
     for k in H.keys:
-        o = H.slot_offset(*k)
+        o = H.slot_offset(k)
         m = (src >= o) & (src < o + W ** 3)
         loc = src[m] - o
         c = np.stack([loc // (W * W), (loc // W) % W, loc % W], -1)
@@ -174,14 +168,14 @@ def test_root_sync_equals_single_grid_periodic():
     G = O.periodic_sync(F, 3)
     pool = _rand(H.pool_doubles, 5)
     for k in H.keys:
-        b = H.topo.bbox_ijk[k]
-        H.view(pool, *k)[:, 3:11, 3:11, 3:11] = F[:, 3 + b[0]:11 + b[0], 3 + b[1]:11 + b[1],
-                                                 3 + b[2]:11 + b[2]]
+        b = H.origin(k)
+        H.view(pool, k)[:, 3:11, 3:11, 3:11] = F[:, 3 + b[0]:11 + b[0], 3 + b[1]:11 + b[1],
+                                                3 + b[2]:11 + b[2]]
     run_program(pool, *H.program_entries())
     for k in H.keys:
-        b = H.topo.bbox_ijk[k]
-        assert np.array_equal(H.view(pool, *k), G[:, b[0]:b[0] + 14, b[1]:b[1] + 14,
-                                                 b[2]:b[2] + 14])
+        b = H.origin(k)
+        assert np.array_equal(H.view(pool, k), G[:, b[0]:b[0] + 14, b[1]:b[1] + 14,
+                                                b[2]:b[2] + 14])
 
 
 def test_child_halo_is_prolongated_parent_interior_kept():

@@ -105,4 +105,47 @@ def test_batch_argument_errors_before_device_work():
     assert lib.bssn_batch_set_launch(None, 0) == 1
     assert lib.bssn_batch_points(None, 1) == -1
     assert lib.bssn_batch_rhs_f64(None, fake, _lib.dbl([1.0] * 16), fake, 4, 0.0, None) == 1
-    assert lib.bssn_batch_halo_f64(fake, None, 5, None) == 1
+    assert lib.bssn_batch_halo_f64(fake, 100, None, 5, None) == 1
+    assert lib.bssn_batch_halo_f64(fake, 0, fake, 5, None) == 1          # empty pool
+
+
+def test_entry_bounds_predicates():
+    """bssn_count_skipped_entries runs, on the host, the test every ghost / transfer kernel
+    applies to its entry before touching the pool: an entry that reaches outside the pool (or a
+    block job beyond the LDS passes' 24-node limit, or with an octant index other than 0/1) is
+    skipped whole."""
+    from bssn3d_hip.amr import BLOCK_DTYPE, PROLONG_DTYPE, block_job, prolong_entries
+    from bssn3d_hip.batch import HALO_DTYPE
+    lib = _lib.load()
+    count = lambda kind, e, pool: lib.bssn_count_skipped_entries(kind, e.ctypes.data, len(e), pool)
+    pool = 24 * 1000
+    h = np.zeros(6, dtype=HALO_DTYPE)
+    h["dst_stride"] = h["src_stride"] = 1000
+    h["dst"] = [0, 999, 1000, 5, -1, 5]
+    h["src"] = [999, 0, 5, 5, 5, 5]
+    h["src_stride"][5] = 1044                # field 23 of the source at 5 + 23 * 1044 > pool
+    assert count(0, h, pool) == 3            # entries 2, 4 and 5
+    assert count(0, h[:2], pool) == 0
+    # prolongation points: the parent's whole padded block must lie in the pool
+    BS, ng = 8, 3
+    W = BS + 2 * ng
+    P = 24 * W ** 3
+    pts = np.array([[0, 0, 0], [W - 1, W - 1, W - 1]])
+    good = prolong_entries(pts, P, 0, (1, 0, 1), BS, ng)
+    assert count(1, good, 2 * P) == 0
+    assert count(1, good, 2 * P - 1) == 1    # the last destination point is the pool's end
+    bad = good.copy()
+    bad["parent"] = P + 1                    # parent block runs past the pool
+    assert count(1, bad, 2 * P) == 2
+    bad = good.copy()
+    bad["pS"][0] = (0, 0, 0)
+    assert count(1, bad, 2 * P) == 1
+    # whole-block jobs
+    jobs = np.concatenate([block_job(P, 0, (1, 1, 0), BS, ng, 0),
+                           block_job(P, 0, (2, 0, 0), BS, ng, 0),      # octant index 2
+                           block_job(P, 0, (0, 0, 0), 20, 3, 0),       # W = 26 > 24
+                           block_job(P + 1, 0, (0, 0, 0), BS, ng, 0)])  # child past the end
+    assert jobs.dtype == BLOCK_DTYPE and good.dtype == PROLONG_DTYPE
+    assert count(2, jobs, 2 * P) == 3 and count(2, jobs[:1], 2 * P) == 0
+    assert lib.bssn_count_skipped_entries(3, None, 0, pool) == -1
+    assert lib.bssn_count_skipped_entries(0, None, 4, pool) == -1

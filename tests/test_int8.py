@@ -135,3 +135,30 @@ def test_composed_step_vs_fused(scheme, tol):
     e = normwise(got, want)
     print(f"{scheme}: max {e.max():.2e}")
     assert e.max() <= tol
+
+
+@pytest.mark.parametrize("k", [7, 4])
+def test_batched_int8_bundle_equals_single_grid(k):
+    """bssn_batch_deriv_bundle_i8_f64 over blocks of uneven extents (3..40 per axis, several
+    tiles and partial tiles) and three spacings: every block's 138 arrays equal
+    bssn_deriv_bundle_i8_f64 on that block alone, bit for bit."""
+    say(f"batched INT8 bundle vs single grid, {k} digits")
+    from bssn3d_hip.batch import Block, BlockBatch, fill_punctures
+    rng = np.random.default_rng(17 + k)
+    blocks = []
+    for _ in range(9):
+        ext = tuple(int(v) for v in rng.choice([3, 8, 12, 29, 40], size=3))
+        h = float(rng.choice([0.25, 0.125, 0.0625]))
+        lo = tuple(float(v) for v in rng.uniform(-2, 2, size=3))
+        blocks.append(Block(ext, (h, h, h), lo))
+    batch = BlockBatch(blocks, ng=3, device=DEV)
+    U = batch.empty_pool()
+    fill_punctures(batch, U, masses=(0.909, 0.0909), positions=[[0.31, 0.07, -0.11],
+                                                                 [-0.52, 0.13, 0.21]], seed=2)
+    D = torch.full((batch.bundle_doubles,), np.nan, dtype=torch.float64, device=DEV)
+    batch.deriv_bundle_i8(U, D, k)
+    torch.cuda.synchronize()
+    for b, blk in enumerate(blocks):
+        want = hip_bundle(batch.view(U, b).cpu().numpy(), blk.h, k)
+        got = batch.bundle_view(D, b).cpu().numpy()
+        assert np.array_equal(got, want), f"block {b} {blk}"

@@ -1,0 +1,94 @@
+"""CPU: the nested patch hierarchy of the batched block-AMR workload (bssn3d_hip.patches).
+
+* The workload geometry: 640 blocks on three levels (h = 1/4, 1/8, 1/16), every interior extent
+  from {8, 12, 16, 24, 32}, each level tiled exactly by its blocks, 2:1 nesting with a margin,
+  both punctures of the 1:10 binary inside the finest box.
+* The ghost fill program, applied with the host emulation of the device kernels
+  (tests/amr_emul.py), on a small three-level hierarchy whose interiors hold a polynomial of
+  degree 5 per axis: every ghost of the refined levels must hold the polynomial at its own
+  coordinates (copies are exact, and the 6-point midpoint prolongation reproduces degree 5
+  exactly), every root ghost the polynomial at its periodic image; every ghost point is written
+  exactly once and nothing else is written. A wrong owner, a shifted fine index, a wrong parent
+  or a missed ghost shows as an O(1) error.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from amr_emul import run_program
+
+from bssn3d_hip.patches import SIZES, config_hierarchy
+
+NF = 24
+
+
+def test_workload_geometry():
+    H, pos = config_hierarchy(seed=0, device="cpu")
+    assert len(H) == 640
+    assert [b.nblocks for b in H.boxes] == [256, 192, 192]
+    assert [b.h for b in H.boxes] == [0.25, 0.125, 0.0625]
+    for blk in H.blocks:
+        assert all(n in SIZES for n in blk.n)
+    for L, box in enumerate(H.boxes):
+        own = H.owner_map(L)                      # tiles exactly: every node owned once
+        ids = np.flatnonzero(np.asarray(H.level) == L)
+        assert set(np.unique(own)) == set(ids.tolist())
+        vol = sum(int(np.prod(H.blocks[b].n)) for b in ids)
+        assert vol == int(np.prod(box.extent))
+    fine = H.boxes[-1]
+    for q in pos:
+        n = [(q[a] - H.x0[a]) / fine.h for a in range(3)]
+        assert all(fine.origin[a] + 8 <= n[a] <= fine.origin[a] + fine.extent[a] - 8
+                   for a in range(3))
+    # blocks sit at the physical position of their lattice nodes
+    for b in (0, 300, 639):
+        L = H.level[b]
+        assert np.allclose(H.blocks[b].lo, [H.x0[a] + H.node0[b][a] * H.boxes[L].h
+                                            for a in range(3)], rtol=0, atol=1e-12)
+
+
+def _poly(x, y, z):
+    px = 0.3 + 0.2 * x - 0.05 * x ** 2 + 0.01 * x ** 3 - 0.002 * x ** 4 + 0.0003 * x ** 5
+    py = 1.1 - 0.1 * y + 0.03 * y ** 2 - 0.004 * y ** 3 + 0.0005 * y ** 4 - 0.00002 * y ** 5
+    pz = -0.7 + 0.15 * z + 0.02 * z ** 2 - 0.003 * z ** 3 + 0.0001 * z ** 4 + 0.00004 * z ** 5
+    return px * py * pz
+
+
+def test_program_reproduces_polynomials():
+    H, _ = config_hierarchy(seed=3, counts=((4, 4, 4), (3, 3, 3), (2, 2, 2)), sizes=(8, 12),
+                            device="cpu", hold_punctures=False)
+    lay = H.layout
+    ng = H.ng
+    pool = np.full(lay.pool_doubles, np.nan)
+    root = H.boxes[0]
+    period = [root.extent[a] * root.h for a in range(3)]
+    want = {}
+    for b, blk in enumerate(H.blocks):
+        g = blk.grid(ng)
+        X, Y, Z = np.meshgrid(g.x, g.y, g.z, indexing="ij")
+        if H.level[b] == 0:                       # periodic images of the root box
+            X = H.x0[0] + np.mod(X - H.x0[0], period[0])
+            Y = H.x0[1] + np.mod(Y - H.x0[1], period[1])
+            Z = H.x0[2] + np.mod(Z - H.x0[2], period[2])
+        f = _poly(X, Y, Z)
+        want[b] = f
+        v = lay.view(pool, b)
+        for k in range(NF):
+            v[k][ng:-ng, ng:-ng, ng:-ng] = (k + 1) * f[ng:-ng, ng:-ng, ng:-ng]
+    pr, jb, cp, pe, be, ce = H.program_entries()
+    dst = np.concatenate([pr["dst"], cp["dst"]])
+    assert len(np.unique(dst)) == len(dst)        # every ghost written once
+    nghost = sum(int(np.prod(lay.S[b])) - int(np.prod(blk.n)) for b, blk in enumerate(H.blocks))
+    assert len(dst) == nghost
+    assert pe[0] == 0 and len(pe) == 3 and ce[0] > 0
+    run_program(pool, pr, jb, cp, pe, be, ce)
+    assert not np.isnan(pool[np.concatenate([np.arange(lay.off[b], lay.off[b] + NF * lay.P[b])
+                                             for b in range(len(H))])]).any()
+    scale = max(np.abs(f).max() for f in want.values())
+    for b in range(len(H)):
+        v = lay.view(pool, b)
+        for k in (0, 7, 23):
+            err = np.abs(v[k] - (k + 1) * want[b]).max() / ((k + 1) * scale)
+            tol = 0.0 if H.level[b] == 0 else 1e-13
+            assert err <= tol, (b, H.level[b], k, err)
