@@ -173,6 +173,18 @@ long bssn_l2_partial_doubles(int Sx, int Sy, int Sz, int ng);
 int bssn_constraint_l2_f64(const double* F, double* HM, double* partial, int Sx, int Sy, int Sz,
                            int fd_order, const double* h3, int ng, double* out2, void* stream);
 
+/* bssn_constraint_l2_f64 for callers that cannot allocate device memory (code under graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): no workspace arguments and no
+ * allocation; HM, the partial sums and the totals live in a static device array of
+ * bssn_l2_static_capacity_doubles() doubles (10485760, 80 MiB; 4 Sx Sy Sz + 2 (Sx - 2 ng)(Sy - 2 ng) + 2
+ * must fit, e.g. 128^3 with ng = 3), which every device the library runs on holds until the
+ * process ends. Larger grids are rejected with BSSN_E_ARG. Results equal bssn_constraint_l2_f64's
+ * bit for bit. Calls on one device share the array, so calls that overlap on one device race.
+ * No counterpart in the reference: its constraint norms allocate through XLA. */
+long bssn_l2_static_capacity_doubles(void);
+int bssn_constraint_l2_static_f64(const double* F, int Sx, int Sy, int Sz, int fd_order,
+                                  const double* h3, int ng, double* out2, void* stream);
+
 /* ------------------------------------------------------------------------- block batches
  * A batch is a set of blocks of different extents and spacings stored in one pool of doubles:
  * block b is a (24, S[3b], S[3b+1], S[3b+2]) state at pool offset off[b], in the single-grid

@@ -64,6 +64,9 @@ SIGNATURES = {
     "bssn_l2_partial_doubles": (_c_long, [_c_int, _c_int, _c_int, _c_int]),
     "bssn_constraint_l2_f64": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_int, _c_int, _c_int,
                                         _c_int, _PD, _c_int, _PD, _c_void_p]),
+    "bssn_l2_static_capacity_doubles": (_c_long, []),
+    "bssn_constraint_l2_static_f64": (_c_int, [_c_void_p, _c_int, _c_int, _c_int, _c_int, _PD,
+                                               _c_int, _PD, _c_void_p]),
     # block batches
     "bssn_batch_plan_bytes": (_c_size_t, [_c_int, _PI, _c_int]),
     "bssn_batch_create": (_c_int, [ctypes.POINTER(_c_void_p), _c_void_p, _c_size_t, _c_int, _PLL,

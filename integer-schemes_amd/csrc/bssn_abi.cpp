@@ -605,6 +605,37 @@ int bssn_constraint_l2_f64(const double* F, double* HM, double* partial, int Sx,
   return BSSN_OK;
 }
 
+long bssn_l2_static_capacity_doubles(void) { return BSSN_L2_SCRATCH_DOUBLES; }
+
+int bssn_constraint_l2_static_f64(const double* F, int Sx, int Sy, int Sz, int fd_order,
+                                  const double* h3, int ng, double* out2, void* stream) {
+  if (!F || !out2) return fail(BSSN_E_ARG, "null pointer");
+  TRY(check_ng(Sx, Sy, Sz, ng));
+  const Dims d = interior_dims(Sx, Sy, Sz, ng);
+  const int nblk = sumsq_blocks(d);
+  const long P = (long)Sx * Sy * Sz;
+  if (4 * P + 2L * nblk + 2 > BSSN_L2_SCRATCH_DOUBLES)
+    return fail(BSSN_E_ARG, "grid too large for the static workspace: 4 Sx Sy Sz + 2 (Sx - 2 ng)"
+                            " (Sy - 2 ng) + 2 must not exceed 10485760 doubles");
+  double* ws = nullptr;
+  TRY(check_hip(l2_scratch(&ws), "static workspace address"));
+  double* HM = ws;
+  double* partial = ws + 4 * P;
+  double* total = partial + 2L * nblk;
+  TRY(bssn_constraints_f64(F, HM, Sx, Sy, Sz, fd_order, h3, stream));
+  hipStream_t st = (hipStream_t)stream;
+  TRY(check_hip(launch_sumsq(HM, d, partial, nblk, st), "sumsq launch"));
+  TRY(check_hip(launch_sum_partials(partial, nblk, total, st), "sum launch"));
+  double host[2];
+  TRY(check_hip(hipMemcpyAsync(host, total, sizeof(host), hipMemcpyDeviceToHost, st),
+                "copy sums"));
+  TRY(check_hip(hipStreamSynchronize(st), "stream synchronize"));
+  const double cnt = (double)d.n[0] * d.n[1] * d.n[2];
+  out2[0] = std::sqrt(host[0] / cnt);
+  out2[1] = std::sqrt(host[1] / cnt);
+  return BSSN_OK;
+}
+
 }  // extern "C"
 
 // ================================================================================ batches

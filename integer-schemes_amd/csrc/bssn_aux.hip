@@ -13,6 +13,7 @@
 
 #include <math.h>
 
+#include <mutex>
 #include <type_traits>
 
 #include "bssn_algebra.h"
@@ -513,6 +514,51 @@ hipError_t launch_sumsq(const double* HM, const Dims& d, double* partial, int nb
                         hipStream_t st) {
   hipLaunchKernelGGL(k_sumsq, dim3((unsigned)nblk), dim3(256), 0, st, HM, d, partial);
   return hipGetLastError();
+}
+
+// total[0..1] = the partial sums added in row order from 0.0, by one thread: the same sums, in
+// the same order, as the host loop of bssn_constraint_l2_f64, so both entry points agree bit
+// for bit
+__global__ void k_sum_partials(const double* __restrict__ partial, int nblk,
+                               double* __restrict__ total) {
+  if (threadIdx.x != 0) return;
+  double sh = 0.0, sm = 0.0;
+  for (int b = 0; b < nblk; ++b) {
+    sh += partial[2 * b];
+    sm += partial[2 * b + 1];
+  }
+  total[0] = sh;
+  total[1] = sm;
+}
+
+hipError_t launch_sum_partials(const double* partial, int nblk, double* total, hipStream_t st) {
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, st, partial, nblk, total);
+  return hipGetLastError();
+}
+
+// Static workspace of bssn_constraint_l2_static_f64 (for callers that cannot allocate): the
+// four constraint arrays, the row partial sums and the two totals, BSSN_L2_SCRATCH_DOUBLES
+// doubles (80 MiB) per device, held by every device the library runs on until the process
+// ends. Host code uses the address hipGetSymbolAddress returns for the current device, never
+// the array's name.
+static __device__ __align__(256) double scratch[BSSN_L2_SCRATCH_DOUBLES];
+
+hipError_t l2_scratch(double** out) {
+  static std::mutex mu;
+  static double* addr[64] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    addr[dev] = (double*)p;
+  }
+  *out = addr[dev];
+  return hipSuccess;
 }
 
 }  // namespace bssn

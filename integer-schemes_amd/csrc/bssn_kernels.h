@@ -178,5 +178,9 @@ hipError_t launch_constraints(const double* F, double* HM, const Dims& d, const 
 hipError_t launch_sumsq(const double* HM, const Dims& d, double* partial, int nblk,
                         hipStream_t st);
 int sumsq_blocks(const Dims& d);
+hipError_t launch_sum_partials(const double* partial, int nblk, double* total, hipStream_t st);
+// doubles of the static workspace of bssn_constraint_l2_static_f64 (80 MiB)
+#define BSSN_L2_SCRATCH_DOUBLES (10L * 1024 * 1024)
+hipError_t l2_scratch(double** out);
 
 }  // namespace bssn
