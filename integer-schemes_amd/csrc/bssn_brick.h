@@ -8,7 +8,8 @@
 // Only the cells a stencil can reach are staged: the core, the six face slabs to depth H
 // (first/second derivatives and Kreiss-Oliger along one axis) and the twelve edge bars to depth
 // R in two axes (mixed derivatives). For FD order 4 with KO order 6 that is 544 of the 1000
-// cells of the box, 8.5 loads per point and field.
+// cells of the box, 8.5 loads per point and field; the 13 fields without mixed derivatives
+// (Gt^i, At_ij, K, B^i) skip the edge bars: 352 cells, 5.5 loads per point.
 //
 // Halo positions are mapped to the grid once per lane (not per field):
 //   CLAMP  x -> min(max(x, 0), S-1) per axis. The reference edge-pads each one-axis operator by
@@ -80,13 +81,24 @@ struct Geo {
   static constexpr int NEDGE = 12 * B * R * R;
   static constexpr int NCELL = NCORE + NFACE + NEDGE;
   static constexpr int PER = (NCELL + NT - 1) / NT;       // staged cells per lane
+  // fields without mixed derivatives need the core and the face slabs only: the first
+  // NCORE + NFACE cells of the table
+  static constexpr int PER_FACES = (NCORE + NFACE + NT - 1) / NT;
 };
 
+// The fields whose second derivatives the RHS, the bundle and the constraints form (so whose
+// mixed derivatives read the edge bars of the box): alpha, chi, gt_ij, beta^i. The others are
+// staged over the core and the face slabs only (Kreiss-Oliger and the one-axis stencils).
+__host__ __device__ constexpr bool needs_edges(int f) {
+  return f == ALPHA || f == CHI || (f >= GT0 && f <= GT5) || (f >= BETA0 && f <= BETA2);
+}
+
 // The staged cells, ordered row by row along z so that the lanes of one load instruction
-// cover contiguous runs of a grid row: the 16 core rows (z over the whole box), the x- and
-// y-face rows (z over the core and, within reach R of the core, the z edge bars) and the xy edge
-// rows (z over the core). Entry = packed box coordinates x | y << 5 | z << 10; slots past the
-// cell count repeat the first core cell.
+// cover contiguous runs of a grid row: the 16 core rows (z over the whole box) and the x- and
+// y-face rows (z over the core) -- the NCORE + NFACE cells every field needs -- then the edge
+// bars of the mixed derivatives: the z extensions of the face rows within reach R of the core
+// and the xy edge rows (z over the core). Entry = packed box coordinates x | y << 5 | z << 10;
+// slots past the cell count repeat the first core cell.
 template <class G>
 struct CellTable {
   short c[G::PER * G::NT];
@@ -98,15 +110,21 @@ struct CellTable {
     };
     for (int x = H; x < H + B; ++x)
       for (int y = H; y < H + B; ++y) row(x, y, 0, N);
-    for (int ax = 0; ax < 2; ++ax)
-      for (int side = 0; side < 2; ++side)
-        for (int d = 1; d <= H; ++d)
-          for (int u = H; u < H + B; ++u) {
-            const int w = side == 0 ? H - d : H + B - 1 + d;
-            const int z0 = d <= R ? H - R : H, z1 = d <= R ? H + B + R : H + B;
-            if (ax == 0) row(w, u, z0, z1);
-            else row(u, w, z0, z1);
-          }
+    // face rows over the core, then (second pass) their z extensions within reach R
+    for (int pass = 0; pass < 2; ++pass)
+      for (int ax = 0; ax < 2; ++ax)
+        for (int side = 0; side < 2; ++side)
+          for (int d = 1; d <= H; ++d)
+            for (int u = H; u < H + B; ++u) {
+              const int w = side == 0 ? H - d : H + B - 1 + d;
+              const int x = ax == 0 ? w : u, y = ax == 0 ? u : w;
+              if (pass == 0) {
+                row(x, y, H, H + B);
+              } else if (d <= R) {
+                row(x, y, H - R, H);
+                row(x, y, H + B, H + B + R);
+              }
+            }
     for (int sx = 0; sx < 2; ++sx)
       for (int sy = 0; sy < 2; ++sy)
         for (int dx = 1; dx <= R; ++dx)
@@ -286,18 +304,27 @@ struct WaveSrc : BoxStencil<Geo<ORD, KO>, Coef<ORD>> {
       if (i < ORDER::N) fetch(i, ORDER::at(i));
   }
 
+  // staged slots of field f: the edge bars only for fields with mixed derivatives
+  __device__ __forceinline__ static int slots(int f) {
+    return needs_edges(f) ? G::PER : G::PER_FACES;
+  }
+
   __device__ __forceinline__ void fetch(int slot, int f) {
     const double* u = F + (long)f * P;
+    const int n = slots(f);
 #pragma unroll
-    for (int k = 0; k < G::PER; ++k) pf[slot][k] = u[off[k]];
+    for (int k = 0; k < G::PER; ++k)
+      if (k < n) pf[slot][k] = u[off[k]];
     // keep the loads here, ahead of the work they overlap
     __builtin_amdgcn_sched_barrier(0);
   }
 
   __device__ __forceinline__ void stage(int f) {
     const int at = ORDER::pos(f), slot = DEPTH == 1 ? 0 : at % DEPTH;
+    const int n = slots(f);
 #pragma unroll
-    for (int k = 0; k < G::PER; ++k) lds[cel[k]] = pf[slot][k];
+    for (int k = 0; k < G::PER; ++k)
+      if (k < n) lds[cel[k]] = pf[slot][k];
     // The wave's LDS accesses complete in issue order; this only stops the compiler from
     // moving this field's reads above the writes.
     __builtin_amdgcn_wave_barrier();
@@ -310,8 +337,10 @@ struct WaveSrc : BoxStencil<Geo<ORD, KO>, Coef<ORD>> {
   template <int SLOT>
   __device__ __forceinline__ void stage_slot(int f) {
     const int at = ORDER::pos(f);
+    const int n = slots(f);
 #pragma unroll
-    for (int k = 0; k < G::PER; ++k) lds[cel[k]] = pf[SLOT][k];
+    for (int k = 0; k < G::PER; ++k)
+      if (k < n) lds[cel[k]] = pf[SLOT][k];
     __builtin_amdgcn_wave_barrier();
     if (at + DEPTH < ORDER::N) fetch(SLOT, ORDER::at(at + DEPTH));
   }
