@@ -153,6 +153,18 @@ def cpu_baseline(seconds: float = 12.0):
             "-march=x86-64-v3, FMA contraction on)", "cpu_model": _cpu_model(), "runs": runs}
 
 
+def schedule() -> dict:
+    """The stage kernels' schedule knobs the library was built with: the autotuner's choice
+    (integer-schemes_amd/tuning.json) or the built-in defaults."""
+    path = ROOT / "integer-schemes_amd" / "tuning.json"
+    if not path.exists():
+        return {"defines": [], "source": "built-in defaults"}
+    t = json.loads(path.read_text())
+    return {"defines": t.get("defines", []), "source": "integer-schemes_amd/tuning.json",
+            "tuned_on": t.get("device"), "tuned_step_ms": t.get("candidates", {}).get(
+                t.get("best", ""), {}).get("step_ms")}
+
+
 def measured_traffic(workload: str):
     """HBM bytes per stage launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes
     (each scaled by the same counter on a 1 GiB elementwise copy), or None for another
@@ -715,6 +727,7 @@ def main() -> int:
         "sommerfeld_step_mpts_s": round(npts / (ms_sommerfeld * 1e-3) / 1e6, 2),
         "sommerfeld_finite": sommerfeld_finite,
         "roofline": roof,
+        "schedule": schedule(),
         "constraints": {"order": 4, "H_l2_initial": float("%.6e" % h0),
                         "M_l2_initial": float("%.6e" % m0),
                         "H_l2_final": float("%.6e" % h1), "M_l2_final": float("%.6e" % m1),

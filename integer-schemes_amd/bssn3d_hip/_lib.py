@@ -8,10 +8,13 @@ package runs the HIP kernels or fails.
 from __future__ import annotations
 
 import ctypes
+import os
 import threading
 from pathlib import Path
 
-LIB_PATH = Path(__file__).resolve().parent / "libbssn_hip.so"
+# BSSN_HIP_LIB names another build of the same library (a variant the autotuner times)
+LIB_PATH = Path(os.environ.get("BSSN_HIP_LIB") or
+                Path(__file__).resolve().parent / "libbssn_hip.so")
 BUILD_HINT = "python integer-schemes_amd/build.py"
 ABI_VERSION = 1
 

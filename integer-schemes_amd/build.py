@@ -4,7 +4,9 @@
 
 Each HIP source is compiled separately (in parallel) for --offload-arch=gfx950 and linked into
 integer-schemes_amd/bssn3d_hip/libbssn_hip.so, which the Python host layer loads.
-``build(out=..., defines=[...])`` builds a variant with extra -D knobs into another path.
+``build(out=..., defines=[...])`` builds a variant with extra -D knobs into another path; the
+installed library is built with the schedule knobs of tuning.json when the autotuner
+(autotune.py) has written one.
 """
 from __future__ import annotations
 
@@ -51,7 +53,22 @@ def _compile(src: str, objdir: Path, defines) -> Path:
     return obj
 
 
-def build(verbose: bool = False, out: Path | None = None, defines=()) -> Path:
+TUNING = PKG / "tuning.json"
+
+
+def tuned_defines() -> list:
+    """The schedule knobs the autotuner chose on the device (tuning.json), [] without one."""
+    if not TUNING.exists():
+        return []
+    import json
+    return list(json.loads(TUNING.read_text()).get("defines", []))
+
+
+def build(verbose: bool = False, out: Path | None = None, defines=None) -> Path:
+    """Compile every source and link the library. defines: -D knobs; None = the tuned ones
+    (tuning.json) for the installed library, none for a variant build."""
+    if defines is None:
+        defines = tuned_defines() if out is None else []
     out = Path(out) if out is not None else OUT
     objdir = PKG / "_build" / (out.parent.name if out != OUT else "lib")
     objdir.mkdir(parents=True, exist_ok=True)
@@ -65,7 +82,7 @@ def build(verbose: bool = False, out: Path | None = None, defines=()) -> Path:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, out)
     if verbose:
-        print(f"built {out}")
+        print(f"built {out}" + (f" with {' '.join(defines)}" if defines else ""))
     return out
 
 

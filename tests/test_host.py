@@ -4,6 +4,7 @@ conventions (scalar pack, output order, grid, initial data, error behaviour)."""
 from __future__ import annotations
 
 import ctypes
+import json
 import math
 import re
 
@@ -178,3 +179,35 @@ def test_gauge_wave_invariants():
         for a in range(3):
             gt = np.abs(d[18 + a]).max()
             assert (gt > 0.0) if a == axis else (gt == 0.0)
+
+
+def _load_module(name, path):
+    import importlib.util
+    spec = importlib.util.spec_from_file_location(name, path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def test_autotune_space_and_installed_choice():
+    """The autotuner's knobs are the ones bssn_stage.h reads (#ifndef NAME), candidate keys are
+    unique, and the installed tuning.json names a candidate of the full space that passed its
+    check; build.tuned_defines() hands exactly those -D knobs to the installed build."""
+    pkg = ROOT / "integer-schemes_amd"
+    at = _load_module("bssn_autotune", pkg / "autotune.py")
+    bd = _load_module("bssn_build", pkg / "build.py")
+    stage_h = (pkg / "csrc" / "bssn_stage.h").read_text()
+    for space in at.SPACES.values():
+        keys = [at.key_of(d) for d in space]
+        assert len(set(keys)) == len(keys) and keys[0] == "default"
+        for d in space:
+            for name, v in d.items():
+                assert f"#ifndef {name}" in stage_h, name
+                assert isinstance(v, int) and v >= 1
+    t = json.loads((pkg / "tuning.json").read_text())
+    full = {at.key_of(d): d for d in at.SPACES["full"]}
+    assert t["best"] in full and t["candidates"][t["best"]]["passed"]
+    assert t["defines"] == [f"{a}={b}" for a, b in full[t["best"]].items()]
+    assert all(t["candidates"][t["best"]]["step_ms"] <= r["step_ms"]
+               for r in t["candidates"].values() if r["passed"])
+    assert bd.tuned_defines() == t["defines"]
