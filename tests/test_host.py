@@ -208,6 +208,16 @@ def test_autotune_space_and_installed_choice():
     full = {at.key_of(d): d for d in at.SPACES["full"]}
     assert t["best"] in full and t["candidates"][t["best"]]["passed"]
     assert t["defines"] == [f"{a}={b}" for a, b in full[t["best"]].items()]
-    assert all(t["candidates"][t["best"]]["step_ms"] <= r["step_ms"]
-               for r in t["candidates"].values() if r["passed"])
+    # the installed record is the selection rule applied to its own measurements
+    assert at.choose(t["candidates"], margin=t["margin"])["best"] == t["best"]
     assert bd.tuned_defines() == t["defines"]
+    # the rule: beat the defaults' step by more than the margin, without a slower bare RHS
+    def cand(step, rhs, passed=True, d=None):
+        return {"step_ms": step, "rhs_ms": rhs, "passed": passed, "defines": d or {}}
+    base = cand(40.0, 6.0)
+    r = {"default": base, "a": cand(39.9, 6.0, d={"BSSN_WB_LEAD": 3}),
+         "b": cand(38.0, 6.5, d={"BSSN_RHS_DEPTH": 1}), "c": cand(37.0, 6.0, passed=False)}
+    assert at.choose(r)["best"] == "default"        # a: noise, b: slower RHS, c: failed
+    r["d"] = cand(39.0, 6.01, d={"BSSN_STAGE_DEPTH": 2})
+    rec = at.choose(r)
+    assert rec["best"] == "d" and rec["defines"] == ["BSSN_STAGE_DEPTH=2"]
