@@ -479,10 +479,11 @@ def bench_bfp(N: int, reps: int) -> dict:
     return out
 
 
-def bench_fd6(sizes, reps: int) -> dict:
-    """FD order 6 with KO order 8 (4 ghosts), the reference's production pairing and the
-    configuration of its published fused-kernel figure (gen_fused.py:63-66 at 128^3): the same
-    equal-mass binary construction at 128^3 and 256^3 (h = 16/N), RK4 step and bare RHS."""
+def bench_order(order: int, sizes, reps: int) -> dict:
+    """FD order 6 or 8 with KO order 8 (4 ghosts): order 6 is the reference's production pairing
+    and the configuration of its published fused-kernel figure (gen_fused.py:63-66 at 128^3);
+    order 8 its highest. The same equal-mass binary construction (h = 16/N), RK4 step and bare
+    RHS."""
     import torch
     from bssn3d_hip import initial_data as idata
     from bssn3d_hip.evolve import BSSNEvolution
@@ -495,7 +496,7 @@ def bench_fd6(sizes, reps: int) -> dict:
     for N in sizes:
         grid, U, _ = idata.config_binary(seed=0, N=N, device=dev, ng=4)
         dt = 0.25 * grid.dx
-        evo = BSSNEvolution(grid, p, order=6, ko_sigma=0.1, bc="periodic", integrator="rk4",
+        evo = BSSNEvolution(grid, p, order=order, ko_sigma=0.1, bc="periodic", integrator="rk4",
                             ko_order=8)
         t = 0.0
         for _ in range(2):
@@ -513,10 +514,10 @@ def bench_fd6(sizes, reps: int) -> dict:
         finite = bool(torch.isfinite(U).all())
         S16 = scalars(p, grid.dx, grid.dy, grid.dz, dt, t)
         o = torch.empty_like(U)
-        rhs_into(U, o, S16, 6)
+        rhs_into(U, o, S16, order)
         a.record()
         for _ in range(reps):
-            rhs_into(U, o, S16, 6)
+            rhs_into(U, o, S16, order)
         b.record()
         torch.cuda.synchronize()
         ms_rhs = a.elapsed_time(b) / reps
@@ -528,7 +529,7 @@ def bench_fd6(sizes, reps: int) -> dict:
                          "finite": finite}
         del U, o, evo
         torch.cuda.empty_cache()
-    out["workload"] = "bl_binary_fd6_ko8_ng4_h16/N"
+    out["workload"] = f"bl_binary_fd{order}_ko8_ng4_h16/N"
     return out
 
 
@@ -740,7 +741,8 @@ def main() -> int:
     del out, U
     torch.cuda.empty_cache()
     if not args.no_fd6:
-        res["fd6_ko8"] = bench_fd6((128, N), reps)
+        res["fd6_ko8"] = bench_order(6, (128, N), reps)
+        res["fd8_ko8"] = bench_order(8, (128,), reps)
         torch.cuda.empty_cache()
     if not args.no_int8:
         res["int8_split"] = bench_int8(N, reps)

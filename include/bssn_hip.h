@@ -18,8 +18,8 @@
  *   D      derivative bundle, 138 arrays: d = 3*f + axis (first derivatives of all 24 fields),
  *          d = 72 + 6*s + pair (second derivatives; s over alpha, chi, gt0..5, beta0..2;
  *          pair over 00,01,02,11,12,22; mixed = d1_j(d1_i f)).
- * fd_order is 4 or 6; the Kreiss-Oliger order is fd_order + 2 (the reference's pairing,
- * common/src/mcs_common/derivatives.py:55-72).
+ * fd_order is 4, 6 or 8; the Kreiss-Oliger order is fd_order + 2, at most 8 (the reference's
+ * pairing and its widest KO stencil, common/src/mcs_common/derivatives.py:45-72): 6, 8, 8.
  */
 #ifndef BSSN_HIP_H
 #define BSSN_HIP_H

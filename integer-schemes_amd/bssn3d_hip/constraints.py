@@ -12,8 +12,8 @@ from .rhs import as_tensor, stream_handle
 
 class ConstraintSolver:
     def __init__(self, grid: Grid, order: int = 6):
-        if order not in (4, 6):
-            raise ValueError(f"order {order} not supported on the device (4 or 6)")
+        if order not in (4, 6, 8):
+            raise ValueError(f"order {order} not supported on the device (4, 6 or 8)")
         ng = ghost_width(order)
         if ng > grid.ng:
             raise ValueError(f"order {order} needs ng>={ng}, grid ng={grid.ng}")

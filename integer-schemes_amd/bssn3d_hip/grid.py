@@ -20,6 +20,13 @@ def ghost_width(order: int, ko_order: int | None = None) -> int:
     return max(order, ko_order) // 2
 
 
+def default_ko_order(order: int) -> int:
+    """The Kreiss-Oliger order the device pairs with FD order ``order``: order + 2 (the
+    reference's production rule, common/src/mcs_common/derivatives.py:55-58), at most 8, its
+    widest KO stencil (derivatives.py:45-49): 4 -> 6, 6 -> 8, 8 -> 8."""
+    return min(order + 2, 8)
+
+
 @dataclass
 class Grid:
     Nx: int

@@ -29,7 +29,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .grid import Grid, ghost_width
+from .grid import Grid, default_ko_order, ghost_width
 from .state import BSSNState, PhysicsParams, VAR_NAMES, pack_S, scalars
 
 SCHEMES = ("fused", "twopass", "int8", "bfp48", "bfp40", "bfp32", "bfp24")
@@ -193,13 +193,14 @@ class BSSNSolver:
             raise ValueError(f"unknown scheme {scheme!r} (use one of {SCHEMES})")
         if precision != "fp64":
             raise ValueError(f"unknown precision {precision!r} (this library computes in fp64)")
-        if order not in (4, 6):
-            raise ValueError(f"order {order} not supported on the device (4 or 6)")
+        if order not in (4, 6, 8):
+            raise ValueError(f"order {order} not supported on the device (4, 6 or 8)")
         if scheme == "int8" and order != 4:
             raise ValueError("the int8 scheme supports FD order 4 only")
-        ko_order = order + 2 if ko_order is None else ko_order
-        if ko_order != order + 2:
-            raise ValueError(f"KO order must be order + 2 = {order + 2} on the device")
+        paired = default_ko_order(order)
+        ko_order = paired if ko_order is None else ko_order
+        if ko_order != paired:
+            raise ValueError(f"KO order must be {paired} with FD order {order} on the device")
         ng = ghost_width(order, ko_order)
         if ng > grid.ng:
             raise ValueError(f"FD order {order} / KO order {ko_order} need ng>={ng} "

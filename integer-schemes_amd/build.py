@@ -24,8 +24,9 @@ OUT = PKG / "bssn3d_hip" / "libbssn_hip.so"
 ARCH = os.environ.get("BSSN_OFFLOAD_ARCH", "gfx950")
 
 # largest compile units first (the fused stage kernels are instantiated per kind)
-SOURCES = ["bssn_fused4.hip", "bssn_fused6.hip", "bssn_batch4.hip", "bssn_batch6.hip",
-           "bssn_fused4s.hip", "bssn_fused6s.hip",
+SOURCES = ["bssn_fused4.hip", "bssn_fused6.hip", "bssn_fused8.hip", "bssn_batch4.hip",
+           "bssn_batch6.hip", "bssn_batch8.hip", "bssn_fused4s.hip", "bssn_fused6s.hip",
+           "bssn_fused8s.hip",
            "bssn_aux.hip", "bssn_batch.hip", "bssn_i8.hip", "bssn_bc.hip", "bssn_amr.hip",
            "bssn_abi.cpp"]
 

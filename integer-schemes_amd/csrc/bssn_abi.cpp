@@ -31,7 +31,9 @@ int check_hip(hipError_t e, const char* what) {
   return fail(BSSN_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-bool valid_order(int ord) { return ord == 4 || ord == 6; }
+bool valid_order(int ord) { return ord == 4 || ord == 6 || ord == 8; }
+// Kreiss-Oliger order paired with an FD order: ord + 2, at most 8 (the reference's widest)
+int ko_order_of(int ord) { return ord + 2 < 8 ? ord + 2 : 8; }
 
 // S16 = eta, C_CAHD, dt, dx_i, h_ssl, sig_ssl, t, lambda0..3, lambda_f0..1, dx, dy, dz
 Scal make_scal(const double* S, double dt, double t) {
@@ -144,7 +146,7 @@ int rk4_step(double* U, double* work, int Sx, int Sy, int Sz, const double* S16,
              const double* Sdev, int fd_order, double ko_sigma, int ng, int periodic, double t,
              const double* tdev, double dt, double chi_floor, double alpha_floor, void* stream) {
   if (!U || !work) return fail(BSSN_E_ARG, "null pointer");
-  if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4 or 6");
+  if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4, 6 or 8");
   if (!(ko_sigma >= 0.0)) return fail(BSSN_E_ARG, "ko_sigma must be >= 0");
   if (periodic < 0 || periodic > 2) return fail(BSSN_E_ARG, "periodic must be 0, 1 or 2");
   if (!Sdev) {
@@ -214,7 +216,7 @@ const char* bssn_last_error(void) { return g_err.c_str(); }
 int bssn_rhs_f64(const double* F, const double* S16, double* OUT, int Sx, int Sy, int Sz,
                  int fd_order, double ko_sigma, void* stream) {
   if (!F || !OUT) return fail(BSSN_E_ARG, "null state pointer");
-  if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4 or 6");
+  if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4, 6 or 8");
   if (!(ko_sigma >= 0.0)) return fail(BSSN_E_ARG, "ko_sigma must be >= 0");
   TRY(check_shape(Sx, Sy, Sz));
   TRY(check_s16(S16));
@@ -232,7 +234,7 @@ int bssn_rhs_f64(const double* F, const double* S16, double* OUT, int Sx, int Sy
 int bssn_rhs_sdev_f64(const double* F, const double* S16dev, double* OUT, int Sx, int Sy, int Sz,
                       int fd_order, double ko_sigma, void* stream) {
   if (!F || !OUT || !S16dev) return fail(BSSN_E_ARG, "null pointer");
-  if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4 or 6");
+  if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4, 6 or 8");
   if (!(ko_sigma >= 0.0)) return fail(BSSN_E_ARG, "ko_sigma must be >= 0");
   TRY(check_shape(Sx, Sy, Sz));
   const double h3[3] = {1.0, 1.0, 1.0};
@@ -253,7 +255,7 @@ int bssn_rhs_sdev_f64(const double* F, const double* S16dev, double* OUT, int Sx
 int bssn_stage_rhs_f64(const double* Y, double* OUT, int Sx, int Sy, int Sz, const double* S16,
                        int fd_order, double ko_sigma, int ng, int periodic, void* stream) {
   if (!Y || !OUT) return fail(BSSN_E_ARG, "null state pointer");
-  if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4 or 6");
+  if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4, 6 or 8");
   if (!(ko_sigma >= 0.0)) return fail(BSSN_E_ARG, "ko_sigma must be >= 0");
   TRY(check_shape(Sx, Sy, Sz));
   TRY(check_s16(S16));
@@ -273,7 +275,7 @@ int bssn_stage_rhs_f64(const double* Y, double* OUT, int Sx, int Sy, int Sz, con
 int bssn_deriv_bundle_f64(const double* F, double* D, int Sx, int Sy, int Sz, int fd_order,
                           const double* h3, void* stream) {
   if (!F || !D) return fail(BSSN_E_ARG, "null pointer");
-  if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4 or 6");
+  if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4, 6 or 8");
   TRY(check_shape(Sx, Sy, Sz));
   TRY(check_h(h3));
   const Scales w = make_scales(h3, fd_order, 0.0);
@@ -306,7 +308,7 @@ size_t bssn_bfp_bytes(int mant_bits, long long npts) {
 int bssn_deriv_bundle_absmax_f64(const double* F, double* amax, int Sx, int Sy, int Sz,
                                  int fd_order, const double* h3, void* stream) {
   if (!F || !amax) return fail(BSSN_E_ARG, "null pointer");
-  if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4 or 6");
+  if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4, 6 or 8");
   TRY(check_shape(Sx, Sy, Sz));
   TRY(check_h(h3));
   const Scales w = make_scales(h3, fd_order, 0.0);
@@ -331,7 +333,7 @@ static int check_bfp(const int* exps, int mant_bits, BundleOut* o, BfpScales* sc
 int bssn_deriv_bundle_bfp(const double* F, void* Q, const int* exps, int mant_bits, int Sx,
                           int Sy, int Sz, int fd_order, const double* h3, void* stream) {
   if (!F || !Q) return fail(BSSN_E_ARG, "null pointer");
-  if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4 or 6");
+  if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4, 6 or 8");
   TRY(check_shape(Sx, Sy, Sz));
   TRY(check_h(h3));
   BundleOut o{};
@@ -367,7 +369,7 @@ int bssn_algebra_f64(const double* F, const double* D, const double* S16, double
 int bssn_ko_add_f64(const double* F, double* OUT, int Sx, int Sy, int Sz, int fd_order,
                     double sigma, const double* h3, int periodic, int ng, void* stream) {
   if (!F || !OUT) return fail(BSSN_E_ARG, "null pointer");
-  if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4 or 6");
+  if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4, 6 or 8");
   TRY(check_shape(Sx, Sy, Sz));
   TRY(check_h(h3));
   if (periodic) TRY(check_ng(Sx, Sy, Sz, ng));
@@ -430,7 +432,7 @@ int bssn_rk4_step_sdev_f64(double* U, double* work, int Sx, int Sy, int Sz, doub
 int bssn_sommerfeld_rhs_f64(const double* F, double* OUT, int Sx, int Sy, int Sz, int fd_order,
                             const double* h3, int ng, const double* xmin3, void* stream) {
   if (!F || !OUT || !xmin3) return fail(BSSN_E_ARG, "null pointer");
-  if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4 or 6");
+  if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4, 6 or 8");
   TRY(check_shape(Sx, Sy, Sz));
   TRY(check_h(h3));
   TRY(check_ng(Sx, Sy, Sz, ng));
@@ -455,14 +457,14 @@ int bssn_rk4_step_sommerfeld_f64(double* U, double* work, int Sx, int Sy, int Sz
                                  const double* xmin3, double t, double dt, double chi_floor,
                                  double alpha_floor, void* stream) {
   if (!U || !work || !xmin3) return fail(BSSN_E_ARG, "null pointer");
-  if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4 or 6");
+  if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4, 6 or 8");
   if (!(ko_sigma >= 0.0)) return fail(BSSN_E_ARG, "ko_sigma must be >= 0");
   if (!(dt != 0.0) || !std::isfinite(dt) || !std::isfinite(t))
     return fail(BSSN_E_ARG, "dt must be finite and nonzero, t finite");
   TRY(check_shape(Sx, Sy, Sz));
   TRY(check_s16(S16));
   TRY(check_ng(Sx, Sy, Sz, ng));
-  const int reach = ko_sigma > 0.0 ? fd_order / 2 + 1 : fd_order / 2;
+  const int reach = ko_sigma > 0.0 ? ko_order_of(fd_order) / 2 : fd_order / 2;
   if (ng < reach) return fail(BSSN_E_ARG, "ng must cover the reach of the interior stencils");
   for (int q = 0; q < 3; ++q)
     if (!std::isfinite(xmin3[q])) return fail(BSSN_E_ARG, "non-finite grid origin");
@@ -514,7 +516,7 @@ int bssn_msrk_step_f64(double* U, double* fprev, double* fnext, double* work, in
                        void* stream) {
   if (!U || !fprev || !fnext || !work) return fail(BSSN_E_ARG, "null pointer");
   if (fprev == fnext) return fail(BSSN_E_ARG, "fprev and fnext must be distinct buffers");
-  if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4 or 6");
+  if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4, 6 or 8");
   if (!(ko_sigma >= 0.0)) return fail(BSSN_E_ARG, "ko_sigma must be >= 0");
   if (periodic < 0 || periodic > 2) return fail(BSSN_E_ARG, "periodic must be 0, 1 or 2");
   if (!(dt != 0.0) || !std::isfinite(dt) || !std::isfinite(t))
@@ -565,7 +567,7 @@ int bssn_msrk_step_f64(double* U, double* fprev, double* fnext, double* work, in
 int bssn_constraints_f64(const double* F, double* HM, int Sx, int Sy, int Sz, int fd_order,
                          const double* h3, void* stream) {
   if (!F || !HM) return fail(BSSN_E_ARG, "null pointer");
-  if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4 or 6");
+  if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4, 6 or 8");
   TRY(check_shape(Sx, Sy, Sz));
   TRY(check_h(h3));
   const Scales w = make_scales(h3, fd_order, 0.0);
@@ -919,7 +921,7 @@ int bssn_batch_rhs_f64(const bssn_batch* b, const double* F, const double* S16, 
                        int fd_order, double ko_sigma, void* stream) {
   TRY(check_batch(b));
   if (!F || !OUT) return fail(BSSN_E_ARG, "null pool pointer");
-  if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4 or 6");
+  if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4, 6 or 8");
   if (!(ko_sigma >= 0.0)) return fail(BSSN_E_ARG, "ko_sigma must be >= 0");
   TRY(check_s16_batch(S16));
   StageArgs s{};
@@ -993,7 +995,7 @@ int batch_rk4(const bssn_batch* b, double* U, double* work, const double* S16, i
               double chi_floor, double alpha_floor, double* tdev, hipStream_t st) {
   TRY(check_batch(b));
   if (!U || !work) return fail(BSSN_E_ARG, "null pointer");
-  if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4 or 6");
+  if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4, 6 or 8");
   if (!(ko_sigma >= 0.0)) return fail(BSSN_E_ARG, "ko_sigma must be >= 0");
   if (!(dt != 0.0) || !std::isfinite(dt) || !std::isfinite(t))
     return fail(BSSN_E_ARG, "dt must be finite and nonzero, t finite");

@@ -120,12 +120,11 @@ __global__ __launch_bounds__(64) void k_deriv_bundle(const double* __restrict__ 
 template <int STORE>
 static void launch_bundle_store(const double* F, const BundleOut& o, const Dims& d,
                                 const Scales& w, int ord, hipStream_t st) {
-  if (ord == 4)
-    hipLaunchKernelGGL((k_deriv_bundle<4, STORE>), dim3(brick_blocks<Geo<4, false>>(d)),
+  with_order(ord, [&](auto oc) {
+    constexpr int O = decltype(oc)::value;
+    hipLaunchKernelGGL((k_deriv_bundle<O, STORE>), dim3(brick_blocks<Geo<O, false>>(d)),
                        dim3(64), 0, st, F, o, d, w);
-  else
-    hipLaunchKernelGGL((k_deriv_bundle<6, STORE>), dim3(brick_blocks<Geo<6, false>>(d)),
-                       dim3(64), 0, st, F, o, d, w);
+  });
 }
 
 hipError_t launch_deriv_bundle(const double* F, double* D, const Dims& d, const Scales& w,
@@ -206,10 +205,10 @@ hipError_t launch_bundle_absmax(const double* F, unsigned long long* amax, const
                                 const Scales& w, int ord, hipStream_t st) {
   hipLaunchKernelGGL(k_clear_u64, dim3(1), dim3(256), 0, st, amax, BSSN_ND);
   const unsigned grid = 2048;
-  if (ord == 4)
-    hipLaunchKernelGGL(k_bundle_absmax<4>, dim3(grid), dim3(64), 0, st, F, amax, d, w);
-  else
-    hipLaunchKernelGGL(k_bundle_absmax<6>, dim3(grid), dim3(64), 0, st, F, amax, d, w);
+  with_order(ord, [&](auto oc) {
+    constexpr int O = decltype(oc)::value;
+    hipLaunchKernelGGL(k_bundle_absmax<O>, dim3(grid), dim3(64), 0, st, F, amax, d, w);
+  });
   return hipGetLastError();
 }
 
@@ -347,6 +346,8 @@ __global__ __launch_bounds__(64) void k_ko_add(const double* __restrict__ F,
   for_fields<DEPTH>(field);
 }
 
+// FD order 8 takes Kreiss-Oliger order 8, as FD order 6 does: the same kernel (the KO stencil
+// and its scale sigma / (256 h) do not depend on the FD order beyond the pairing).
 hipError_t launch_ko_add(const double* F, double* OUT, const Dims& d, const Scales& w, int ord,
                          int mode, hipStream_t st) {
   if (ord == 4) {
@@ -469,12 +470,11 @@ __global__ __launch_bounds__(64) void k_constraints(const double* __restrict__ F
 
 hipError_t launch_constraints(const double* F, double* HM, const Dims& d, const Scales& w,
                               int ord, hipStream_t st) {
-  if (ord == 4)
-    hipLaunchKernelGGL(k_constraints<4>, dim3(brick_blocks<Geo<4, false>>(d)), dim3(64), 0, st,
+  with_order(ord, [&](auto oc) {
+    constexpr int O = decltype(oc)::value;
+    hipLaunchKernelGGL(k_constraints<O>, dim3(brick_blocks<Geo<O, false>>(d)), dim3(64), 0, st,
                        F, HM, d, w);
-  else
-    hipLaunchKernelGGL(k_constraints<6>, dim3(brick_blocks<Geo<6, false>>(d)), dim3(64), 0, st,
-                       F, HM, d, w);
+  });
   return hipGetLastError();
 }
 

@@ -90,7 +90,7 @@ __device__ __forceinline__ void batch_brick(double* lds, const StageArgs& args,
   a.K1 = shifted(args.K1, blk.off);
   a.FP = shifted(args.FP, blk.off);
   a.Uout = shifted(args.Uout, blk.off);
-  WaveSrc<ORD, KO, MODE, RhsOrder, fused_depth<KIND>()> src(lds, a.Yin, w, d, ox, oy, oz, lane);
+  WaveSrc<ORD, KO, MODE, RhsOrder, fused_depth<KIND, ORD>()> src(lds, a.Yin, w, d, ox, oy, oz, lane);
   StageOut<KIND> out(a, d.P, src.gc, lane_valid<G>(d, ox, oy, oz, lane), true);
   bssn_rhs_point<KO>(src, sc, out);
 }

@@ -11,8 +11,9 @@ namespace bssn {
 
 hipError_t launch_fused_batch(const StageArgs& a, const BatchArgs& b, int grid, int ord, bool ko,
                               int mode, hipStream_t st) {
-  return ord == 4 ? launch_batch_o4(a, b, grid, ko, mode, st)
-                  : launch_batch_o6(a, b, grid, ko, mode, st);
+  return ord == 4   ? launch_batch_o4(a, b, grid, ko, mode, st)
+         : ord == 6 ? launch_batch_o6(a, b, grid, ko, mode, st)
+                    : launch_batch_o8(a, b, grid, ko, mode, st);
 }
 
 // Queue counters are cleared by a kernel (not a memset node) so that a captured step replays

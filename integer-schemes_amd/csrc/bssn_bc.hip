@@ -80,7 +80,9 @@ static hipError_t launch_bc_ord(const StageArgs& a, const Dims& d, const Scales&
 hipError_t launch_sommerfeld(const StageArgs& a, const Dims& d, const Scales& w, const BcGeom& g,
                              int ord, hipStream_t st) {
   if (d.ng <= 0) return hipSuccess;
-  return ord == 4 ? launch_bc_ord<4>(a, d, w, g, st) : launch_bc_ord<6>(a, d, w, g, st);
+  return ord == 4   ? launch_bc_ord<4>(a, d, w, g, st)
+         : ord == 6 ? launch_bc_ord<6>(a, d, w, g, st)
+                    : launch_bc_ord<8>(a, d, w, g, st);
 }
 
 }  // namespace bssn

@@ -62,6 +62,21 @@ struct Coef<6> {
     return k == 0 ? -70.0 : (k == 1 ? 56.0 : (k == 2 ? -28.0 : (k == 3 ? 8.0 : -1.0)));
   }
 };
+template <>
+struct Coef<8> {
+  static constexpr int R = 4;
+  __device__ static constexpr double d1(int k) {
+    return k == 1 ? 672.0 : (k == 2 ? -168.0 : (k == 3 ? 32.0 : (k == 4 ? -3.0 : 0.0)));
+  }
+  __device__ static constexpr double d2(int k) {
+    return k == 0 ? -14350.0
+         : (k == 1 ? 8064.0 : (k == 2 ? -1008.0 : (k == 3 ? 128.0 : -9.0)));
+  }
+  // Kreiss-Oliger of order 8 as well: the reference's KO stencils stop at order 8, and FD order
+  // 8 takes it by default (common/src/mcs_common/derivatives.py:45-49, 64)
+  static constexpr int RK = 4;
+  __device__ static constexpr double ko(int k) { return Coef<6>::ko(k); }
+};
 
 template <int ORD, bool KO>
 struct Geo {
@@ -76,6 +91,7 @@ struct Geo {
   static constexpr int PZ = 12;
   static constexpr int SY = PZ, SX = N * PZ;              // LDS strides (z contiguous)
   static constexpr int VOL = N * N * PZ;
+  static_assert(N <= PZ, "box rows must fit the LDS pitch");
   static constexpr int NCORE = B * B * B;
   static constexpr int NFACE = 6 * B * B * H;
   static constexpr int NEDGE = 12 * B * R * R;

@@ -39,7 +39,7 @@ void k_fused(StageArgs args, Dims d, Scales w, Scal sc) {
   if (!brick_origin<G>(d, &ox, &oy, &oz)) return;
   const int lane = (int)threadIdx.x;
   if constexpr (SDEV) args_from_pack<ORD>(args, w, sc);
-  WaveSrc<ORD, KO, MODE, RhsOrder, fused_depth<KIND>()> src(lds, args.Yin, w, d, ox, oy, oz, lane);
+  WaveSrc<ORD, KO, MODE, RhsOrder, fused_depth<KIND, ORD>()> src(lds, args.Yin, w, d, ox, oy, oz, lane);
   StageOut<KIND> out(args, d.P, src.gc, lane_valid<G>(d, ox, oy, oz, lane),
                      true);
   bssn_rhs_point<KO>(src, with_ssl(sc, args.tdev, args.toff), out);

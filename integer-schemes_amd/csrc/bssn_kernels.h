@@ -1,6 +1,8 @@
 // Internal launch interface between the C ABI (bssn_abi.cpp) and the kernels.
 #pragma once
 
+#include <type_traits>
+
 #include <hip/hip_runtime.h>
 
 #include "bssn_params.h"
@@ -55,16 +57,31 @@ __device__ __forceinline__ void args_from_pack(StageArgs& a, Scales& w, Scal& sc
   a.c0 = uniform(dt * a.c0mul / a.c0div);
 }
 
+// Calls fn(std::integral_constant<int, ORD>{}) with the FD order ord (4, 6 or 8) as a constant.
+template <class Fn>
+inline void with_order(int ord, Fn&& fn) {
+  if (ord == 4)
+    fn(std::integral_constant<int, 4>{});
+  else if (ord == 6)
+    fn(std::integral_constant<int, 6>{});
+  else
+    fn(std::integral_constant<int, 8>{});
+}
+
 hipError_t launch_fused(const StageArgs& a, const Dims& d, const Scales& w, const Scal& sc, int ord,
                         bool ko, int mode, hipStream_t st);
 hipError_t launch_fused_o4(const StageArgs& a, const Dims& d, const Scales& w, const Scal& sc,
                            bool ko, int mode, hipStream_t st);
 hipError_t launch_fused_o6(const StageArgs& a, const Dims& d, const Scales& w, const Scal& sc,
                            bool ko, int mode, hipStream_t st);
+hipError_t launch_fused_o8(const StageArgs& a, const Dims& d, const Scales& w, const Scal& sc,
+                           bool ko, int mode, hipStream_t st);
 // device-pack instances (StageArgs::sdev set): the bare RHS and RK4 kinds
 hipError_t launch_fused_o4s(const StageArgs& a, const Dims& d, const Scales& w, const Scal& sc,
                             bool ko, int mode, hipStream_t st);
 hipError_t launch_fused_o6s(const StageArgs& a, const Dims& d, const Scales& w, const Scal& sc,
+                            bool ko, int mode, hipStream_t st);
+hipError_t launch_fused_o8s(const StageArgs& a, const Dims& d, const Scales& w, const Scal& sc,
                             bool ko, int mode, hipStream_t st);
 
 // ----------------------------------------------------------------------------- block batches
@@ -109,6 +126,8 @@ hipError_t launch_fused_batch(const StageArgs& a, const BatchArgs& b, int grid, 
 hipError_t launch_batch_o4(const StageArgs& a, const BatchArgs& b, int grid, bool ko, int mode,
                            hipStream_t st);
 hipError_t launch_batch_o6(const StageArgs& a, const BatchArgs& b, int grid, bool ko, int mode,
+                           hipStream_t st);
+hipError_t launch_batch_o8(const StageArgs& a, const BatchArgs& b, int grid, bool ko, int mode,
                            hipStream_t st);
 hipError_t launch_batch_reset(int* ctr, hipStream_t st);
 hipError_t launch_batch_periodic(double* pool, const BlockDesc* blk, const BrickList& ghost,

@@ -76,9 +76,9 @@ struct Dims {
 __host__ __device__ inline Scales make_scales(const double h3[3], int ord, double sigma) {
 #pragma clang fp contract(off)
   Scales s;
-  const double den1 = ord == 4 ? 12.0 : 60.0;    // first derivative
-  const double den2 = ord == 4 ? 12.0 : 180.0;   // second derivative
-  const double denk = ord == 4 ? 64.0 : 256.0;   // KO of order ord + 2
+  const double den1 = ord == 4 ? 12.0 : (ord == 6 ? 60.0 : 840.0);     // first derivative
+  const double den2 = ord == 4 ? 12.0 : (ord == 6 ? 180.0 : 5040.0);    // second derivative
+  const double denk = ord == 4 ? 64.0 : 256.0;   // KO of order min(ord + 2, 8)
   for (int a = 0; a < 3; ++a) {
     s.s1[a] = 1.0 / (den1 * h3[a]);
     s.s2[a] = 1.0 / (den2 * (h3[a] * h3[a]));

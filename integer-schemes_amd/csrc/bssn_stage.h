@@ -62,9 +62,13 @@ constexpr int fused_waves() {
 #ifndef BSSN_STAGE_DEPTH
 #define BSSN_STAGE_DEPTH 1
 #endif
-template <int KIND>
+// FD order 8 (19 staged cells per lane) keeps one field in flight in every kind: its stencils
+// alone push the stage kinds past the register file.
+template <int KIND, int ORD = 4>
 constexpr int fused_depth() {
-  return fused_waves<KIND>() != 1 ? 1 : (KIND == KIND_RHS ? BSSN_RHS_DEPTH : BSSN_STAGE_DEPTH);
+  return fused_waves<KIND>() != 1 || ORD == 8
+             ? 1
+             : (KIND == KIND_RHS ? BSSN_RHS_DEPTH : BSSN_STAGE_DEPTH);
 }
 
 // Write-back operands are loaded this many phases of bssn_rhs_point before the phase that

@@ -160,7 +160,7 @@ def rk4_step(blocks: dict, hier: dict, nb_root, BS: int, ng: int, h0: float, S16
              alpha_floor: float = 1e-4) -> dict:
     """One shared-dt RK4 step of a synced hierarchy; returns the synced new hierarchy."""
     I = (slice(None),) + (slice(ng, ng + BS),) * 3
-    ko_order = order + 2 if ko_sigma > 0.0 else 0
+    ko_order = 0                         # the default pairing, min(order + 2, 8)
 
     def rhs(Y, tt):
         k = {}

@@ -538,6 +538,12 @@ static real ko_at(const real* u, const Dims* d, const int p[3], real wko[3][9], 
   return acc;
 }
 
+/* The Kreiss-Oliger order paired with an FD order: order + 2 (an order-2m FD stencil with
+ * order-2m+2 dissipation, common/src/mcs_common/derivatives.py:55-58), capped at 8, the widest
+ * KO stencil the reference has (derivatives.py:45-49): FD order 8 pairs with KO order 8, the
+ * reference's own default for it (derivatives.py:64). */
+static int default_ko_order(int order) { return order + 2 < 8 ? order + 2 : 8; }
+
 static int ko_weights(real wko[3][9], const real* h3, int ko_order, real sigma) {
   real c[9];
   const int nt = orc_stencil(2, ko_order, c);
@@ -551,7 +557,7 @@ int orc_rhs_points(const real* F, int Sx, int Sy, int Sz, const real* S16, int o
                    int ko_order, real ko_sigma, long npts, const long* idx, real* OUT) {
   FDW fw;
   const real h3[3] = {S16[13], S16[14], S16[15]};
-  if (ko_order == 0) ko_order = order + 2;
+  if (ko_order == 0) ko_order = default_ko_order(order);
   if (fd_weights(&fw, h3, order)) return -1;
   real wko[3][9];
   const int nko = ko_weights(wko, h3, ko_order, ko_sigma);
@@ -580,7 +586,7 @@ int orc_rhs(const real* F, real* OUT, int Sx, int Sy, int Sz, const real* S16, i
             int ko_order, real ko_sigma) {
   FDW fw;
   const real h3[3] = {S16[13], S16[14], S16[15]};
-  if (ko_order == 0) ko_order = order + 2;
+  if (ko_order == 0) ko_order = default_ko_order(order);
   if (fd_weights(&fw, h3, order)) return -1;
   real wko[3][9];
   const int nko = ko_weights(wko, h3, ko_order, ko_sigma);

@@ -56,7 +56,7 @@ void orc_deriv_points(const real* F, int Sx, int Sy, int Sz, const real* h3, int
 void orc_algebra(long n, const real* F, const real* D, const real* sc13, real* OUT);
 
 /* Full padded RHS: bundle + algebra (rhs.py:102-139), optional KO added when ko_sigma > 0
- * (evolve.py:76-90). ko_order 0 selects the default pairing (order + 2). */
+ * (evolve.py:76-90). ko_order 0 selects the default pairing (order + 2, at most 8). */
 int orc_rhs(const real* F, real* OUT, int Sx, int Sy, int Sz, const real* S16,
             int order, int ko_order, real ko_sigma);
 

@@ -56,7 +56,7 @@ def series(U0: np.ndarray, S16, dt: float, h: float, ng: int, total: int, chunk:
     rows = [row(U, t)]
     for _ in range(samples):
         U = O.evolve(U, S16, dt, chunk, t0=t, integrator=integrator, order=order,
-                     ko_order=order + 2 if ko_sigma > 0.0 else 0, ko_sigma=ko_sigma, ng=ng,
+                     ko_order=0, ko_sigma=ko_sigma, ng=ng,
                      periodic=True, chi_floor=chi_floor, alpha_floor=alpha_floor)
         t += chunk * dt
         rows.append(row(U, t))

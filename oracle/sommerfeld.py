@@ -61,8 +61,7 @@ def apply(dtu: np.ndarray, u: np.ndarray, ng: int, xmin3, h3, order: int = 4) ->
 def stage_rhs(F: np.ndarray, S16, ng: int, xmin3, order: int = 4, ko_sigma: float = 0.0):
     """f(F): the edge-padded RHS + KO at every padded point, ghosts replaced by the radiative
     condition (all 24 fields)."""
-    ko_order = order + 2 if ko_sigma > 0.0 else 0
-    out = O.rhs(F, S16, order=order, ko_order=ko_order, ko_sigma=ko_sigma)
+    out = O.rhs(F, S16, order=order, ko_sigma=ko_sigma)   # KO order: the default pairing
     h3 = (S16[13], S16[14], S16[15])
     for f in range(F.shape[0]):
         out[f] = apply(out[f], F[f], ng, xmin3, h3, order)

@@ -52,7 +52,7 @@ def make_batch(blocks, ng=3, seed=0):
     return batch, U
 
 
-@pytest.mark.parametrize("order,ko", [(4, 0.0), (4, 0.1), (6, 0.1)])
+@pytest.mark.parametrize("order,ko", [(4, 0.0), (4, 0.1), (6, 0.1), (8, 0.1)])
 @pytest.mark.parametrize("persistent", [0, 512])
 def test_batch_rhs_equals_single_grid(order, ko, persistent):
     say(f"batch rhs order={order} ko={ko} persistent={persistent}")

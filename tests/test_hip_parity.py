@@ -134,7 +134,8 @@ def test_twopass_matches_oracle():
     assert err.max() < TOL, err
 
 
-@pytest.mark.parametrize("shape,order", [((13, 17, 22), 4), ((9, 11, 21), 6), ((5, 6, 7), 4)])
+@pytest.mark.parametrize("shape,order", [((13, 17, 22), 4), ((9, 11, 21), 6), ((11, 13, 18), 8),
+                                         ((5, 6, 7), 4)])
 def test_ragged_shapes(shape, order):
     say(f"fused RHS on ragged extents {shape}, order {order}")
     F = smooth_state(shape, seed=sum(shape))
@@ -265,6 +266,33 @@ def test_order6_step():
     ref = O.periodic_sync(O.rk4_step(F, scalars(p, grid.dx, grid.dy, grid.dz, dt, 0.0), 0.0, dt,
                                      order=6, ko_sigma=0.1, ng=4), 4)
     assert normwise(got, ref).max() < TOL
+
+
+def test_order8_step():
+    say("rk4 step, order 8 with KO 8, ng 4, 16^3 single puncture")
+    grid, st, _ = idata.config_single_puncture(seed=4, N=16, ng=4)
+    F = st.numpy()
+    dt = 0.25 * grid.dx
+    p = PhysicsParams()
+    evo = BSSNEvolution(grid, p, order=8, ko_sigma=0.1, bc="periodic")
+    got = to_host(evo.step(to_dev(F), 0.0, dt).data)
+    ref = O.periodic_sync(O.rk4_step(F, scalars(p, grid.dx, grid.dy, grid.dz, dt, 0.0), 0.0, dt,
+                                     order=8, ko_sigma=0.1, ng=4), 4)
+    assert normwise(got, ref).max() < TOL
+
+
+def test_order8_constraints():
+    say("constraints and interior norms, order 8, 20^3 single puncture, ng 4")
+    grid, st, _ = idata.config_single_puncture(seed=7, N=20, ng=4)
+    F = st.numpy()
+    cs = ConstraintSolver(grid, order=8)
+    got = cs.constraints(to_dev(F))
+    HM = np.stack([to_host(got[k]) for k in ("H", "M0", "M1", "M2")])
+    h3 = [grid.dx, grid.dy, grid.dz]
+    assert normwise(HM, O.constraints(F, h3, 8)).max() < TOL
+    H, M = cs.l2(to_dev(F))
+    Hr, Mr = O.constraint_l2(F, h3, 8, grid.ng)
+    assert abs(H - Hr) <= 1e-12 * Hr and abs(M - Mr) <= 1e-12 * Mr
 
 
 def test_sync_and_enforce_kernels():

@@ -32,6 +32,19 @@ def test_header_and_binding_agree():
     assert declared_symbols() == sorted(_lib.SIGNATURES)
 
 
+def test_library_has_no_undefined_internal_symbols():
+    """Every launcher the C ABI calls is linked in (a compile unit missing from the build would
+    leave an undefined bssn:: symbol that only fails at load time)."""
+    import shutil
+    import subprocess
+    nm = shutil.which("nm")
+    if nm is None:
+        pytest.skip("nm not available")
+    out = subprocess.run([nm, "-D", "--undefined-only", str(_lib.LIB_PATH)], capture_output=True,
+                         text=True, check=True).stdout
+    assert not [l for l in out.splitlines() if "4bssn" in l]
+
+
 def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(str(_lib.LIB_PATH))
     for name in declared_symbols():
@@ -61,7 +74,17 @@ def test_argument_errors_before_device_work():
     # device scalar pack: null pack, bad order, bad shape are caught on the host
     assert lib.bssn_rhs_sdev_f64(fake, None, fake, 8, 8, 8, 4, 0.0, None) == 1
     assert b"null" in lib.bssn_last_error()
-    assert lib.bssn_rhs_sdev_f64(fake, fake, fake, 8, 8, 8, 8, 0.0, None) == 1
+    assert lib.bssn_rhs_sdev_f64(fake, fake, fake, 8, 8, 8, 10, 0.0, None) == 1
+    assert b"fd_order" in lib.bssn_last_error()
+    # FD order 8 needs 4 ghosts for its KO 8 stencil on the Sommerfeld step
+    xm = _lib.dbl([0.0, 0.0, 0.0])
+    assert lib.bssn_rk4_step_sommerfeld_f64(fake, fake, 16, 16, 16, S, 8, 0.1, 3, xm, 0.0,
+                                            0.1, 1e-4, 1e-4, None) == 1
+    assert b"reach" in lib.bssn_last_error()
+    # the INT8-split derivatives are FD order 4 only
+    assert lib.bssn_deriv_bundle_i8_f64(fake, fake, 16, 16, 16, 8, _lib.dbl([0.1] * 3), 7,
+                                        None) == 1
+    assert b"fd_order" in lib.bssn_last_error()
     assert lib.bssn_rk4_step_sdev_f64(fake, fake, 8, 8, 8, None, 4, 0.1, 3, 1, 1e-4, 1e-4,
                                       None) == 1
     assert lib.bssn_rk4_step_sdev_f64(fake, fake, 5, 5, 5, fake, 4, 0.1, 3, 1, 1e-4, 1e-4,
@@ -112,6 +135,16 @@ def test_solver_validation():
         BSSNSolver(g, order=4, scheme="nope")
     with pytest.raises(ValueError, match="precision"):
         BSSNSolver(g, order=4, precision="fp32_contraction")
+    g8 = Grid.from_domain(8, order=8)
+    assert g8.ng == 4
+    s8 = BSSNSolver(g8, order=8)
+    assert s8.ko_order == 8                  # FD 8 pairs with KO 8 (the widest KO stencil)
+    with pytest.raises(ValueError, match="KO order"):
+        BSSNSolver(g8, order=8, ko_order=6)
+    with pytest.raises(ValueError, match="order"):
+        BSSNSolver(g8, order=10)
+    with pytest.raises(ValueError, match="int8"):
+        BSSNSolver(g8, order=8, scheme="int8")
     s = BSSNSolver(g, order=4, scheme="verbatim")
     assert s.scheme == "twopass" and s.dt == 0.25 * g.dx
 

@@ -103,6 +103,9 @@ def test_stencil_coefficients():
     np.testing.assert_allclose(O.stencil(1, 6), np.array([2, -27, 270, -490, 270, -27, 2]) / 180.0, rtol=0, atol=0)
     np.testing.assert_allclose(O.stencil(2, 6), np.array([1, -6, 15, -20, 15, -6, 1]) / 64.0, rtol=0, atol=0)
     np.testing.assert_allclose(O.stencil(2, 8), np.array([-1, 8, -28, 56, -70, 56, -28, 8, -1]) / 256.0, rtol=0, atol=0)
+    # order 8 (common/src/mcs_common/derivatives.py:36, 42)
+    np.testing.assert_allclose(O.stencil(0, 8), np.array([3, -32, 168, -672, 0, 672, -168, 32, -3]) / 840.0, rtol=0, atol=0)
+    np.testing.assert_allclose(O.stencil(1, 8), np.array([-9, 128, -1008, 8064, -14350, 8064, -1008, 128, -9]) / 5040.0, rtol=0, atol=0)
     for order in (4, 6, 8):
         c1, c2 = O.stencil(0, order), O.stencil(1, order)
         k = np.arange(order + 1) - order // 2
@@ -113,7 +116,7 @@ def test_stencil_coefficients():
 
 
 @pytest.mark.parametrize("axis", [0, 1, 2])
-@pytest.mark.parametrize("order", [4, 6])
+@pytest.mark.parametrize("order", [4, 6, 8])
 def test_polynomial_exactness(axis, order):
     rng = np.random.default_rng(axis + 10 * order)
     S = 16
@@ -151,7 +154,7 @@ def test_sine_accuracy(axis):
     assert np.abs(d2 + f)[tuple(sl)].max() < 1e-3
 
 
-@pytest.mark.parametrize("order", [4, 6])
+@pytest.mark.parametrize("order", [4, 6, 8])
 def test_edge_padding_against_numpy(order):
     rng = np.random.default_rng(order)
     u = rng.normal(size=(7, 9, 8))
@@ -165,6 +168,24 @@ def test_edge_padding_against_numpy(order):
         ref = sum(c[k] * 2.0 * np.take(up, np.arange(k, k + n), axis=axis) for k in range(order + 1))
         got = O.apply_axis(u, axis, 0, order, 2.0)
         assert np.abs(got - ref).max() < 1e-12
+
+
+def test_default_ko_pairing():
+    """ko_order 0 pairs FD order 4 with KO 6, 6 with KO 8 and 8 with KO 8 (the reference's widest
+    KO stencil, derivatives.py:45-49); an order-10 KO request is refused."""
+    rng = np.random.default_rng(8)
+    F = 0.01 * rng.normal(size=(24, 11, 12, 13))
+    for f in (0, 1, 3, 6, 8):
+        F[f] += 1.0
+    S16 = [2.0, 0.06, 0.02, 0.1, 0.6, 20.0, 0.4, 1.1, 0.9, 1.2, 0.8, 1.0, 0.5, 0.1, 0.11, 0.09]
+    for order, ko in ((4, 6), (6, 8), (8, 8)):
+        np.testing.assert_array_equal(O.rhs(F, S16, order, ko_sigma=0.1),
+                                      O.rhs(F, S16, order, ko_order=ko, ko_sigma=0.1))
+        other = 4 if ko != 4 else 6
+        assert not np.array_equal(O.rhs(F, S16, order, ko_sigma=0.1),
+                                  O.rhs(F, S16, order, ko_order=other, ko_sigma=0.1))
+    with pytest.raises(ValueError):
+        O.rhs(F, S16, 8, ko_order=10, ko_sigma=0.1)
 
 
 def test_mixed_is_nested_first_derivatives():

@@ -35,7 +35,7 @@ def _state(N=20, seed=3):
     return grid, st.numpy()
 
 
-@pytest.mark.parametrize("order,ko", [(4, 0.0), (4, 0.1), (6, 0.0), (6, 0.1)])
+@pytest.mark.parametrize("order,ko", [(4, 0.0), (4, 0.1), (6, 0.0), (6, 0.1), (8, 0.1)])
 def test_rhs_device_pack_is_bit_identical(order, ko):
     say(f"bssn_rhs_sdev_f64 vs bssn_rhs_f64, order {order}, ko {ko}")
     grid, F = _state()
@@ -57,12 +57,12 @@ def test_rhs_device_pack_is_bit_identical(order, ko):
     assert bool(torch.isfinite(a).all())
 
 
-@pytest.mark.parametrize("order,periodic", [(4, 1), (4, 2), (6, 1), (4, 0)])
+@pytest.mark.parametrize("order,periodic", [(4, 1), (4, 2), (6, 1), (8, 1), (4, 0)])
 def test_rk4_step_device_pack_is_bit_identical(order, periodic):
     say(f"bssn_rk4_step_sdev_f64 vs bssn_rk4_step_f64, order {order}, periodic {periodic}")
     grid, F = _state(N=16)
     ng = grid.ng if order == 4 else 4
-    if order == 6:
+    if order >= 6:
         pad = np.pad(F, ((0, 0), (1, 1), (1, 1), (1, 1)), mode="edge")
         F = pad
     Sx, Sy, Sz = F.shape[1:]

@@ -32,7 +32,7 @@ def say(msg):
     print(f"[case] {msg}", flush=True)
 
 
-@pytest.mark.parametrize("order", [4, 6])
+@pytest.mark.parametrize("order", [4, 6, 8])
 def test_boundary_rhs_matches_restatement(order):
     say(f"sommerfeld rhs order {order}")
     ng = 3 if order == 4 else 4
