@@ -65,8 +65,10 @@ __device__ __forceinline__ void bundle_put(const BundleOut& o, int slot, long P,
     if (valid) o.D[(long)slot * P + gc] = v;
   } else {
     if (!valid) return;
+    // saturate to [-2^(k-1), 2^(k-1) - 1]; a NaN (which an integer mantissa cannot hold) stores
+    // as the negative saturation value, like -inf, so the conversion below is always defined
     double m = rint(ldexp(v, o.e[slot]));
-    m = m < -o.lim ? -o.lim : (m > o.lim - 1.0 ? o.lim - 1.0 : m);
+    m = !(m >= -o.lim) ? -o.lim : (m > o.lim - 1.0 ? o.lim - 1.0 : m);
     const long i = (long)slot * P + gc;
     if constexpr (STORE == BS_Q32) {
       ((int*)o.Q)[i] = (int)m;

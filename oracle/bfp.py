@@ -30,13 +30,19 @@ def block_exponent(a: np.ndarray, k: int) -> int:
 
 
 def mantissas(a: np.ndarray, k: int, e: int) -> np.ndarray:
+    """The stored integer mantissas. Parity unpinned for NaN: the reference's emulation keeps
+    floats (a NaN stays NaN, bfp_compress.py:88-90); an integer mantissa cannot hold one, so the
+    device stores it as the negative saturation value -2^(k-1), like -inf."""
     lim = 2.0 ** (k - 1)
-    return np.clip(np.rint(np.ldexp(a, e)), -lim, lim - 1.0)
+    m = np.clip(np.rint(np.ldexp(a, e)), -lim, lim - 1.0)
+    return np.where(np.isnan(m), -lim, m)
 
 
 def quantize(a: np.ndarray, k: int, e: int | None = None) -> np.ndarray:
+    """The reference's float round trip (bfp_compress.py:70-90): NaN stays NaN."""
     e = block_exponent(a, k) if e is None else e
-    return np.ldexp(mantissas(a, k, e), -e)
+    a = np.asarray(a, dtype=np.float64)
+    return np.where(np.isnan(a), np.nan, np.ldexp(mantissas(a, k, e), -e))
 
 
 def pack48(m: np.ndarray) -> np.ndarray:

@@ -111,6 +111,23 @@ def test_bfp_storage_bitwise(k):
     assert np.array_equal(m, want.astype(np.int64))
 
 
+@pytest.mark.parametrize("k", [24, 48])
+def test_bfp_storage_nan_saturates(k):
+    say(f"BFP{k} storage of a bundle with a NaN, fixed exponents")
+    grid, F = puncture(16, 4)
+    h3 = (grid.dx, grid.dy, grid.dz)
+    D0 = gpu_bundle(F, h3, 4).cpu().numpy().reshape(138, -1)
+    exps = bfp_exponents(np.max(np.abs(D0), axis=1), k)     # fixed from the finite state
+    F[5, 9, 10, 11] = float("nan")                          # gt2 (gt_xz) at one interior point
+    D = gpu_bundle(F, h3, 4).cpu().numpy().reshape(138, -1)
+    assert np.isnan(D).any()
+    Q = gpu_store(F, h3, 4, k, exps)
+    _, m = unpack(Q, k, D.shape[1])
+    want = np.stack([B.mantissas(D[i], k, int(exps[i])) for i in range(138)])
+    assert np.array_equal(m, want.astype(np.int64))
+    assert np.all(m[np.isnan(D)] == -(2 ** (k - 1)))
+
+
 @pytest.mark.parametrize("k", [32, 48])
 def test_bfp_algebra_bitwise(k):
     say(f"algebra from BFP{k}")

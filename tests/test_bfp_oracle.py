@@ -42,6 +42,14 @@ def test_quantize_closed_forms():
         assert np.max(np.abs(B.quantize(x, k) - x)) <= 2.0 ** (-e - 1)
     # saturation with a given exponent, round half to even
     assert np.array_equal(B.mantissas(np.array([10.0, -10.0]), 4, 0), [7.0, -8.0])
+    # non-finite values with a given exponent: +-inf saturate, NaN stores as -2^(k-1) (an integer
+    # mantissa cannot hold it); the float round trip keeps NaN as the reference's emulation does
+    v = np.array([np.nan, -np.inf, np.inf, -1e300, 1e300])
+    for k in (24, 32, 48):
+        lim = 2.0 ** (k - 1)
+        assert np.array_equal(B.mantissas(v, k, 3), [-lim, -lim, lim - 1, -lim, lim - 1])
+    q = B.quantize(np.array([np.nan, 0.5]), 24, 3)
+    assert np.isnan(q[0]) and q[1] == 0.5
     assert np.array_equal(B.mantissas(np.array([0.5, 1.5, 2.5, -0.5]), 8, 0), [0, 2, 2, -0])
     assert B.block_exponent(np.zeros(5), 32) == 0 and np.all(B.quantize(np.zeros(5), 32) == 0)
     # the product's exponent helper is the same formula
