@@ -3,9 +3,11 @@
     python integer-schemes_amd/autotune.py [--n 256] [--steps 6] [--space full|quick]
                                            [--write] [--out tuning.json]
 
-The stage kernels' schedule is a set of compile-time knobs (bssn_stage.h): how many fields are
-staged ahead of the one being evaluated (BSSN_STAGE_DEPTH for the RK/MSRK stage kernels,
-BSSN_RHS_DEPTH for the bare RHS), how many algebra phases ahead of its store a field's
+The stage kernels' schedule is a set of compile-time knobs (bssn_stage.h, bssn_brick.h): the
+LDS box's x-plane stride (BSSN_BOX_XPLANE_MOD32: dense, or padded to a residue modulo 32 doubles
+that spreads the brick's lanes over the banks), how many fields are staged ahead of the one
+being evaluated (BSSN_STAGE_DEPTH for the RK/MSRK stage kernels, BSSN_RHS_DEPTH for the bare
+RHS), how many algebra phases ahead of its store a field's
 write-back operands are loaded (BSSN_WB_LEAD), and how many waves per SIMD the register
 allocator targets (BSSN_STAGE_WAVES, BSSN_RHS_WAVES; two waves force one field in flight). A
 knob moves loads and register allocation, not arithmetic, so every variant computes the same
@@ -40,6 +42,7 @@ SPACES = {
     "quick": [
         {},
         {"BSSN_STAGE_DEPTH": 2},
+        {"BSSN_BOX_XPLANE_MOD32": 16},
     ],
     "full": [
         {},
@@ -51,6 +54,8 @@ SPACES = {
         {"BSSN_RHS_DEPTH": 1},
         {"BSSN_STAGE_WAVES": 2},
         {"BSSN_RHS_WAVES": 2},
+        {"BSSN_BOX_XPLANE_MOD32": 16},
+        {"BSSN_BOX_XPLANE_MOD32": 16, "BSSN_WB_LEAD": 3},
     ],
 }
 

@@ -78,6 +78,11 @@ struct Coef<8> {
   __device__ static constexpr double ko(int k) { return Coef<6>::ko(k); }
 };
 
+// LDS box x-plane stride modulo 32 doubles (-1: dense planes); a schedule knob of the autotuner
+#ifndef BSSN_BOX_XPLANE_MOD32
+#define BSSN_BOX_XPLANE_MOD32 -1
+#endif
+
 template <int ORD, bool KO>
 struct Geo {
   static constexpr int R = Coef<ORD>::R;                  // FD reach
@@ -89,8 +94,15 @@ struct Geo {
   // z rows are padded to a pitch of 12 doubles: for the 16 lanes of one LDS access phase
   // (4 y-rows x 4 z of the brick) the addresses ty*12 + tz fall on 16 distinct bank pairs.
   static constexpr int PZ = 12;
-  static constexpr int SY = PZ, SX = N * PZ;              // LDS strides (z contiguous)
-  static constexpr int VOL = N * N * PZ;
+  // x planes: dense (N * PZ), or, with BSSN_BOX_XPLANE_MOD32 = r, padded to the next stride
+  // congruent to r modulo 32 doubles: r = 16 puts the two x-planes of the brick that 32 lanes
+  // span (tx, tx + 1) on disjoint halves of 64 banks.
+  static constexpr int SX0 = N * PZ;
+  static constexpr int SX = BSSN_BOX_XPLANE_MOD32 < 0
+                                ? SX0
+                                : SX0 + ((BSSN_BOX_XPLANE_MOD32 - SX0 % 32) % 32 + 32) % 32;
+  static constexpr int SY = PZ;                           // LDS strides (z contiguous)
+  static constexpr int VOL = N * SX;
   static_assert(N <= PZ, "box rows must fit the LDS pitch");
   static constexpr int NCORE = B * B * B;
   static constexpr int NFACE = 6 * B * B * H;

@@ -223,13 +223,13 @@ def _load_module(name, path):
 
 
 def test_autotune_space_and_installed_choice():
-    """The autotuner's knobs are the ones bssn_stage.h reads (#ifndef NAME), candidate keys are
+    """The autotuner's knobs are the ones the kernel headers read (#ifndef NAME), candidate keys are
     unique, and the installed tuning.json names a candidate of the full space that passed its
     check; build.tuned_defines() hands exactly those -D knobs to the installed build."""
     pkg = ROOT / "integer-schemes_amd"
     at = _load_module("bssn_autotune", pkg / "autotune.py")
     bd = _load_module("bssn_build", pkg / "build.py")
-    stage_h = (pkg / "csrc" / "bssn_stage.h").read_text()
+    stage_h = "".join((pkg / "csrc" / h).read_text() for h in ("bssn_stage.h", "bssn_brick.h"))
     for space in at.SPACES.values():
         keys = [at.key_of(d) for d in space]
         assert len(set(keys)) == len(keys) and keys[0] == "default"
