@@ -3,15 +3,16 @@
     python integer-schemes_amd/autotune.py [--n 256] [--steps 6] [--space full|quick]
                                            [--write] [--out tuning.json]
 
-The stage kernels' schedule is a set of compile-time knobs (bssn_stage.h, bssn_brick.h): the
-LDS box's x-plane stride (BSSN_BOX_XPLANE_MOD32: dense, or padded to a residue modulo 32 doubles
-that spreads the brick's lanes over the banks), how many fields are staged ahead of the one
-being evaluated (BSSN_STAGE_DEPTH for the RK/MSRK stage kernels, BSSN_RHS_DEPTH for the bare
-RHS), how many algebra phases ahead of its store a field's
-write-back operands are loaded (BSSN_WB_LEAD), and how many waves per SIMD the register
-allocator targets (BSSN_STAGE_WAVES, BSSN_RHS_WAVES; two waves force one field in flight). A
-knob moves loads and register allocation, not arithmetic, so every variant computes the same
-RHS.
+The stage kernels' schedule is a set of compile-time knobs (bssn_params.h, bssn_brick.h,
+bssn_stage.h): the brick of output points one wave owns (BSSN_BRICK_X/Y/Z, 64 points: the
+4x4x4 cube stages the fewest halo cells, the flatter bricks longer z runs), the LDS box's
+x-plane stride (BSSN_BOX_XPLANE_MOD32: dense, or padded to a residue modulo 32 doubles that
+spreads the brick's lanes over the banks), how many fields are staged ahead of the one being
+evaluated (BSSN_STAGE_DEPTH for the RK/MSRK stage kernels, BSSN_RHS_DEPTH for the bare RHS),
+how many algebra phases ahead of its store a field's write-back operands are loaded
+(BSSN_WB_LEAD), and how many waves per SIMD the register allocator targets (BSSN_STAGE_WAVES,
+BSSN_RHS_WAVES; two waves force one field in flight). A knob moves loads, LDS addresses and
+register allocation, not arithmetic, so every variant computes the same RHS.
 
 For every candidate the tuner
   1. builds a variant library (build.build(out=..., defines=...)) under integer-schemes_amd/_tune/;
@@ -56,6 +57,11 @@ SPACES = {
         {"BSSN_RHS_WAVES": 2},
         {"BSSN_BOX_XPLANE_MOD32": 16},
         {"BSSN_BOX_XPLANE_MOD32": 16, "BSSN_WB_LEAD": 3},
+        {"BSSN_BRICK_X": 2, "BSSN_BRICK_Y": 4, "BSSN_BRICK_Z": 8},
+        {"BSSN_BRICK_X": 4, "BSSN_BRICK_Y": 2, "BSSN_BRICK_Z": 8},
+        {"BSSN_BRICK_X": 2, "BSSN_BRICK_Y": 8, "BSSN_BRICK_Z": 4},
+        {"BSSN_BRICK_X": 8, "BSSN_BRICK_Y": 4, "BSSN_BRICK_Z": 2},
+        {"BSSN_BRICK_X": 2, "BSSN_BRICK_Y": 2, "BSSN_BRICK_Z": 16},
     ],
 }
 

@@ -661,7 +661,7 @@ constexpr size_t kAlign = 256;
 
 size_t align_up(size_t n) { return (n + kAlign - 1) & ~(kAlign - 1); }
 
-int cdiv4(int n) { return (n + 3) / 4; }
+int cdiv(int n, int e) { return (n + e - 1) / e; }
 
 // Brick lists of one batch, built on the host. A brick of the padded block holds a ghost point
 // unless it lies inside [ng, S - ng) on every axis.
@@ -673,22 +673,23 @@ struct HostLists {
 void block_bricks(int b, const int* S, int ng, std::vector<BrickItem>* full,
                   std::vector<BrickItem>* inner, std::vector<BrickItem>* ghost) {
   auto pack = [](int x, int y, int z) { return x | (y << 10) | (z << 20); };
-  for (int x = 0; x < S[0]; x += 4)
-    for (int y = 0; y < S[1]; y += 4)
-      for (int z = 0; z < S[2]; z += 4) {
+  const int E[3] = {BSSN_BRICK_X, BSSN_BRICK_Y, BSSN_BRICK_Z};
+  for (int x = 0; x < S[0]; x += E[0])
+    for (int y = 0; y < S[1]; y += E[1])
+      for (int z = 0; z < S[2]; z += E[2]) {
         if (full) full->push_back({b, pack(x, y, z)});
         if (ghost) {
           const int o[3] = {x, y, z};
           bool inside = true;
           for (int a = 0; a < 3; ++a)
-            inside = inside && o[a] >= ng && o[a] + 4 <= S[a] - ng;
+            inside = inside && o[a] >= ng && o[a] + E[a] <= S[a] - ng;
           if (!inside) ghost->push_back({b, pack(x, y, z)});
         }
       }
   if (inner)
-    for (int x = ng; x < S[0] - ng; x += 4)
-      for (int y = ng; y < S[1] - ng; y += 4)
-        for (int z = ng; z < S[2] - ng; z += 4) inner->push_back({b, pack(x, y, z)});
+    for (int x = ng; x < S[0] - ng; x += E[0])
+      for (int y = ng; y < S[1] - ng; y += E[1])
+        for (int z = ng; z < S[2] - ng; z += E[2]) inner->push_back({b, pack(x, y, z)});
 }
 
 long list_count(int nblk, const int* S, int ng, int which) {
@@ -696,9 +697,10 @@ long list_count(int nblk, const int* S, int ng, int which) {
   for (int b = 0; b < nblk; ++b) {
     const int* s = S + 3 * b;
     if (which == 0) {
-      n += (long)cdiv4(s[0]) * cdiv4(s[1]) * cdiv4(s[2]);
+      n += (long)cdiv(s[0], BSSN_BRICK_X) * cdiv(s[1], BSSN_BRICK_Y) * cdiv(s[2], BSSN_BRICK_Z);
     } else if (which == 1) {
-      n += (long)cdiv4(s[0] - 2 * ng) * cdiv4(s[1] - 2 * ng) * cdiv4(s[2] - 2 * ng);
+      n += (long)cdiv(s[0] - 2 * ng, BSSN_BRICK_X) * cdiv(s[1] - 2 * ng, BSSN_BRICK_Y) *
+           cdiv(s[2] - 2 * ng, BSSN_BRICK_Z);
     } else {
       std::vector<BrickItem> g;
       block_bricks(b, s, ng, nullptr, nullptr, &g);

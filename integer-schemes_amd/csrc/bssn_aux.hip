@@ -156,8 +156,8 @@ __global__ __launch_bounds__(64) void k_bundle_absmax(const double* __restrict__
   using G = Geo<ORD, false>;
   __shared__ double lds[G::VOL];
   const int lane = (int)threadIdx.x;
-  const int nbx = (d.n[0] + G::B - 1) / G::B, nby = (d.n[1] + G::B - 1) / G::B,
-            nbz = (d.n[2] + G::B - 1) / G::B;
+  const int nbx = (d.n[0] + G::BX - 1) / G::BX, nby = (d.n[1] + G::BY - 1) / G::BY,
+            nbz = (d.n[2] + G::BZ - 1) / G::BZ;
   const int nb = nbx * nby * nbz;
   double r0 = 0.0, r1 = 0.0, r2 = 0.0;
   auto keep = [&](int slot, double v, bool valid) {
@@ -173,7 +173,7 @@ __global__ __launch_bounds__(64) void k_bundle_absmax(const double* __restrict__
   };
   for (int L = (int)blockIdx.x; L < nb; L += (int)gridDim.x) {
     const int bz = L % nbz, t = L / nbz, by = t % nby, bx = t / nby;
-    const int ox = d.lo[0] + bx * G::B, oy = d.lo[1] + by * G::B, oz = d.lo[2] + bz * G::B;
+    const int ox = d.lo[0] + bx * G::BX, oy = d.lo[1] + by * G::BY, oz = d.lo[2] + bz * G::BZ;
     const bool valid = lane_valid<G>(d, ox, oy, oz, lane);
     // one field in flight (two measured no faster here)
     WaveSrc<ORD, false, CLAMP, StateOrder> s(lds, F, w, d, ox, oy, oz, lane);

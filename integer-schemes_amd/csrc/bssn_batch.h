@@ -4,7 +4,7 @@
 // Replaces, on the reference side, the slot-batched step of the block-AMR solver
 // (2D/src/mcs2d/amr/evolve.py:659-775): there every level is a fixed-shape slot array and the
 // RHS is vmapped over all slots, inactive ones included (:710-717). Here the work is a list of
-// 4x4x4 bricks over the blocks that exist, of any extent, and each brick is evaluated by the
+// bricks over the blocks that exist, of any extent, and each brick is evaluated by the
 // same wave program as the single-grid kernel (bssn_fused.h), so a block's result is
 // bit-identical to running the single-grid kernel on that block alone.
 //

@@ -41,8 +41,8 @@ __global__ __launch_bounds__(64) void k_batch_periodic(double* __restrict__ pool
   const BrickItem item = ghost.items[i];
   const BlockDesc blk = blks[item.blk];
   const int lane = (int)threadIdx.x;
-  const int c[3] = {(item.org & 1023) + lane / 16, ((item.org >> 10) & 1023) + (lane / 4) % 4,
-                    ((item.org >> 20) & 1023) + lane % 4};
+  const int c[3] = {(item.org & 1023) + brick_lx(lane), ((item.org >> 10) & 1023) + brick_ly(lane),
+                    ((item.org >> 20) & 1023) + brick_lz(lane)};
   bool ghost_pt = false;
   int s[3];
 #pragma unroll
@@ -90,8 +90,8 @@ __global__ __launch_bounds__(64) void k_batch_hold(const double* __restrict__ U,
   const BrickItem item = ghost.items[i];
   const BlockDesc blk = blks[item.blk];
   const int lane = (int)threadIdx.x;
-  const int c[3] = {(item.org & 1023) + lane / 16, ((item.org >> 10) & 1023) + (lane / 4) % 4,
-                    ((item.org >> 20) & 1023) + lane % 4};
+  const int c[3] = {(item.org & 1023) + brick_lx(lane), ((item.org >> 10) & 1023) + brick_ly(lane),
+                    ((item.org >> 20) & 1023) + brick_lz(lane)};
   bool ghost_pt = false;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {

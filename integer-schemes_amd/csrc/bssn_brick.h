@@ -1,14 +1,15 @@
 // Wave-private LDS bricks for the fused kernels.
 //
-// One workgroup is one wave of 64 lanes owning a 4x4x4 brick of output points (one point per
-// lane, z fastest). The 24 fields are staged one at a time into a private LDS box of edge
-// 4 + 2H; since a wave's LDS operations execute in issue order, staging needs no workgroup
+// One workgroup is one wave of 64 lanes owning a brick of output points (one point per lane,
+// z fastest): 4x4x4 by default, any BSSN_BRICK_X x BSSN_BRICK_Y x BSSN_BRICK_Z of 64 points.
+// The 24 fields are staged one at a time into a private LDS box (the brick plus H on every
+// side); since a wave's LDS operations execute in issue order, staging needs no workgroup
 // barrier, so every wave runs independently and latency is hidden across the waves of a CU.
 //
 // Only the cells a stencil can reach are staged: the core, the six face slabs to depth H
 // (first/second derivatives and Kreiss-Oliger along one axis) and the twelve edge bars to depth
-// R in two axes (mixed derivatives). For FD order 4 with KO order 6 that is 544 of the 1000
-// cells of the box, 8.5 loads per point and field; the 13 fields without mixed derivatives
+// R in two axes (mixed derivatives). For the 4x4x4 brick, FD order 4 with KO order 6, that is
+// 544 of the 1000 cells of the box, 8.5 loads per point and field; the 13 fields without mixed derivatives
 // (Gt^i, At_ij, K, B^i) skip the edge bars: 352 cells, 5.5 loads per point.
 //
 // Halo positions are mapped to the grid once per lane (not per field):
@@ -88,25 +89,30 @@ struct Geo {
   static constexpr int R = Coef<ORD>::R;                  // FD reach
   static constexpr int RK = KO ? Coef<ORD>::RK : 0;       // KO reach
   static constexpr int H = R > RK ? R : RK;               // box halo
-  static constexpr int B = 4;                             // brick edge
+  static constexpr int BX = BSSN_BRICK_X, BY = BSSN_BRICK_Y, BZ = BSSN_BRICK_Z;  // brick
   static constexpr int NT = 64;                           // lanes (one wave)
-  static constexpr int N = B + 2 * H;                     // box edge
-  // z rows are padded to a pitch of 12 doubles: for the 16 lanes of one LDS access phase
-  // (4 y-rows x 4 z of the brick) the addresses ty*12 + tz fall on 16 distinct bank pairs.
-  static constexpr int PZ = 12;
-  // x planes: dense (N * PZ), or, with BSSN_BOX_XPLANE_MOD32 = r, padded to the next stride
-  // congruent to r modulo 32 doubles: r = 16 puts the two x-planes of the brick that 32 lanes
-  // span (tx, tx + 1) on disjoint halves of 64 banks.
-  static constexpr int SX0 = N * PZ;
+  static constexpr int NX = BX + 2 * H, NY = BY + 2 * H, NZ = BZ + 2 * H;   // box extents
+  // z rows are padded to a pitch of 12 doubles (the 4x4x4 brick): for the 16 lanes of one LDS
+  // access phase (4 y-rows x 4 z of the brick) the addresses ty*12 + tz fall on 16 distinct
+  // bank pairs. Other bricks use the smallest even pitch that holds a row.
+  static constexpr int PZ = NZ <= 12 ? 12 : NZ + (NZ & 1);
+  // x planes: dense (NY * PZ), or, with BSSN_BOX_XPLANE_MOD32 = r, padded to the next stride
+  // congruent to r modulo 32 doubles: r = 16 puts the two x-planes of the 4x4x4 brick that 32
+  // lanes span (tx, tx + 1) on disjoint halves of 64 banks.
+  static constexpr int SX0 = NY * PZ;
   static constexpr int SX = BSSN_BOX_XPLANE_MOD32 < 0
                                 ? SX0
                                 : SX0 + ((BSSN_BOX_XPLANE_MOD32 - SX0 % 32) % 32 + 32) % 32;
   static constexpr int SY = PZ;                           // LDS strides (z contiguous)
-  static constexpr int VOL = N * SX;
-  static_assert(N <= PZ, "box rows must fit the LDS pitch");
-  static constexpr int NCORE = B * B * B;
-  static constexpr int NFACE = 6 * B * B * H;
-  static constexpr int NEDGE = 12 * B * R * R;
+  static constexpr int VOL = NX * SX;
+  static_assert(NX < 32 && NY < 32 && NZ < 32, "box coordinates are packed in 5 bits");
+  static constexpr int NCORE = BX * BY * BZ;
+  static constexpr int NFACE = 2 * H * (BY * BZ + BX * BZ + BX * BY);
+  static constexpr int NEDGE = 4 * R * R * (BX + BY + BZ);
+  // lane -> brick coordinates (z fastest)
+  __host__ __device__ static constexpr int lx(int lane) { return brick_lx(lane); }
+  __host__ __device__ static constexpr int ly(int lane) { return brick_ly(lane); }
+  __host__ __device__ static constexpr int lz(int lane) { return brick_lz(lane); }
   static constexpr int NCELL = NCORE + NFACE + NEDGE;
   static constexpr int PER = (NCELL + NT - 1) / NT;       // staged cells per lane
   // fields without mixed derivatives need the core and the face slabs only: the first
@@ -131,33 +137,34 @@ template <class G>
 struct CellTable {
   short c[G::PER * G::NT];
   constexpr CellTable() : c() {
-    constexpr int B = G::B, H = G::H, R = G::R, N = G::N;
+    constexpr int BX = G::BX, BY = G::BY, BZ = G::BZ, H = G::H, R = G::R, NZ = G::NZ;
     int n = 0;
     auto row = [&](int x, int y, int z0, int z1) {
       for (int z = z0; z < z1; ++z) c[n++] = (short)(x | (y << 5) | (z << 10));
     };
-    for (int x = H; x < H + B; ++x)
-      for (int y = H; y < H + B; ++y) row(x, y, 0, N);
+    for (int x = H; x < H + BX; ++x)
+      for (int y = H; y < H + BY; ++y) row(x, y, 0, NZ);
     // face rows over the core, then (second pass) their z extensions within reach R
     for (int pass = 0; pass < 2; ++pass)
       for (int ax = 0; ax < 2; ++ax)
         for (int side = 0; side < 2; ++side)
           for (int d = 1; d <= H; ++d)
-            for (int u = H; u < H + B; ++u) {
-              const int w = side == 0 ? H - d : H + B - 1 + d;
+            for (int u = H; u < H + (ax == 0 ? BY : BX); ++u) {
+              const int w = side == 0 ? H - d : H + (ax == 0 ? BX : BY) - 1 + d;
               const int x = ax == 0 ? w : u, y = ax == 0 ? u : w;
               if (pass == 0) {
-                row(x, y, H, H + B);
+                row(x, y, H, H + BZ);
               } else if (d <= R) {
                 row(x, y, H - R, H);
-                row(x, y, H + B, H + B + R);
+                row(x, y, H + BZ, H + BZ + R);
               }
             }
     for (int sx = 0; sx < 2; ++sx)
       for (int sy = 0; sy < 2; ++sy)
         for (int dx = 1; dx <= R; ++dx)
           for (int dy = 1; dy <= R; ++dy)
-            row(sx == 0 ? H - dx : H + B - 1 + dx, sy == 0 ? H - dy : H + B - 1 + dy, H, H + B);
+            row(sx == 0 ? H - dx : H + BX - 1 + dx, sy == 0 ? H - dy : H + BY - 1 + dy, H,
+                H + BZ);
     while (n < G::PER * G::NT) c[n++] = (short)(H | (H << 5) | (H << 10));
   }
 };
@@ -216,7 +223,7 @@ struct ConsOrder {
 // flat index of the lane's point, clamped into the grid (write-back index of invalid lanes)
 template <class G>
 __device__ __forceinline__ long lane_point(const Dims& d, int ox, int oy, int oz, int lane) {
-  const int tx = lane / (G::B * G::B), ty = (lane / G::B) % G::B, tz = lane % G::B;
+  const int tx = G::lx(lane), ty = G::ly(lane), tz = G::lz(lane);
   int p[3] = {ox + tx, oy + ty, oz + tz};
 #pragma unroll
   for (int a = 0; a < 3; ++a) p[a] = p[a] > d.S[a] - 1 ? d.S[a] - 1 : p[a];
@@ -313,7 +320,7 @@ struct WaveSrc : BoxStencil<Geo<ORD, KO>, Coef<ORD>> {
   __device__ __forceinline__ WaveSrc(double* lds_, const double* F_, const Scales& sc_,
                                      const Dims& d, int ox, int oy, int oz, int lane)
       : BoxStencil<G, C>(lds_, sc_), F(F_), P(d.P) {
-    const int tx = lane / (G::B * G::B), ty = (lane / G::B) % G::B, tz = lane % G::B;
+    const int tx = G::lx(lane), ty = G::ly(lane), tz = G::lz(lane);
     c = (tx + G::H) * G::SX + (ty + G::H) * G::SY + (tz + G::H);
     gc = lane_point<G>(d, ox, oy, oz, lane);
     const int o[3] = {ox, oy, oz};
@@ -433,27 +440,27 @@ __device__ __forceinline__ bool brick_of_block(int nbx, int nby, int nbz, int* b
 
 template <class G>
 __device__ __forceinline__ bool brick_origin(const Dims& d, int* ox, int* oy, int* oz) {
-  const int nbx = (d.n[0] + G::B - 1) / G::B;
-  const int nby = (d.n[1] + G::B - 1) / G::B;
-  const int nbz = (d.n[2] + G::B - 1) / G::B;
+  const int nbx = (d.n[0] + G::BX - 1) / G::BX;
+  const int nby = (d.n[1] + G::BY - 1) / G::BY;
+  const int nbz = (d.n[2] + G::BZ - 1) / G::BZ;
   int bx, by, bz;
   if (!brick_of_block(nbx, nby, nbz, &bx, &by, &bz)) return false;
-  *ox = d.lo[0] + bx * G::B;
-  *oy = d.lo[1] + by * G::B;
-  *oz = d.lo[2] + bz * G::B;
+  *ox = d.lo[0] + bx * G::BX;
+  *oy = d.lo[1] + by * G::BY;
+  *oz = d.lo[2] + bz * G::BZ;
   return true;
 }
 
 template <class G>
 inline unsigned brick_blocks(const Dims& d) {
-  const long nb = (long)((d.n[0] + G::B - 1) / G::B) * ((d.n[1] + G::B - 1) / G::B) *
-                  ((d.n[2] + G::B - 1) / G::B);
+  const long nb = (long)((d.n[0] + G::BX - 1) / G::BX) * ((d.n[1] + G::BY - 1) / G::BY) *
+                  ((d.n[2] + G::BZ - 1) / G::BZ);
   return (unsigned)((nb + 7) & ~7L);
 }
 
 template <class G>
 __device__ __forceinline__ bool lane_valid(const Dims& d, int ox, int oy, int oz, int lane) {
-  const int tx = lane / (G::B * G::B), ty = (lane / G::B) % G::B, tz = lane % G::B;
+  const int tx = G::lx(lane), ty = G::ly(lane), tz = G::lz(lane);
   return (ox + tx < d.lo[0] + d.n[0]) && (oy + ty < d.lo[1] + d.n[1]) &&
          (oz + tz < d.lo[2] + d.n[2]);
 }

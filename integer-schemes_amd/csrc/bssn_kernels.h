@@ -95,7 +95,7 @@ struct BlockDesc {
   long doff;      // offset of the block's 138 derivative arrays in a batch bundle, in doubles
 };
 
-// One 4x4x4 brick of one block: the block index and the brick origin in the block's padded
+// One brick of one block: the block index and the brick origin in the block's padded
 // coordinates (10 bits per axis, x | y << 10 | z << 20).
 struct BrickItem {
   int blk;

@@ -14,6 +14,19 @@
 #define BSSN_ND 138
 #define BSSN_MAXTAP 9
 
+// Brick of output points one wave owns (one point per lane): extents along x, y, z, product 64.
+// A schedule knob of the autotuner; host brick lists and device kernels share it.
+#ifndef BSSN_BRICK_X
+#define BSSN_BRICK_X 4
+#endif
+#ifndef BSSN_BRICK_Y
+#define BSSN_BRICK_Y 4
+#endif
+#ifndef BSSN_BRICK_Z
+#define BSSN_BRICK_Z 4
+#endif
+static_assert(BSSN_BRICK_X * BSSN_BRICK_Y * BSSN_BRICK_Z == 64, "a brick is one 64-lane wave");
+
 namespace bssn {
 
 // Halo handling of the stencil source (see bssn_brick.h).
@@ -27,6 +40,11 @@ enum Field : int {
   GTI0 = 18, GTI1 = 19, GTI2 = 20,
   BB0 = 21, BB1 = 22, BB2 = 23
 };
+
+// lane -> coordinates within the brick (z fastest), for the kernels that walk brick lists
+constexpr int brick_lx(int lane) { return lane / (BSSN_BRICK_Y * BSSN_BRICK_Z); }
+constexpr int brick_ly(int lane) { return (lane / BSSN_BRICK_Z) % BSSN_BRICK_Y; }
+constexpr int brick_lz(int lane) { return lane % BSSN_BRICK_Z; }
 
 // Physics scalars, unpacked from the reference's 16-double pack
 // (eta, C_CAHD, dt, dx_i, h_ssl, sig_ssl, t, lambda0..3, lambda_f0..1, dx, dy, dz;

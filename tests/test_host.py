@@ -229,11 +229,14 @@ def test_autotune_space_and_installed_choice():
     pkg = ROOT / "integer-schemes_amd"
     at = _load_module("bssn_autotune", pkg / "autotune.py")
     bd = _load_module("bssn_build", pkg / "build.py")
-    stage_h = "".join((pkg / "csrc" / h).read_text() for h in ("bssn_stage.h", "bssn_brick.h"))
+    stage_h = "".join((pkg / "csrc" / h).read_text()
+                      for h in ("bssn_stage.h", "bssn_brick.h", "bssn_params.h"))
     for space in at.SPACES.values():
         keys = [at.key_of(d) for d in space]
         assert len(set(keys)) == len(keys) and keys[0] == "default"
         for d in space:
+            shape = [d.get(f"BSSN_BRICK_{a}", 4) for a in "XYZ"]
+            assert shape[0] * shape[1] * shape[2] == 64
             for name, v in d.items():
                 assert f"#ifndef {name}" in stage_h, name
                 assert isinstance(v, int) and v >= 1
