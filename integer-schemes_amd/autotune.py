@@ -64,6 +64,8 @@ SPACES = {
         {"BSSN_BRICK_X": 2, "BSSN_BRICK_Y": 2, "BSSN_BRICK_Z": 16},
     ],
 }
+# the brick shapes alone (with the defaults as the baseline every space starts with)
+SPACES["bricks"] = [{}] + [d for d in SPACES["full"] if "BSSN_BRICK_X" in d]
 
 # Runs in the candidate's process: time the step and the RHS, compare the 48^3 step with the
 # installed library's result (written by the parent).
@@ -184,6 +186,17 @@ def choose(results: dict, margin: float = MARGIN, **info) -> dict:
             "best": best, "margin": margin, **info, "candidates": results}
 
 
+def merge(old: dict, new: dict) -> dict:
+    """One record from two runs on the same device and workload: every candidate of both (the
+    newer measurement where both have one, the newer defaults as the baseline), chosen again."""
+    if old.get("workload") != new.get("workload"):
+        raise ValueError("records of different workloads")
+    cands = dict(old.get("candidates", {}))
+    cands.update(new["candidates"])
+    info = {k: new[k] for k in ("workload", "device", "steps_timed") if k in new}
+    return choose(cands, margin=new.get("margin", MARGIN), **info)
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--space", default="full", choices=sorted(SPACES))
@@ -191,8 +204,13 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=6)
     ap.add_argument("--write", action="store_true", help="install the result as tuning.json")
     ap.add_argument("--out", default=None, help="also write the result here")
+    ap.add_argument("--merge", default=None,
+                    help="a record of earlier candidates (same device and workload) to choose "
+                         "among together with this run's")
     a = ap.parse_args(argv)
     res = tune(a.space, a.n, a.steps)
+    if a.merge:
+        res = merge(json.loads(Path(a.merge).read_text()), res)
     text = json.dumps(res, indent=1)
     print(text)
     if a.out:

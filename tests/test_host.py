@@ -257,3 +257,11 @@ def test_autotune_space_and_installed_choice():
     r["d"] = cand(39.0, 6.01, d={"BSSN_STAGE_DEPTH": 2})
     rec = at.choose(r)
     assert rec["best"] == "d" and rec["defines"] == ["BSSN_STAGE_DEPTH=2"]
+    # merging a second run: its candidates join, its defaults are the baseline
+    new = at.choose({"default": cand(39.5, 6.0), "e": cand(38.5, 5.9, d={"BSSN_BRICK_X": 2})},
+                    workload="w")
+    m = at.merge(dict(rec, workload="w"), new)
+    assert set(m["candidates"]) == {"default", "a", "b", "c", "d", "e"}
+    assert m["candidates"]["default"]["step_ms"] == 39.5 and m["best"] == "e"
+    with pytest.raises(ValueError):
+        at.merge(dict(rec, workload="other"), new)
