@@ -270,7 +270,8 @@ def bench_batched(reps: int) -> dict:
     {8, 12, 16, 24, 32}) on three nested levels (h = 1/4, 1/8, 1/16) around the 1:10 binary
     (m = 0.909, 0.0909, 4M apart); per RK stage one batched launch over every block, then the
     ghost fill program (root periodic, same-level copies, prolongation from the parent level).
-    The launch shape is autotuned; steps are timed eagerly and replayed from a captured graph."""
+    The launch shape is autotuned; steps are timed eagerly and replayed from a captured graph,
+    and again as linked steps (stages read copied ghosts at their sources)."""
     import torch
     from bssn3d_hip.batch import timed
     from bssn3d_hip.patches import config_hierarchy, fill_binary
@@ -312,6 +313,26 @@ def bench_batched(reps: int) -> dict:
     U.copy_(U0)
     ms_graph = timed(g.replay, reps, dev)
     assert bool(torch.isfinite(U).all()), "non-finite hierarchy state"
+    # linked steps: copied ghosts read at their sources, only the prolongations (and the copies
+    # their taps need) between stages; sync_end=False leaves the other ghosts to an explicit sync
+    linked = H.linked_program
+    H.link
+    lk = dict(kw, ghosts="linked")
+    U.copy_(U0)
+    H.step_(U, 0.0, dt, **lk)
+    torch.cuda.synchronize()
+    U.copy_(U0)
+    ms_link_synced = timed(lambda: H.step_(U, 0.0, dt, **lk), reps, dev)
+    U.copy_(U0)
+    ms_link = timed(lambda: H.step_(U, 0.0, dt, sync_end=False, **lk), reps, dev)
+    assert bool(torch.isfinite(U).all()), "non-finite hierarchy state"
+    U.copy_(U0)
+    tdev.fill_(0.0)
+    g = H.capture_steps(U, tdev, dt, 1, sync_end=False, **lk)
+    g.replay()
+    torch.cuda.synchronize()
+    U.copy_(U0)
+    ms_link_graph = timed(g.replay, reps, dev)
     pts = batch.interior_points
     return {"workload": "640_blocks_3_nested_levels_h1/4_1/8_1/16_sizes_8-32_masses_0.909_0.0909"
                         "_sep4",
@@ -326,7 +347,14 @@ def bench_batched(reps: int) -> dict:
             "steps_timed": reps,
             "step_ms": round(ms_step, 4), "step_mpts_s": round(pts / (ms_step * 1e-3) / 1e6, 2),
             "graph_step_ms": round(ms_graph, 4),
-            "graph_step_mpts_s": round(pts / (ms_graph * 1e-3) / 1e6, 2)}
+            "graph_step_mpts_s": round(pts / (ms_graph * 1e-3) / 1e6, 2),
+            "linked": {"program_copies": linked.n_copy, "prolongations": linked.n_prolong,
+                       "step_ms": round(ms_link, 4),
+                       "step_mpts_s": round(pts / (ms_link * 1e-3) / 1e6, 2),
+                       "graph_step_ms": round(ms_link_graph, 4),
+                       "graph_step_mpts_s": round(pts / (ms_link_graph * 1e-3) / 1e6, 2),
+                       "synced_step_ms": round(ms_link_synced, 4),
+                       "synced_step_mpts_s": round(pts / (ms_link_synced * 1e-3) / 1e6, 2)}}
 
 
 def bench_longrun() -> dict:

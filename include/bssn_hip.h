@@ -369,6 +369,26 @@ int bssn_batch_rk4_step_prog_f64(const bssn_batch* b, double* U, double* work, c
                                  double t, double dt, double chi_floor, double alpha_floor,
                                  double* tdev, void* stream);
 
+/* bssn_batch_rk4_step_prog_f64 without the ghost copies between stages: the stage kernels read
+ * a block's ghost cells where the link map points -- a same-level neighbour's (or, on the root
+ * level, the periodic image's) interior point -- and only the prolongations, with the copies
+ * their parent reads need (`linked`, a ghost fill program), refill ghosts after every stage and
+ * at the end. `full` (optional) runs once at the end, so that U leaves the step synced exactly
+ * as the program step leaves it; interior values equal the program step's bit for bit either
+ * way. link: device array of 2 ints per padded point, block b's points at [m_b, m_b + P_b) with
+ * m_b = P_0 + ... + P_(b-1) (blocks in the order given to bssn_batch_create,
+ * bssn_batch_points(b, 0) points in all): (pool offset of field 0 of the point to read, field
+ * stride of its block); read only at ghost points. Needs ng >= the stencils' reach and a pool
+ * below 2^31 doubles. Same batched step as the reference's slot-batched N-level step
+ * (2D/src/mcs2d/amr/evolve.py:659-775); the ghost exchange by reference instead of by copy
+ * has no counterpart there. */
+int bssn_batch_rk4_step_link_f64(const bssn_batch* b, double* U, double* work,
+                                 const double* S16, int fd_order, double ko_sigma,
+                                 const int* link, const bssn_ghost_program* linked,
+                                 const bssn_ghost_program* full, double t, double dt,
+                                 double chi_floor, double alpha_floor, double* tdev,
+                                 void* stream);
+
 /* Refinement indicator of every block of a batch: out[24 b + f] = the largest undivided
  * difference |u(p + e_a) - u(p)| of field f over the block's interior and one ring of ghosts,
  * all three axes (NaN if any is NaN); blocks in the order they were given to

@@ -95,6 +95,10 @@ SIGNATURES = {
     "bssn_batch_rk4_step_prog_f64": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _PD, _c_int,
                                               _c_double, _c_void_p, _c_double, _c_double,
                                               _c_double, _c_double, _c_void_p, _c_void_p]),
+    "bssn_batch_rk4_step_link_f64": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _PD, _c_int,
+                                              _c_double, _c_void_p, _c_void_p, _c_void_p,
+                                              _c_double, _c_double, _c_double, _c_double,
+                                              _c_void_p, _c_void_p]),
     "bssn_batch_indicator_f64": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_void_p]),
     "bssn_bfp_bytes": (ctypes.c_size_t, [_c_int, _c_ll]),
     "bssn_deriv_bundle_absmax_f64": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, _c_int,

@@ -19,6 +19,11 @@ from its parent, then from same-level neighbours (2D/src/mcs2d/amr/evolve.py:719
 3D/src/multipatch/amr/sync.py:84-122) -- for equal blocks in fixed slot arrays
 (amr/state.py:27-31); here the blocks are uneven and all of them step in one batched launch per
 RK stage (bssn_batch_rk4_step_prog_f64), with the program after every stage.
+
+Linked steps (bssn_batch_rk4_step_link_f64, ``step_(..., ghosts="linked")``) skip the copies: a
+link map sends each copied ghost point to its source, which the stage kernels read in place,
+and after every stage only the prolongations run, with the copies into the parent ghosts that
+their taps read (``linked_program``). Interior values equal the program step's bit for bit.
 """
 from __future__ import annotations
 
@@ -88,6 +93,8 @@ class PatchHierarchy:
         self._device = device
         self._batch = None
         self._prog = None
+        self._linked = None
+        self._link = None
 
     @property
     def batch(self) -> BlockBatch:
@@ -216,6 +223,96 @@ class PatchHierarchy:
             self._prog = DeviceProgram(pr, jb, cp, pe, be, ce, self.device)
         return self._prog
 
+    # linked steps --------------------------------------------------------------------------
+    def link_offsets(self) -> np.ndarray:
+        """m_b: first padded point of block b in the link map (P_0 + ... + P_(b-1))."""
+        P = np.asarray(self.layout.P, dtype=np.int64)
+        return np.concatenate([[0], np.cumsum(P)[:-1]])
+
+    def link_map(self, copies: np.ndarray | None = None) -> np.ndarray:
+        """(n_points, 2) int32: for every padded point of every block (block order, each block
+        row-major), the pool offset of field 0 of the point its value is read from and that
+        block's field stride -- the copy entry's source for a copied ghost, the point itself
+        otherwise (interior points, prolongated ghosts)."""
+        lay = self.layout
+        off = np.asarray(lay.off, dtype=np.int64)
+        P = np.asarray(lay.P, dtype=np.int64)
+        if lay.pool_doubles >= 2 ** 31:
+            raise ValueError("linked steps need a pool below 2^31 doubles")
+        m = self.link_offsets()
+        n = int(P.sum())
+        blk = np.repeat(np.arange(len(P)), P)
+        q = np.arange(n, dtype=np.int64) - m[blk]
+        out = np.empty((n, 2), dtype=np.int32)
+        out[:, 0] = off[blk] + q
+        out[:, 1] = P[blk]
+        if copies is None:
+            copies = self.program_entries()[2]
+        if len(copies):
+            dst = copies["dst"].astype(np.int64)
+            b = np.searchsorted(off, dst, side="right") - 1
+            i = m[b] + (dst - off[b])
+            out[i, 0] = copies["src"]
+            out[i, 1] = copies["src_stride"]
+        return out
+
+    @staticmethod
+    def prolong_taps(pr: np.ndarray, ng: int) -> np.ndarray:
+        """Pool offsets (field 0) of the parent ghost points that the prolongation entries read:
+        per axis coarse node fine / 2 (even fine) or (fine - 1) / 2 - 2 .. + 3 (odd), clamped to
+        the parent's padded block, as bssn_prolong_f64 reads them."""
+        found = []
+        fine = pr["fine"].astype(np.int64)
+        pS = pr["pS"].astype(np.int64)
+        par = pr["parent"].astype(np.int64)
+        cls = (fine[:, 0] & 1) << 2 | (fine[:, 1] & 1) << 1 | (fine[:, 2] & 1)
+        for c in range(8):
+            sel = np.flatnonzero(cls == c)
+            for lo in range(0, len(sel), 1 << 15):
+                e = sel[lo:lo + (1 << 15)]
+                taps, ghost = [], []
+                for a in range(3):
+                    f, S = fine[e, a], pS[e, a]
+                    if (c >> (2 - a)) & 1:
+                        t = (f[:, None] - 1) // 2 + np.arange(-2, 4)[None, :]
+                    else:
+                        t = (f // 2)[:, None]
+                    t = np.clip(t, 0, S[:, None] - 1)
+                    taps.append(t)
+                    ghost.append((t < ng) | (t >= S[:, None] - ng))
+                tx, ty, tz = taps
+                S1, S2 = pS[e, 1][:, None, None, None], pS[e, 2][:, None, None, None]
+                flat = par[e][:, None, None, None] + (tx[:, :, None, None] * S1
+                                                      + ty[:, None, :, None]) * S2 \
+                    + tz[:, None, None, :]
+                g = ghost[0][:, :, None, None] | ghost[1][:, None, :, None] | \
+                    ghost[2][:, None, None, :]
+                found.append(flat[g])
+        return np.unique(np.concatenate(found)) if found else np.empty(0, np.int64)
+
+    def linked_program_entries(self):
+        """The program of a linked step: every prolongation, and only the copies whose
+        destination a prolongation reads, in the program's phases."""
+        pr, jb, cp, pe, be, ce = self.program_entries()
+        need = np.isin(cp["dst"].astype(np.int64), self.prolong_taps(pr, self.ng))
+        kept = np.cumsum(need)
+        ce2 = [int(kept[e - 1]) if e > 0 else 0 for e in ce]
+        return pr, jb, cp[need], pe, be, ce2
+
+    @property
+    def linked_program(self) -> DeviceProgram:
+        if self._linked is None:
+            pr, jb, cp, pe, be, ce = self.linked_program_entries()
+            self._linked = DeviceProgram(pr, jb, cp, pe, be, ce, self.device)
+        return self._linked
+
+    @property
+    def link(self) -> torch.Tensor:
+        """The link map on the device."""
+        if self._link is None:
+            self._link = torch.from_numpy(self.link_map()).to(self.device)
+        return self._link
+
     # operators -----------------------------------------------------------------------------
     def empty_pool(self) -> torch.Tensor:
         return self.batch.empty_pool()
@@ -230,9 +327,13 @@ class PatchHierarchy:
         return U
 
     def step_(self, U: torch.Tensor, t: float, dt: float, params: PhysicsParams | None = None,
-              order: int = 4, ko_sigma: float = 0.1, tdev: torch.Tensor | None = None):
+              order: int = 4, ko_sigma: float = 0.1, tdev: torch.Tensor | None = None,
+              ghosts: str = "program", sync_end: bool = True):
         """One shared-dt RK4 step of every block in place; U must be synced on entry (every
-        step leaves it synced)."""
+        step leaves it synced). ghosts="linked": stages read copied ghosts at their sources
+        (bssn_batch_rk4_step_link_f64); with sync_end=False the step leaves only the ghosts the
+        next linked step reads (prolongated ones and the parents' ghosts they need) up to
+        date."""
         params = params or PhysicsParams()
         b = self.batch
         b._check_pool(U)
@@ -241,6 +342,16 @@ class PatchHierarchy:
             if tdev.dtype != torch.float64 or tdev.device != self.device or tdev.numel() != 1:
                 raise ValueError("tdev must be one float64 value on the hierarchy's device")
             tptr = tdev.data_ptr()
+        if ghosts == "linked":
+            _lib.check(_lib.load().bssn_batch_rk4_step_link_f64(
+                b._handle, U.data_ptr(), b.workspace().data_ptr(),
+                _lib.dbl(b.S16(params, dt, t)), int(order), float(ko_sigma),
+                self.link.data_ptr(), self.linked_program.ptr,
+                self.program.ptr if sync_end else None, float(t), float(dt), params.chi_floor,
+                params.alpha_floor, tptr, stream_handle(self.device)))
+            return U
+        if ghosts != "program":
+            raise ValueError(f"ghosts must be 'program' or 'linked', not {ghosts!r}")
         _lib.check(_lib.load().bssn_batch_rk4_step_prog_f64(
             b._handle, U.data_ptr(), b.workspace().data_ptr(), _lib.dbl(b.S16(params, dt, t)),
             int(order), float(ko_sigma), self.program.ptr, float(t), float(dt),
@@ -253,6 +364,8 @@ class PatchHierarchy:
         replay)."""
         self.batch.workspace()
         prog = self.program
+        if kw.get("ghosts") == "linked":
+            prog = (prog, self.linked_program, self.link)
         scratch = U.clone()
         self.step_(scratch, 0.0, dt, **kw)
         torch.cuda.synchronize(self.device)

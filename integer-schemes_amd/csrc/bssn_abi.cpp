@@ -841,6 +841,7 @@ int bssn_batch_create(bssn_batch** out, void* plan, size_t plan_bytes, int nblk,
     BlockDesc& d = desc[(size_t)b];
     d.off = (long)off[b];
     d.doff = (long)doff;
+    d.moff = (long)padded;
     doff += (((long long)BSSN_ND * s[0] * s[1] * s[2]) + 63) / 64 * 64;
     for (int k = 0; k < 5; ++k)
       pre[(size_t)k * (nblk + 1) + b + 1] = pre[(size_t)k * (nblk + 1) + b] + i8_pass_tiles(k, s);
@@ -994,7 +995,8 @@ namespace {
 template <class Sync>
 int batch_rk4(const bssn_batch* b, double* U, double* work, const double* S16, int fd_order,
               double ko_sigma, int bc, Sync sync, bool hold, double t, double dt,
-              double chi_floor, double alpha_floor, double* tdev, hipStream_t st) {
+              double chi_floor, double alpha_floor, double* tdev, hipStream_t st,
+              const int* link = nullptr) {
   TRY(check_batch(b));
   if (!U || !work) return fail(BSSN_E_ARG, "null pointer");
   if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4, 6 or 8");
@@ -1007,7 +1009,7 @@ int batch_rk4(const bssn_batch* b, double* U, double* work, const double* S16, i
   double* Yb = work + NS;
   double* KS = work + 2 * NS;
   const bool interior = bc != 0;
-  const int mode = bc == 1 ? WRAP : CLAMP;
+  const int mode = link ? LINK : (bc == 1 ? WRAP : CLAMP);
   const bool ko = ko_sigma > 0.0;
   const double hdt = 0.5 * dt;
   StageArgs s{};
@@ -1019,7 +1021,8 @@ int batch_rk4(const bssn_batch* b, double* U, double* work, const double* S16, i
   s.alpha_floor = alpha_floor;
   s.tdev = tdev;
   auto stage = [&](double toff, int kind, double c0, const double* yin, double* y) {
-    const BatchArgs a = batch_args(b, interior, S16, dt, t + toff, ko_sigma);
+    BatchArgs a = batch_args(b, interior, S16, dt, t + toff, ko_sigma);
+    a.link = link;
     s.toff = toff;
     s.Yin = yin;
     s.Y = y;
@@ -1131,6 +1134,29 @@ int bssn_batch_rk4_step_prog_f64(const bssn_batch* b, double* U, double* work, c
   auto sync = [&](double* pool) { return run_program(pool, b->pool_doubles, prog, st); };
   return batch_rk4(b, U, work, S16, fd_order, ko_sigma, 2, sync, false, t, dt, chi_floor,
                    alpha_floor, tdev, st);
+}
+
+int bssn_batch_rk4_step_link_f64(const bssn_batch* b, double* U, double* work,
+                                 const double* S16, int fd_order, double ko_sigma,
+                                 const int* link, const bssn_ghost_program* linked,
+                                 const bssn_ghost_program* full, double t, double dt,
+                                 double chi_floor, double alpha_floor, double* tdev,
+                                 void* stream) {
+  TRY(check_batch(b));
+  if (!link) return fail(BSSN_E_ARG, "null link map");
+  TRY(check_program(linked));
+  if (full) TRY(check_program(full));
+  if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4, 6 or 8");
+  const int reach = ko_sigma > 0.0 ? std::max(fd_order, ko_order_of(fd_order)) / 2 : fd_order / 2;
+  if (b->ng < reach) return fail(BSSN_E_ARG, "linked steps need ng >= the stencils' reach");
+  if (b->pool_doubles >= (1LL << 31))
+    return fail(BSSN_E_ARG, "linked steps need a pool below 2^31 doubles");
+  hipStream_t st = (hipStream_t)stream;
+  auto sync = [&](double* pool) { return run_program(pool, b->pool_doubles, linked, st); };
+  TRY(batch_rk4(b, U, work, S16, fd_order, ko_sigma, 2, sync, false, t, dt, chi_floor,
+                alpha_floor, tdev, st, link));
+  if (full) TRY(run_program(U, b->pool_doubles, full, st));
+  return BSSN_OK;
 }
 
 int bssn_batch_indicator_f64(const bssn_batch* b, const double* U, double* out, void* stream) {

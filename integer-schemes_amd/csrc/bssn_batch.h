@@ -90,9 +90,16 @@ __device__ __forceinline__ void batch_brick(double* lds, const StageArgs& args,
   a.K1 = shifted(args.K1, blk.off);
   a.FP = shifted(args.FP, blk.off);
   a.Uout = shifted(args.Uout, blk.off);
-  WaveSrc<ORD, KO, MODE, RhsOrder, fused_depth<KIND, ORD>()> src(lds, a.Yin, w, d, ox, oy, oz, lane);
-  StageOut<KIND> out(a, d.P, src.gc, lane_valid<G>(d, ox, oy, oz, lane), true);
-  bssn_rhs_point<KO>(src, sc, out);
+  using Src = WaveSrc<ORD, KO, MODE, RhsOrder, fused_depth<KIND, ORD>()>;
+  if constexpr (MODE == LINK) {
+    Src src(lds, args.Yin, blk.off, b.link + 2 * blk.moff, w, d, ox, oy, oz, lane);
+    StageOut<KIND> out(a, d.P, src.gc, lane_valid<G>(d, ox, oy, oz, lane), true);
+    bssn_rhs_point<KO>(src, sc, out);
+  } else {
+    Src src(lds, a.Yin, w, d, ox, oy, oz, lane);
+    StageOut<KIND> out(a, d.P, src.gc, lane_valid<G>(d, ox, oy, oz, lane), true);
+    bssn_rhs_point<KO>(src, sc, out);
+  }
 }
 
 template <int ORD, bool KO, int MODE, bool PERSIST, int KIND>
@@ -135,10 +142,13 @@ static hipError_t launch_batch_ord(const StageArgs& a, const BatchArgs& b, int g
                                    int mode, hipStream_t st) {
   return with_kind<false>(a.kind, [&](auto k) -> hipError_t {
     constexpr int K = decltype(k)::value;
-    if (ko) return mode == WRAP ? launch_batch_t<ORD, true, WRAP, K>(a, b, grid, st)
-                                : launch_batch_t<ORD, true, CLAMP, K>(a, b, grid, st);
-    return mode == WRAP ? launch_batch_t<ORD, false, WRAP, K>(a, b, grid, st)
-                        : launch_batch_t<ORD, false, CLAMP, K>(a, b, grid, st);
+    if (ko)
+      return mode == WRAP   ? launch_batch_t<ORD, true, WRAP, K>(a, b, grid, st)
+             : mode == LINK ? launch_batch_t<ORD, true, LINK, K>(a, b, grid, st)
+                            : launch_batch_t<ORD, true, CLAMP, K>(a, b, grid, st);
+    return mode == WRAP   ? launch_batch_t<ORD, false, WRAP, K>(a, b, grid, st)
+           : mode == LINK ? launch_batch_t<ORD, false, LINK, K>(a, b, grid, st)
+                          : launch_batch_t<ORD, false, CLAMP, K>(a, b, grid, st);
   });
 }
 

@@ -309,9 +309,11 @@ struct WaveSrc : BoxStencil<Geo<ORD, KO>, Coef<ORD>> {
   using C = Coef<ORD>;
   using BoxStencil<G, C>::lds;
   using BoxStencil<G, C>::c;
-  const double* F;      // stencil source state
+  const double* F;      // stencil source state (LINK: the pool)
+  const double* Fc;     // the block's state (centre values)
   long P;
-  int off[G::PER];      // grid offsets of this lane's staged cells
+  int off[G::PER];      // grid offsets of this lane's staged cells (LINK: pool offsets)
+  int psrc[MODE == LINK ? G::PER : 1];   // LINK: field stride of each staged cell's block
   int cel[G::PER];      // box indices of this lane's staged cells
   double pf[DEPTH][G::PER];   // the next DEPTH fields' cells, in flight (slot = position % DEPTH)
   long gc;              // flat index of the lane's point (clamped into the grid)
@@ -319,7 +321,8 @@ struct WaveSrc : BoxStencil<Geo<ORD, KO>, Coef<ORD>> {
 
   __device__ __forceinline__ WaveSrc(double* lds_, const double* F_, const Scales& sc_,
                                      const Dims& d, int ox, int oy, int oz, int lane)
-      : BoxStencil<G, C>(lds_, sc_), F(F_), P(d.P) {
+      : BoxStencil<G, C>(lds_, sc_), F(F_), Fc(F_), P(d.P) {
+    static_assert(MODE != LINK, "LINK sources take the pool and the link map");
     const int tx = G::lx(lane), ty = G::ly(lane), tz = G::lz(lane);
     c = (tx + G::H) * G::SX + (ty + G::H) * G::SY + (tz + G::H);
     gc = lane_point<G>(d, ox, oy, oz, lane);
@@ -339,6 +342,45 @@ struct WaveSrc : BoxStencil<Geo<ORD, KO>, Coef<ORD>> {
       if (i < ORDER::N) fetch(i, ORDER::at(i));
   }
 
+  // LINK: a block of a batch (pool offset boff, link map lmap of its padded points). Cells in
+  // the block's interior are read in place; a ghost cell is read where the map points. Needs
+  // ng >= H (the box stays inside the padded block) and pool offsets below 2^31.
+  __device__ __forceinline__ WaveSrc(double* lds_, const double* pool, long boff,
+                                     const int* lmap, const Scales& sc_, const Dims& d, int ox,
+                                     int oy, int oz, int lane)
+      : BoxStencil<G, C>(lds_, sc_), F(pool), Fc(pool + boff), P(d.P) {
+    static_assert(MODE == LINK, "the link constructor is for LINK sources");
+    const int tx = G::lx(lane), ty = G::ly(lane), tz = G::lz(lane);
+    c = (tx + G::H) * G::SX + (ty + G::H) * G::SY + (tz + G::H);
+    gc = lane_point<G>(d, ox, oy, oz, lane);
+    const int o[3] = {ox, oy, oz};
+#pragma unroll
+    for (int k = 0; k < G::PER; ++k) {
+      const int e = kCells<G>.c[lane + k * G::NT];
+      const int cc[3] = {e & 31, (e >> 5) & 31, (e >> 10) & 31};
+      cel[k] = cc[0] * G::SX + cc[1] * G::SY + cc[2];
+      int g[3];
+      bool inner = true;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        g[a] = o[a] + cc[a] - G::H;
+        inner = inner && g[a] >= d.ng && g[a] < d.S[a] - d.ng;
+      }
+      const int q = (g[0] * d.S[1] + g[1]) * d.S[2] + g[2];
+      if (inner) {
+        off[k] = (int)boff + q;
+        psrc[k] = (int)d.P;
+      } else {
+        const int2 m = reinterpret_cast<const int2*>(lmap)[q];
+        off[k] = m.x;
+        psrc[k] = m.y;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < DEPTH; ++i)
+      if (i < ORDER::N) fetch(i, ORDER::at(i));
+  }
+
   // staged slots of field f: the edge bars only for fields with mixed derivatives
   __device__ __forceinline__ static int slots(int f) {
     return needs_edges(f) ? G::PER : G::PER_FACES;
@@ -348,8 +390,13 @@ struct WaveSrc : BoxStencil<Geo<ORD, KO>, Coef<ORD>> {
     const double* u = F + (long)f * P;
     const int n = slots(f);
 #pragma unroll
-    for (int k = 0; k < G::PER; ++k)
-      if (k < n) pf[slot][k] = u[off[k]];
+    for (int k = 0; k < G::PER; ++k) {
+      if constexpr (MODE == LINK) {
+        if (k < n) pf[slot][k] = F[(long)f * psrc[k] + off[k]];
+      } else {
+        if (k < n) pf[slot][k] = u[off[k]];
+      }
+    }
     // keep the loads here, ahead of the work they overlap
     __builtin_amdgcn_sched_barrier(0);
   }
@@ -380,7 +427,7 @@ struct WaveSrc : BoxStencil<Geo<ORD, KO>, Coef<ORD>> {
     if (at + DEPTH < ORDER::N) fetch(SLOT, ORDER::at(at + DEPTH));
   }
 
-  __device__ __forceinline__ double center(int f) const { return F[(long)f * P + gc]; }
+  __device__ __forceinline__ double center(int f) const { return Fc[(long)f * P + gc]; }
 };
 
 // The ghost points of a padded block, enumerated without the interior: x-ghost planes

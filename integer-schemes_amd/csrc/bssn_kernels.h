@@ -93,6 +93,7 @@ struct BlockDesc {
   int ng;         // ghost width
   double h[3];    // spacings
   long doff;      // offset of the block's 138 derivative arrays in a batch bundle, in doubles
+  long moff;      // first padded point of the block in a link map (sum of the earlier blocks' P)
 };
 
 // One brick of one block: the block index and the brick origin in the block's padded
@@ -119,6 +120,7 @@ struct BatchArgs {
   int interior;      // the list covers block interiors (else whole padded blocks)
   Scal base;         // shared scalars; cahd is set per block from its spacing
   double cahd_c, dt, sigma;
+  const int* link;   // LINK mode: (source pool offset, source field stride) per padded point
 };
 
 hipError_t launch_fused_batch(const StageArgs& a, const BatchArgs& b, int grid, int ord, bool ko,

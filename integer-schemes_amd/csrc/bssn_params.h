@@ -30,7 +30,10 @@ static_assert(BSSN_BRICK_X * BSSN_BRICK_Y * BSSN_BRICK_Z == 64, "a brick is one 
 namespace bssn {
 
 // Halo handling of the stencil source (see bssn_brick.h).
-enum Mode : int { CLAMP = 0, WRAP = 1 };
+// LINK (batched blocks only): cells that are ghosts of the block are read where the link map
+// points (a neighbouring block's interior point, or the block's own ghost where a prolongation
+// fills it), so a step needs no ghost copies between its stages.
+enum Mode : int { CLAMP = 0, WRAP = 1, LINK = 2 };
 
 enum Field : int {
   ALPHA = 0, CHI = 1, KTR = 2,

@@ -14,7 +14,11 @@ Bars:
     batched RHS + KO at the interior points, the RK4 combinations in torch, enforce), per block
     and field within 1e-12 normwise (the fused update rounds with FMAs);
   * steps captured into a HIP graph and replayed (time on the device), with eager work between
-    the replays, equal eager steps to 1e-12 (the device's exp forms the SSL factor).
+    the replays, equal eager steps to 1e-12 (the device's exp forms the SSL factor);
+  * a linked step (ghost copies read at their sources, only the prolongations and the copies
+    they need between stages) equals the program step bit for bit on every block -- ghosts
+    included when it ends with the full program, interiors when it does not -- and captured
+    linked steps replay to the eager ones.
 """
 from __future__ import annotations
 
@@ -190,4 +194,66 @@ def test_graph_replay_equals_eager(hier):
         worst = max(worst, float(normwise(H.view(U, b).cpu().numpy(),
                                           H.view(V, b).cpu().numpy()).max()))
     print(f"graph vs eager {worst:.2e}")
+    assert worst <= 1e-12
+
+
+def _blocks_equal(H, A, B, interior_only=False):
+    ng = H.ng
+    for b in range(len(H)):
+        a, c = H.view(A, b), H.view(B, b)
+        if interior_only:
+            a, c = a[:, ng:-ng, ng:-ng, ng:-ng], c[:, ng:-ng, ng:-ng, ng:-ng]
+        if not torch.equal(a, c):
+            return b
+    return None
+
+
+def test_linked_step_equals_program_step(hier):
+    H, _ = hier
+    say("640-block hierarchy: linked steps vs program steps")
+    U = fill_binary(H, H.empty_pool())
+    p = PhysicsParams()
+    dt, t = 0.25 * H.boxes[-1].h, 0.15
+    A, B = U.clone(), U.clone()
+    H.step_(A, t, dt, p, order=4, ko_sigma=0.1)
+    H.step_(B, t, dt, p, order=4, ko_sigma=0.1, ghosts="linked")
+    torch.cuda.synchronize()
+    assert _blocks_equal(H, A, B) is None
+    # a linked step that leaves the copied ghosts stale: the next linked step reads none of them
+    H.step_(A, t + dt, dt, p, order=4, ko_sigma=0.1)
+    H.step_(A, t + 2 * dt, dt, p, order=4, ko_sigma=0.1)
+    H.step_(B, t + dt, dt, p, order=4, ko_sigma=0.1, ghosts="linked", sync_end=False)
+    H.step_(B, t + 2 * dt, dt, p, order=4, ko_sigma=0.1, ghosts="linked", sync_end=False)
+    torch.cuda.synchronize()
+    assert _blocks_equal(H, A, B, interior_only=True) is None
+    H.sync_(B)
+    torch.cuda.synchronize()
+    assert _blocks_equal(H, A, B) is None
+
+
+def test_linked_graph_replay_equals_eager(hier):
+    H, _ = hier
+    say("640-block hierarchy: captured linked steps replayed")
+    U = fill_binary(H, H.empty_pool())
+    V = U.clone()
+    p = PhysicsParams()
+    dt, t0 = 0.25 * H.boxes[-1].h, 0.05
+    tdev = torch.full((1,), t0, dtype=torch.float64, device=DEV)
+    g = H.capture_steps(U, tdev, dt, 2, params=p, order=4, ko_sigma=0.1, ghosts="linked",
+                        sync_end=False)
+    for r in range(2):
+        g.replay()
+        torch.cuda.synchronize()
+        for k in range(2):
+            H.step_(V, t0 + (2 * r + k) * dt, dt, p, order=4, ko_sigma=0.1, ghosts="linked",
+                    sync_end=False)
+        torch.cuda.synchronize()
+    assert abs(float(tdev.item()) - (t0 + 4 * dt)) < 1e-14
+    worst = 0.0
+    ng = H.ng
+    for b in range(len(H)):
+        a = H.view(U, b)[:, ng:-ng, ng:-ng, ng:-ng].cpu().numpy()
+        c = H.view(V, b)[:, ng:-ng, ng:-ng, ng:-ng].cpu().numpy()
+        worst = max(worst, float(normwise(a, c).max()))
+    print(f"linked graph vs eager {worst:.2e}")
     assert worst <= 1e-12

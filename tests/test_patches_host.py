@@ -92,3 +92,51 @@ def test_program_reproduces_polynomials():
             err = np.abs(v[k] - (k + 1) * want[b]).max() / ((k + 1) * scale)
             tol = 0.0 if H.level[b] == 0 else 1e-13
             assert err <= tol, (b, H.level[b], k, err)
+
+
+def test_linked_program_and_link_map():
+    """A linked step reads a copied ghost at its source and refills only what the
+    prolongations need. On a small three-level hierarchy whose interiors hold seeded values:
+    the link map sends every copied ghost to its source and every other point to itself; the
+    linked program (all prolongations, only the copies into parent ghosts that their taps read)
+    gives every prolongated ghost exactly the value the full program gives it; and the copies
+    it leaves out are not read by any prolongation (filling them with NaN changes nothing)."""
+    H, _ = config_hierarchy(seed=5, counts=((4, 4, 4), (3, 3, 3), (2, 2, 2)), sizes=(8, 12),
+                            device="cpu", hold_punctures=False)
+    lay = H.layout
+    ng = H.ng
+    pr, jb, cp, pe, be, ce = H.program_entries()
+    lpr, ljb, lcp, lpe, lbe, lce = H.linked_program_entries()
+    assert len(lpr) == len(pr) and 0 < len(lcp) < len(cp)
+    assert lce[-1] == len(lcp) and all(a <= b for a, b in zip(lce, lce[1:]))
+    # link map
+    L = H.link_map(cp)
+    m = H.link_offsets()
+    off = np.asarray(lay.off)
+    assert L.shape == (int(np.sum(lay.P)), 2)
+    b = np.searchsorted(off, cp["dst"], side="right") - 1
+    i = m[b] + (cp["dst"] - off[b])
+    assert np.array_equal(L[i, 0], cp["src"]) and np.array_equal(L[i, 1], cp["src_stride"])
+    rest = np.ones(len(L), bool)
+    rest[i] = False
+    blk = np.repeat(np.arange(len(lay.P)), lay.P)
+    q = np.arange(len(L)) - m[blk]
+    assert np.array_equal(L[rest, 0], (off[blk] + q)[rest])
+    assert np.array_equal(L[rest, 1], np.asarray(lay.P)[blk][rest])
+    # the linked program reproduces every prolongated ghost
+    rng = np.random.default_rng(2)
+    base = np.full(lay.pool_doubles, np.nan)
+    for k, blk_ in enumerate(H.blocks):
+        v = lay.view(base, k)
+        v[:, ng:-ng, ng:-ng, ng:-ng] = rng.standard_normal(v[:, ng:-ng, ng:-ng, ng:-ng].shape)
+    full = base.copy()
+    run_program(full, pr, jb, cp, pe, be, ce)
+    linked = base.copy()
+    run_program(linked, lpr, ljb, lcp, lpe, lbe, lce)
+    for e in (pr["dst"],):
+        idx = (e[:, None] + np.arange(NF)[None, :] * pr["dst_stride"][:, None]).ravel()
+        assert not np.isnan(full[idx]).any()
+        assert np.array_equal(linked[idx], full[idx])
+    # copies left out hold NaN in the linked pool and no prolongated ghost became NaN
+    skipped = np.setdiff1d(cp["dst"], lcp["dst"])
+    assert np.isnan(linked[skipped]).all()
