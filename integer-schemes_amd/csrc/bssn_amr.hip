@@ -48,63 +48,63 @@ __constant__ double kWMid[6] = {3.0 / 256.0, -25.0 / 256.0, 150.0 / 256.0,
 
 }  // namespace
 
-// one destination point, all 24 fields, with the tap counts of its parity class fixed at
-// compile time (OX: x fine index odd, ...)
+// one field of one destination point, with the tap counts of its parity class fixed at compile
+// time (OX: x fine index odd, ...)
 template <bool OX, bool OY, bool OZ>
-__device__ __forceinline__ void prolong_point(double* __restrict__ pool,
-                                              const bssn_prolong_point& m) {
+__device__ __forceinline__ void prolong_field(double* __restrict__ pool,
+                                              const bssn_prolong_point& m, int f) {
 #pragma clang fp contract(off)
-  constexpr int NX = OX ? 6 : 1, NY = OY ? 6 : 1, NZ = OZ ? 6 : 1;
+  constexpr int NY = OY ? 6 : 1, NZ = OZ ? 6 : 1;
   const Taps tx = taps(m.fine[0], m.pS[0]);
   const Taps ty = taps(m.fine[1], m.pS[1]);
   const Taps tz = taps(m.fine[2], m.pS[2]);
   const long sy = m.pS[2], sx = (long)m.pS[1] * m.pS[2];
   const long P = sx * m.pS[0];
-#pragma unroll 1
-  for (int f = 0; f < BSSN_NF; ++f) {
-    const double* c = pool + m.parent + (long)f * P;
-    double vz = 0.0;
+  const double* c = pool + m.parent + (long)f * P;
+  double vz = 0.0;
 #pragma unroll
-    for (int kz = 0; kz < NZ; ++kz) {
-      double vy = 0.0;
+  for (int kz = 0; kz < NZ; ++kz) {
+    double vy = 0.0;
 #pragma unroll
-      for (int ky = 0; ky < NY; ++ky) {
-        const double* row = c + (long)ty.idx[ky] * sy + tz.idx[kz];
-        double vx;
-        if constexpr (OX) {
-          vx = 0.0;
+    for (int ky = 0; ky < NY; ++ky) {
+      const double* row = c + (long)ty.idx[ky] * sy + tz.idx[kz];
+      double vx;
+      if constexpr (OX) {
+        vx = 0.0;
 #pragma unroll
-          for (int kx = 0; kx < 6; ++kx) vx = vx + kWMid[kx] * row[(long)tx.idx[kx] * sx];
-        } else {
-          vx = row[(long)tx.idx[0] * sx];
-        }
-        if constexpr (OY) vy = vy + kWMid[ky] * vx; else vy = vx;
+        for (int kx = 0; kx < 6; ++kx) vx = vx + kWMid[kx] * row[(long)tx.idx[kx] * sx];
+      } else {
+        vx = row[(long)tx.idx[0] * sx];
       }
-      if constexpr (OZ) vz = vz + kWMid[kz] * vy; else vz = vy;
+      if constexpr (OY) vy = vy + kWMid[ky] * vx; else vy = vx;
     }
-    pool[m.dst + (long)f * m.dst_stride] = vz;
+    if constexpr (OZ) vz = vz + kWMid[kz] * vy; else vz = vy;
   }
+  pool[m.dst + (long)f * m.dst_stride] = vz;
 }
 
-// One thread per entry. Entries sorted by parity class keep a wave on one branch. An entry
+// One thread per entry and field (blockIdx.y = field): the up to 6^3 taps of a point are
+// independent loads, and 24 threads per point keep many more of them in flight than one thread
+// looping over the fields. Entries sorted by parity class keep a wave on one branch. An entry
 // that would reach outside the pool (pool_n doubles) is skipped.
 __global__ __launch_bounds__(256) void k_prolong(double* __restrict__ pool,
                                                  const bssn_prolong_point* __restrict__ pts,
                                                  long n, long long pool_n) {
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = (int)blockIdx.y;
   if (e >= n) return;
   const bssn_prolong_point m = pts[e];
   if (!prolong_entry_ok(m, pool_n)) return;
   const int cls = (m.fine[0] & 1) << 2 | (m.fine[1] & 1) << 1 | (m.fine[2] & 1);
   switch (cls) {
-    case 0: prolong_point<false, false, false>(pool, m); break;
-    case 1: prolong_point<false, false, true>(pool, m); break;
-    case 2: prolong_point<false, true, false>(pool, m); break;
-    case 3: prolong_point<false, true, true>(pool, m); break;
-    case 4: prolong_point<true, false, false>(pool, m); break;
-    case 5: prolong_point<true, false, true>(pool, m); break;
-    case 6: prolong_point<true, true, false>(pool, m); break;
-    default: prolong_point<true, true, true>(pool, m); break;
+    case 0: prolong_field<false, false, false>(pool, m, f); break;
+    case 1: prolong_field<false, false, true>(pool, m, f); break;
+    case 2: prolong_field<false, true, false>(pool, m, f); break;
+    case 3: prolong_field<false, true, true>(pool, m, f); break;
+    case 4: prolong_field<true, false, false>(pool, m, f); break;
+    case 5: prolong_field<true, false, true>(pool, m, f); break;
+    case 6: prolong_field<true, true, false>(pool, m, f); break;
+    default: prolong_field<true, true, true>(pool, m, f); break;
   }
 }
 
@@ -112,7 +112,7 @@ hipError_t launch_prolong(double* pool, long long pool_n, const void* pts, long 
                           hipStream_t st) {
   if (n <= 0) return hipSuccess;
   const long blocks = (n + 255) / 256;
-  hipLaunchKernelGGL(k_prolong, dim3((unsigned)blocks), dim3(256), 0, st, pool,
+  hipLaunchKernelGGL(k_prolong, dim3((unsigned)blocks, BSSN_NF), dim3(256), 0, st, pool,
                      (const bssn_prolong_point*)pts, n, pool_n);
   return hipGetLastError();
 }
