@@ -233,13 +233,36 @@ typedef struct {
   int BS, ng;
 } bssn_prolong_block;
 
+/* Prolongation points grouped by fine tile: the points of one parent whose fine indices lie in
+ * the 4 x 4 x 4 tile starting at origin (multiples of 4 in the parent's 2:1 refinement), with
+ * the same values as their bssn_prolong_point entries, bit for bit. The tile's 7^3 coarse
+ * window is read once and the x, y and z passes of the tensor-product prolongation are formed
+ * in LDS for the whole tile, so neighbouring points share their taps. Points
+ * [first, first + count) of the tile point array belong to the tile. */
+typedef struct {
+  long long parent, first;
+  int pS[3];
+  int origin[3];
+  int count;
+  int pad;
+} bssn_prolong_tile;
+
+/* One point of a tile: destination (pool offset of field 0), its block's field stride, and its
+ * fine index within the tile, (x * 4 + y) * 4 + z. */
+typedef struct {
+  long long dst;
+  int dst_stride;
+  int local;
+} bssn_prolong_tile_point;
+
 /* A ghost fill program for a block hierarchy: phases run in order; phase p applies the
- * prolongation entries [prolong_end[p-1], prolong_end[p]), the block jobs
- * [block_end[p-1], block_end[p]) and the copies [copy_end[p-1], copy_end[p]) (ends cumulative,
- * host arrays of nphase entries; the entry arrays are device memory; any of the three may be
- * empty). Within a phase the destinations must be disjoint and no entry may read a point the
- * phase writes; a phase may read what earlier phases wrote (a level's prolongation reads its
- * parent level's ghosts). */
+ * prolongation entries [prolong_end[p-1], prolong_end[p]), the prolongation tiles
+ * [tile_end[p-1], tile_end[p]), the block jobs [block_end[p-1], block_end[p]) and the copies
+ * [copy_end[p-1], copy_end[p]) (ends cumulative, host arrays of nphase entries; the entry arrays
+ * are device memory; any may be empty, and tile_end / tiles / tile_points may be null for a
+ * program without tiles). Within a phase the destinations must be disjoint and no entry may
+ * read a point the phase writes; a phase may read what earlier phases wrote (a level's
+ * prolongation reads its parent level's ghosts). */
 typedef struct {
   int nphase;
   const long long* prolong_end;
@@ -248,6 +271,10 @@ typedef struct {
   const bssn_prolong_point* prolongs;
   const bssn_prolong_block* blocks;
   const bssn_halo_copy* copies;
+  const long long* tile_end;
+  const bssn_prolong_tile* tiles;
+  const bssn_prolong_tile_point* tile_points;
+  long long n_tile_points;          /* length of tile_points */
 } bssn_ghost_program;
 
 /* Plan size for nblk blocks of padded extents S (3 ints per block); 0 for invalid extents. */
@@ -302,9 +329,12 @@ int bssn_batch_halo_f64(double* pool, long long pool_doubles, const bssn_halo_co
                         long long n, void* stream);
 
 /* How many of n entries (a host array; kind 0 = bssn_halo_copy, 1 = bssn_prolong_point,
- * 2 = bssn_prolong_block) the device kernels would skip on a pool of pool_doubles doubles because
- * they reach outside it (for a block job also: padded extent above 24 or an octant index other
- * than 0 or 1); -1 for invalid arguments. The kernels run the same test per entry. */
+ * 2 = bssn_prolong_block, 3 = bssn_prolong_tile, 4 = bssn_prolong_tile_point) the device
+ * kernels would skip on a pool of pool_doubles doubles because they reach outside it (for a
+ * block job also: padded extent above 24 or an octant index other than 0 or 1; for a tile: an
+ * origin not a multiple of 4 or a count outside 0..4096; for a tile point: a local index outside
+ * 0..63); -1 for invalid arguments. The kernels run the same test per entry (a tile's points
+ * with its tile). */
 long long bssn_count_skipped_entries(int kind, const void* entries, long long n,
                                      long long pool_doubles);
 

@@ -118,7 +118,11 @@ class GhostProgram(ctypes.Structure):
                 ("copy_end", ctypes.POINTER(ctypes.c_longlong)),
                 ("prolongs", ctypes.c_void_p),
                 ("blocks", ctypes.c_void_p),
-                ("copies", ctypes.c_void_p)]
+                ("copies", ctypes.c_void_p),
+                ("tile_end", ctypes.POINTER(ctypes.c_longlong)),
+                ("tiles", ctypes.c_void_p),
+                ("tile_points", ctypes.c_void_p),
+                ("n_tile_points", ctypes.c_longlong)]
 
 _lib = None
 _lock = threading.Lock()

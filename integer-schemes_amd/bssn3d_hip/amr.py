@@ -55,6 +55,12 @@ BLOCK_DTYPE = np.dtype([("dst", "<i8"), ("parent", "<i8"), ("corner", "<i4", (3,
 assert BLOCK_DTYPE.itemsize == 40
 BLOCK_WMAX = 24
 SKIP_INTERIOR_WRITE = 64
+# bssn_prolong_tile / bssn_prolong_tile_point: prolongation points grouped by 4x4x4 fine tile
+TILE_DTYPE = np.dtype([("parent", "<i8"), ("first", "<i8"), ("pS", "<i4", (3,)),
+                       ("origin", "<i4", (3,)), ("count", "<i4"), ("pad", "<i4")])
+assert TILE_DTYPE.itemsize == 48
+TILE_POINT_DTYPE = np.dtype([("dst", "<i8"), ("dst_stride", "<i4"), ("local", "<i4")])
+assert TILE_POINT_DTYPE.itemsize == 16
 
 Tile = Tuple[int, int, int, int]
 
@@ -310,6 +316,54 @@ def by_parity(e: np.ndarray) -> np.ndarray:
     return e[np.argsort(cls, kind="stable")]
 
 
+def prolong_tiles(e: np.ndarray, first: int = 0):
+    """Point entries grouped by parent and 4x4x4 fine tile (fine index // 4):
+    (tiles, points) for bssn_prolong_tile / bssn_prolong_tile_point, the same values point for
+    point; tile point indices start at ``first``."""
+    tiles = np.empty(0, dtype=TILE_DTYPE)
+    pts = np.empty(0, dtype=TILE_POINT_DTYPE)
+    if len(e) == 0:
+        return tiles, pts
+    f = e["fine"].astype(np.int64)
+    t = f >> 2
+    order = np.lexsort((t[:, 2], t[:, 1], t[:, 0], e["parent"]))
+    e, f, t = e[order], f[order], t[order]
+    new = np.ones(len(e), bool)
+    new[1:] = (e["parent"][1:] != e["parent"][:-1]) | np.any(t[1:] != t[:-1], axis=1)
+    start = np.flatnonzero(new)
+    tiles = np.empty(len(start), dtype=TILE_DTYPE)
+    tiles["parent"] = e["parent"][start]
+    tiles["first"] = first + start
+    tiles["pS"] = e["pS"][start]
+    tiles["origin"] = (t[start] << 2).astype(np.int32)
+    tiles["count"] = np.diff(np.append(start, len(e)))
+    tiles["pad"] = 0
+    pts = np.empty(len(e), dtype=TILE_POINT_DTYPE)
+    pts["dst"] = e["dst"]
+    pts["dst_stride"] = e["dst_stride"]
+    loc = f & 3
+    pts["local"] = ((loc[:, 0] * 4 + loc[:, 1]) * 4 + loc[:, 2]).astype(np.int32)
+    return tiles, pts
+
+
+def tile_prolong_points(tiles: np.ndarray, pts: np.ndarray) -> np.ndarray:
+    """The point entries a tile list stands for (the inverse of prolong_tiles up to order)."""
+    out = np.empty(len(pts), dtype=PROLONG_DTYPE)
+    if len(pts) == 0:
+        return out
+    own = np.repeat(np.arange(len(tiles)), tiles["count"])
+    idx = np.concatenate([np.arange(a, a + c) for a, c in zip(tiles["first"], tiles["count"])])
+    idx = idx - int(tiles["first"].min())
+    q = pts[idx]
+    loc = np.stack([q["local"] >> 4, (q["local"] >> 2) & 3, q["local"] & 3], -1)
+    out["dst"] = q["dst"]
+    out["dst_stride"] = q["dst_stride"]
+    out["parent"] = tiles["parent"][own]
+    out["pS"] = tiles["pS"][own]
+    out["fine"] = tiles["origin"][own] + loc
+    return out
+
+
 def restrict_entries(parent_off: int, child_off: int, corner, BS: int, ng: int) -> np.ndarray:
     """Injection copies of a child's even interior nodes onto the parent nodes they coincide
     with: parent ng + corner BS/2 + a <- child ng + 2 a, a in [0, BS/2)^3."""
@@ -419,17 +473,38 @@ def build_ghost_program(tree: Octree, BS: int, ng: int, offset_of: Callable[[Til
 
 
 class DeviceProgram:
-    """A ghost fill program in device memory (keeps the entry arrays alive)."""
+    """A ghost fill program in device memory (keeps the entry arrays alive). With tiles=True the
+    point prolongations go to the device grouped by 4x4x4 fine tile (prolong_tiles: the same
+    values, each tile's coarse window read once and its x and y passes shared)."""
 
     def __init__(self, prolongs: np.ndarray, jobs: np.ndarray, copies: np.ndarray, pe, be, ce,
-                 device):
+                 device, tiles: bool = False):
         self.n_prolong, self.n_block, self.n_copy = len(prolongs), len(jobs), len(copies)
         put = lambda a: upload(a, device) if len(a) else None
+        te = [0] * len(pe)
+        self.tiles = np.empty(0, dtype=TILE_DTYPE)
+        self.tile_points = np.empty(0, dtype=TILE_POINT_DTYPE)
+        if tiles and len(prolongs):
+            tparts, qparts, lo, nq = [], [], 0, 0
+            for i, hi in enumerate(pe):
+                tt, qq = prolong_tiles(prolongs[lo:hi], first=nq)
+                tparts.append(tt)
+                qparts.append(qq)
+                nq += len(qq)
+                te[i] = sum(len(x) for x in tparts)
+                lo = hi
+            self.tiles = np.concatenate(tparts)
+            self.tile_points = np.concatenate(qparts)
+            prolongs = prolongs[:0]
+            pe = [0] * len(pe)
+        self.n_tile = len(self.tiles)
         self._p, self._b, self._c = put(prolongs), put(jobs), put(copies)
-        self._ends = [(ctypes.c_longlong * len(x))(*x) for x in (pe, be, ce)]
+        self._t, self._q = put(self.tiles), put(self.tile_points)
+        self._ends = [(ctypes.c_longlong * len(x))(*x) for x in (pe, be, ce, te)]
         addr = lambda t: t.data_ptr() if t is not None else None
-        self.struct = _lib.GhostProgram(len(pe), *self._ends, addr(self._p), addr(self._b),
-                                        addr(self._c))
+        self.struct = _lib.GhostProgram(len(pe), *self._ends[:3], addr(self._p), addr(self._b),
+                                        addr(self._c), self._ends[3], addr(self._t),
+                                        addr(self._q), len(self.tile_points))
 
     @property
     def ptr(self):

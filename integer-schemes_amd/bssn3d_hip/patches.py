@@ -95,6 +95,7 @@ class PatchHierarchy:
         self._prog = None
         self._linked = None
         self._link = None
+        self.tiles = True
 
     @property
     def batch(self) -> BlockBatch:
@@ -210,17 +211,23 @@ class PatchHierarchy:
         return (cat(pro_lists, PROLONG_DTYPE), np.empty(0, dtype=BLOCK_DTYPE),
                 cat(cop_lists, HALO_DTYPE), *ends)
 
+    def _device_program(self, entries) -> DeviceProgram:
+        pr, jb, cp, pe, be, ce = entries
+        prog = DeviceProgram(pr, jb, cp, pe, be, ce, self.device, tiles=self.tiles)
+        lib = _lib.load()
+        pool = self.layout.pool_doubles
+        for kind, arr in ((1, pr), (0, cp), (3, prog.tiles), (4, prog.tile_points)):
+            if len(arr) and lib.bssn_count_skipped_entries(
+                    kind, np.ascontiguousarray(arr).ctypes.data, len(arr), pool) != 0:
+                raise RuntimeError("ghost program entry outside the pool")
+        return prog
+
     @property
     def program(self) -> DeviceProgram:
+        """The ghost fill program on the device (prolongations grouped by 4x4x4 fine tile
+        unless ``tiles`` is False)."""
         if self._prog is None:
-            pr, jb, cp, pe, be, ce = self.program_entries()
-            lib = _lib.load()
-            pool = self.layout.pool_doubles
-            for kind, arr in ((1, pr), (0, cp)):
-                if len(arr) and lib.bssn_count_skipped_entries(
-                        kind, np.ascontiguousarray(arr).ctypes.data, len(arr), pool) != 0:
-                    raise RuntimeError("ghost program entry outside the pool")
-            self._prog = DeviceProgram(pr, jb, cp, pe, be, ce, self.device)
+            self._prog = self._device_program(self.program_entries())
         return self._prog
 
     # linked steps --------------------------------------------------------------------------
@@ -302,8 +309,7 @@ class PatchHierarchy:
     @property
     def linked_program(self) -> DeviceProgram:
         if self._linked is None:
-            pr, jb, cp, pe, be, ce = self.linked_program_entries()
-            self._linked = DeviceProgram(pr, jb, cp, pe, be, ce, self.device)
+            self._linked = self._device_program(self.linked_program_entries())
         return self._linked
 
     @property

@@ -974,13 +974,15 @@ int bssn_batch_halo_f64(double* pool, long long pool_doubles, const bssn_halo_co
 
 long long bssn_count_skipped_entries(int kind, const void* entries, long long n,
                                      long long pool_doubles) {
-  if (n < 0 || (n > 0 && !entries) || pool_doubles <= 0 || kind < 0 || kind > 2) return -1;
+  if (n < 0 || (n > 0 && !entries) || pool_doubles <= 0 || kind < 0 || kind > 4) return -1;
   long long bad = 0;
   for (long long i = 0; i < n; ++i) {
     bool ok;
     if (kind == 0) ok = halo_entry_ok(((const bssn_halo_copy*)entries)[i], pool_doubles);
     else if (kind == 1) ok = prolong_entry_ok(((const bssn_prolong_point*)entries)[i], pool_doubles);
-    else ok = block_job_ok(((const bssn_prolong_block*)entries)[i], pool_doubles);
+    else if (kind == 2) ok = block_job_ok(((const bssn_prolong_block*)entries)[i], pool_doubles);
+    else if (kind == 3) ok = prolong_tile_ok(((const bssn_prolong_tile*)entries)[i], pool_doubles);
+    else ok = prolong_tile_point_ok(((const bssn_prolong_tile_point*)entries)[i], pool_doubles);
     bad += !ok;
   }
   return bad;
@@ -1050,26 +1052,35 @@ int check_program(const bssn_ghost_program* p) {
   if (p->nphase < 0 ||
       (p->nphase > 0 && (!p->prolong_end || !p->block_end || !p->copy_end)))
     return fail(BSSN_E_ARG, "bad ghost program phase table");
-  long long pe = 0, be = 0, ce = 0;
+  long long pe = 0, be = 0, ce = 0, te = 0;
   for (int i = 0; i < p->nphase; ++i) {
-    if (p->prolong_end[i] < pe || p->block_end[i] < be || p->copy_end[i] < ce)
+    const long long t = p->tile_end ? p->tile_end[i] : 0;
+    if (p->prolong_end[i] < pe || p->block_end[i] < be || p->copy_end[i] < ce || t < te)
       return fail(BSSN_E_ARG, "ghost program phase ends must not decrease");
     pe = p->prolong_end[i];
     be = p->block_end[i];
     ce = p->copy_end[i];
+    te = t;
   }
-  if ((pe > 0 && !p->prolongs) || (be > 0 && !p->blocks) || (ce > 0 && !p->copies))
+  if ((pe > 0 && !p->prolongs) || (be > 0 && !p->blocks) || (ce > 0 && !p->copies) ||
+      (te > 0 && (!p->tiles || !p->tile_points || p->n_tile_points < 0)))
     return fail(BSSN_E_ARG, "ghost program entries missing");
   return BSSN_OK;
 }
 
 int run_program(double* pool, long long pool_n, const bssn_ghost_program* p, hipStream_t st) {
-  long long pb = 0, bb = 0, cb = 0;
+  long long pb = 0, bb = 0, cb = 0, tb = 0;
   for (int i = 0; i < p->nphase; ++i) {
     const long long pe = p->prolong_end[i], be = p->block_end[i], ce = p->copy_end[i];
+    const long long te = p->tile_end ? p->tile_end[i] : 0;
     if (pe > pb)
       TRY(check_hip(launch_prolong(pool, pool_n, p->prolongs + pb, (long)(pe - pb), st),
                     "prolongation"));
+    if (te > tb)
+      TRY(check_hip(launch_prolong_tiles(pool, pool_n, p->tiles + tb, (long)(te - tb),
+                                         p->tile_points, p->n_tile_points, st),
+                    "prolongation tiles"));
+    tb = te;
     if (be > bb)
       TRY(check_hip(launch_prolong_blocks(pool, pool_n, p->blocks + bb, (long)(be - bb), st),
                     "block prolongation"));

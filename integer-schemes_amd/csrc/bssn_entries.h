@@ -39,6 +39,22 @@ __host__ __device__ inline bool prolong_entry_ok(const bssn_prolong_point& m, lo
          m.parent + (long long)BSSN_NF * P <= pool_n;
 }
 
+// a prolongation tile: the parent's padded block in the pool, an aligned origin
+__host__ __device__ inline bool prolong_tile_ok(const bssn_prolong_tile& T, long long pool_n) {
+  for (int a = 0; a < 3; ++a) {
+    if (T.pS[a] < 1 || T.pS[a] > 4096) return false;
+    if ((T.origin[a] & 3) != 0) return false;
+  }
+  if (T.count < 0 || T.count > 4096 || T.first < 0) return false;
+  const long long P = (long long)T.pS[0] * T.pS[1] * T.pS[2];
+  return T.parent >= 0 && T.parent + (long long)BSSN_NF * P <= pool_n;
+}
+
+__host__ __device__ inline bool prolong_tile_point_ok(const bssn_prolong_tile_point& q,
+                                                      long long pool_n) {
+  return q.local >= 0 && q.local < 64 && fields_in_pool(q.dst, q.dst_stride, pool_n);
+}
+
 __host__ __device__ inline bool block_job_ok(const bssn_prolong_block& j, long long pool_n) {
   if (j.BS < 1 || j.ng < 0 || j.BS + 2 * j.ng > PB_WMAX) return false;
   for (int a = 0; a < 3; ++a)

@@ -142,6 +142,8 @@ hipError_t launch_halo_copy(double* pool, long long pool_n, const void* map, lon
                             hipStream_t st);
 hipError_t launch_prolong(double* pool, long long pool_n, const void* pts, long n,
                           hipStream_t st);
+hipError_t launch_prolong_tiles(double* pool, long long pool_n, const void* tiles, long n,
+                                const void* tp, long long ntp, hipStream_t st);
 hipError_t launch_prolong_blocks(double* pool, long long pool_n, const void* jobs, long n,
                                  hipStream_t st);
 hipError_t launch_indicator(const double* pool, const BlockDesc* blk, int nblk, double* out,

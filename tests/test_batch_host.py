@@ -147,5 +147,5 @@ def test_entry_bounds_predicates():
                            block_job(P + 1, 0, (0, 0, 0), BS, ng, 0)])  # child past the end
     assert jobs.dtype == BLOCK_DTYPE and good.dtype == PROLONG_DTYPE
     assert count(2, jobs, 2 * P) == 3 and count(2, jobs[:1], 2 * P) == 0
-    assert lib.bssn_count_skipped_entries(3, None, 0, pool) == -1
+    assert lib.bssn_count_skipped_entries(5, None, 0, pool) == -1
     assert lib.bssn_count_skipped_entries(0, None, 4, pool) == -1

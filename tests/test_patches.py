@@ -106,6 +106,22 @@ def test_ghost_fill_reproduces_polynomials(hier):
     assert worst[0] == 0.0 and worst[1] <= 1e-13
 
 
+def test_tile_program_equals_point_program(hier):
+    H, _ = hier
+    say("640-block ghost fill: prolongation tiles vs point entries")
+    from bssn3d_hip.amr import DeviceProgram
+    U = fill_binary(H, H.empty_pool(), sync=False)
+    V = U.clone()
+    pts = DeviceProgram(*H.program_entries(), H.device, tiles=False)
+    assert H.program.n_tile > 0 and pts.n_tile == 0
+    H.sync_(U)
+    _lib.check(_lib.load().bssn_ghost_fill_f64(V.data_ptr(), V.numel(), pts.ptr,
+                                               stream_handle(V.device)))
+    torch.cuda.synchronize()
+    for b in range(len(H)):
+        assert torch.equal(H.view(U, b), H.view(V, b)), b
+
+
 def test_batched_rhs_equals_single_grid_sample(hier):
     H, _ = hier
     say("640-block batched RHS+KO vs bssn_rhs_f64 on 48 sampled blocks")

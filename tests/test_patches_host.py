@@ -140,3 +140,37 @@ def test_linked_program_and_link_map():
     # copies left out hold NaN in the linked pool and no prolongated ghost became NaN
     skipped = np.setdiff1d(cp["dst"], lcp["dst"])
     assert np.isnan(linked[skipped]).all()
+
+
+def test_prolongation_tiles_cover_the_points():
+    """The tile form of the program's prolongations (prolong_tiles) stands for exactly its
+    point entries (tile_prolong_points inverts it); tiles are aligned 4x4x4 blocks of one
+    parent; the C-ABI's bounds predicates accept every tile and tile point on the hierarchy's
+    pool and reject misaligned, oversized and out-of-pool ones."""
+    from bssn3d_hip import _lib
+    from bssn3d_hip.amr import prolong_tiles, tile_prolong_points
+    H, _ = config_hierarchy(seed=0, device="cpu")
+    pr = H.program_entries()[0]
+    tiles, pts = prolong_tiles(pr)
+    assert 0 < len(tiles) < len(pr) and tiles["count"].sum() == len(pr)
+    assert np.all(tiles["origin"] % 4 == 0) and np.all((pts["local"] >= 0) & (pts["local"] < 64))
+    back = tile_prolong_points(tiles, pts)
+    key = lambda e: np.lexsort((e["fine"][:, 2], e["fine"][:, 1], e["fine"][:, 0], e["dst"]))
+    a, b = pr[key(pr)], back[key(back)]
+    for k in ("dst", "parent", "dst_stride", "fine", "pS"):
+        assert np.array_equal(a[k], b[k]), k
+    lib = _lib.load()
+    pool = H.layout.pool_doubles
+    count = lambda kind, arr: lib.bssn_count_skipped_entries(
+        kind, np.ascontiguousarray(arr).ctypes.data, len(arr), pool)
+    assert count(3, tiles) == 0 and count(4, pts) == 0
+    bad = tiles[:4].copy()
+    bad["origin"][1, 0] += 2          # misaligned
+    bad["count"][2] = 5000            # too many points
+    bad["parent"][3] = pool           # parent past the pool
+    assert count(3, bad) == 3
+    badq = pts[:3].copy()
+    badq["local"][0] = 64
+    badq["dst"][1] = pool
+    assert count(4, badq) == 2
+    assert lib.bssn_count_skipped_entries(5, None, 0, pool) == -1
