@@ -409,7 +409,7 @@ int bssn_batch_rk4_step_prog_f64(const bssn_batch* b, double* U, double* work, c
  * m_b = P_0 + ... + P_(b-1) (blocks in the order given to bssn_batch_create,
  * bssn_batch_points(b, 0) points in all): (pool offset of field 0 of the point to read, field
  * stride of its block); read only at ghost points. Needs ng >= the stencils' reach and a pool
- * below 2^31 doubles. Same batched step as the reference's slot-batched N-level step
+ * below 2^29 doubles (4 GiB). Same batched step as the reference's slot-batched N-level step
  * (2D/src/mcs2d/amr/evolve.py:659-775); the ghost exchange by reference instead of by copy
  * has no counterpart there. */
 int bssn_batch_rk4_step_link_f64(const bssn_batch* b, double* U, double* work,

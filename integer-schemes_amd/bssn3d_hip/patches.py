@@ -244,8 +244,8 @@ class PatchHierarchy:
         lay = self.layout
         off = np.asarray(lay.off, dtype=np.int64)
         P = np.asarray(lay.P, dtype=np.int64)
-        if lay.pool_doubles >= 2 ** 31:
-            raise ValueError("linked steps need a pool below 2^31 doubles")
+        if lay.pool_doubles >= 2 ** 29:
+            raise ValueError("linked steps need a pool below 2^29 doubles (4 GiB)")
         m = self.link_offsets()
         n = int(P.sum())
         blk = np.repeat(np.arange(len(P)), P)

@@ -1160,8 +1160,8 @@ int bssn_batch_rk4_step_link_f64(const bssn_batch* b, double* U, double* work,
   if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4, 6 or 8");
   const int reach = ko_sigma > 0.0 ? std::max(fd_order, ko_order_of(fd_order)) / 2 : fd_order / 2;
   if (b->ng < reach) return fail(BSSN_E_ARG, "linked steps need ng >= the stencils' reach");
-  if (b->pool_doubles >= (1LL << 31))
-    return fail(BSSN_E_ARG, "linked steps need a pool below 2^31 doubles");
+  if (b->pool_doubles >= (1LL << 29))
+    return fail(BSSN_E_ARG, "linked steps need a pool below 2^29 doubles");
   hipStream_t st = (hipStream_t)stream;
   auto sync = [&](double* pool) { return run_program(pool, b->pool_doubles, linked, st); };
   TRY(batch_rk4(b, U, work, S16, fd_order, ko_sigma, 2, sync, false, t, dt, chi_floor,

@@ -344,7 +344,7 @@ struct WaveSrc : BoxStencil<Geo<ORD, KO>, Coef<ORD>> {
 
   // LINK: a block of a batch (pool offset boff, link map lmap of its padded points). Cells in
   // the block's interior are read in place; a ghost cell is read where the map points. Needs
-  // ng >= H (the box stays inside the padded block) and pool offsets below 2^31.
+  // ng >= H (the box stays inside the padded block) and a pool below 2^29 doubles.
   __device__ __forceinline__ WaveSrc(double* lds_, const double* pool, long boff,
                                      const int* lmap, const Scales& sc_, const Dims& d, int ox,
                                      int oy, int oz, int lane)
@@ -392,7 +392,11 @@ struct WaveSrc : BoxStencil<Geo<ORD, KO>, Coef<ORD>> {
 #pragma unroll
     for (int k = 0; k < G::PER; ++k) {
       if constexpr (MODE == LINK) {
-        if (k < n) pf[slot][k] = F[(long)f * psrc[k] + off[k]];
+        // a 32-bit byte offset from the uniform pool base (pools below 2^29 doubles), so the
+        // load takes its base from scalar registers and no 64-bit address arithmetic per cell
+        const unsigned b = (unsigned)(off[k] + f * psrc[k]) << 3;
+        if (k < n) pf[slot][k] = *reinterpret_cast<const double*>(
+                                     reinterpret_cast<const char*>(F) + b);
       } else {
         if (k < n) pf[slot][k] = u[off[k]];
       }
