@@ -423,6 +423,15 @@ def bench_amr(reps: int) -> dict:
     ms_step = timed(step, reps, dev)
     assert bool(torch.isfinite(U).all()), "non-finite hierarchy state"
     ms_sync = timed(lambda: H.sync_(U), reps, dev)
+    # linked steps: face-copied ghosts read at their sources, only the prolongations (and the
+    # copies their taps read) between stages
+    H.linked_program, H.link
+    lk = dict(params=p, order=4, ko_sigma=0.1, ghosts="linked", sync_end=False)
+    H.step_(U, 0.0, dt, **lk)
+    torch.cuda.synchronize()
+    ms_link = timed(lambda: H.step_(U, 0.0, dt, **lk), reps, dev)
+    assert bool(torch.isfinite(U).all()), "non-finite hierarchy state"
+    H.sync_(U)
     pts = sum(counts) * 16 ** 3
     prog = H.program
     # a regrid pass that changes nothing costs the indicator and the decisions; time one that
@@ -444,6 +453,8 @@ def bench_amr(reps: int) -> dict:
             "steps_timed": reps, "step_ms": round(ms_step, 4),
             "step_mpts_s": round(pts / (ms_step * 1e-3) / 1e6, 2),
             "ghost_fill_ms": round(ms_sync, 4),
+            "linked": {"copies": H.linked_program.n_copy, "step_ms": round(ms_link, 4),
+                       "step_mpts_s": round(pts / (ms_link * 1e-3) / 1e6, 2)},
             "regrid_refine_pass_ms": round(ms_regrid, 1)}
 
 

@@ -407,6 +407,98 @@ def _copies(dst_off: int, dst_pts: np.ndarray, src_off, src_pts: np.ndarray, W: 
     return e
 
 
+def job_prolong_points(jobs: np.ndarray) -> np.ndarray:
+    """Whole-block jobs as the point entries with the same values: the padded-block points a
+    job writes (ghosts outside its skipped face slabs, the interior too with bit 6)."""
+    out = []
+    for j in jobs:
+        BS, ng = int(j["BS"]), int(j["ng"])
+        W = BS + 2 * ng
+        c = _cube(W)
+        inside = (c >= ng) & (c < ng + BS)
+        keep = np.ones(len(c), bool) if j["skip"] & SKIP_INTERIOR_WRITE else \
+            ~np.all(inside, axis=1)
+        for ax in range(3):
+            slab = ~inside[:, ax] & np.all(np.delete(inside, ax, axis=1), axis=1)
+            lo = c[:, ax] < ng
+            keep &= ~(slab & lo & bool(j["skip"] & (1 << (2 * ax))))
+            keep &= ~(slab & ~lo & bool(j["skip"] & (1 << (2 * ax + 1))))
+        out.append(prolong_entries(c[keep], int(j["dst"]), int(j["parent"]),
+                                   tuple(int(v) for v in j["corner"]), BS, ng))
+    return np.concatenate(out) if out else np.empty(0, dtype=PROLONG_DTYPE)
+
+
+def prolong_taps(pr: np.ndarray, ng: int) -> np.ndarray:
+    """Pool offsets (field 0) of the parent ghost points that prolongation entries read: per
+    axis coarse node fine / 2 (even fine) or (fine - 1) / 2 - 2 .. + 3 (odd), clamped to the
+    parent's padded block, as bssn_prolong_f64 reads them."""
+    found = []
+    fine = pr["fine"].astype(np.int64)
+    pS = pr["pS"].astype(np.int64)
+    par = pr["parent"].astype(np.int64)
+    cls = (fine[:, 0] & 1) << 2 | (fine[:, 1] & 1) << 1 | (fine[:, 2] & 1)
+    for c in range(8):
+        sel = np.flatnonzero(cls == c)
+        for lo in range(0, len(sel), 1 << 15):
+            e = sel[lo:lo + (1 << 15)]
+            taps, ghost = [], []
+            for a in range(3):
+                f, S = fine[e, a], pS[e, a]
+                if (c >> (2 - a)) & 1:
+                    t = (f[:, None] - 1) // 2 + np.arange(-2, 4)[None, :]
+                else:
+                    t = (f // 2)[:, None]
+                t = np.clip(t, 0, S[:, None] - 1)
+                taps.append(t)
+                ghost.append((t < ng) | (t >= S[:, None] - ng))
+            tx, ty, tz = taps
+            S1, S2 = pS[e, 1][:, None, None, None], pS[e, 2][:, None, None, None]
+            flat = par[e][:, None, None, None] + (tx[:, :, None, None] * S1
+                                                  + ty[:, None, :, None]) * S2 \
+                + tz[:, None, None, :]
+            g = ghost[0][:, :, None, None] | ghost[1][:, None, :, None] | \
+                ghost[2][:, None, None, :]
+            found.append(flat[g])
+    return np.unique(np.concatenate(found)) if found else np.empty(0, np.int64)
+
+
+def linked_entries(entries, ng: int):
+    """The ghost program of a linked step: every prolongation (points and whole-block jobs), and
+    only the copies whose destination a prolongation reads, in the program's phases."""
+    pr, jb, cp, pe, be, ce = entries
+    taps = prolong_taps(np.concatenate([pr, job_prolong_points(jb)]), ng)
+    need = np.isin(cp["dst"].astype(np.int64), taps)
+    kept = np.cumsum(need)
+    ce2 = [int(kept[e - 1]) if e > 0 else 0 for e in ce]
+    return pr, jb, cp[need], pe, be, ce2
+
+
+def link_map(off, P, copies: np.ndarray, pool_doubles: int) -> np.ndarray:
+    """(n_points, 2) int32 over the padded points of blocks at pool offsets ``off`` with ``P``
+    points each (block order, each block row-major): the pool offset of field 0 of the point a
+    linked stage reads there and that block's field stride -- a copy entry's source at a copied
+    ghost, the point itself elsewhere (interior points, prolongated ghosts)."""
+    off = np.asarray(off, dtype=np.int64)
+    P = np.asarray(P, dtype=np.int64)
+    if pool_doubles >= 2 ** 29:
+        raise ValueError("linked steps need a pool below 2^29 doubles (4 GiB)")
+    m = np.concatenate([[0], np.cumsum(P)[:-1]])
+    n = int(P.sum())
+    blk = np.repeat(np.arange(len(P)), P)
+    q = np.arange(n, dtype=np.int64) - m[blk]
+    out = np.empty((n, 2), dtype=np.int32)
+    out[:, 0] = off[blk] + q
+    out[:, 1] = P[blk]
+    if len(copies):
+        dst = copies["dst"].astype(np.int64)
+        order = np.argsort(off)
+        b = order[np.searchsorted(off[order], dst, side="right") - 1]
+        i = m[b] + (dst - off[b])
+        out[i, 0] = copies["src"]
+        out[i, 1] = copies["src_stride"]
+    return out
+
+
 def build_ghost_program(tree: Octree, BS: int, ng: int, offset_of: Callable[[Tile], int],
                         use_blocks: bool | None = None):
     """The hierarchy sync as ghost fill program entries, one phase per level.
@@ -545,6 +637,8 @@ class AMRHierarchy:
         self._place()
         self._batch = None
         self._prog = None
+        self._linked = None
+        self._link = None
 
     # pool layout ---------------------------------------------------------------------------
     def _place(self):
@@ -591,6 +685,8 @@ class AMRHierarchy:
             self._batch.close()
         self._batch = None
         self._prog = None
+        self._linked = None
+        self._link = None
 
     @property
     def keys(self) -> List[Tile]:
@@ -617,6 +713,27 @@ class AMRHierarchy:
             self._validate(pr, cp, jb)
             self._prog = DeviceProgram(pr, jb, cp, pe, be, ce, self.device)
         return self._prog
+
+    @property
+    def linked_program(self) -> DeviceProgram:
+        """The program of a linked step: every prolongation (points and whole-block jobs) and
+        only the copies into the parent ghosts their taps read."""
+        if self._linked is None:
+            pr, jb, cp, pe, be, ce = linked_entries(self.program_entries(), self.ng)
+            self._validate(pr, cp, jb)
+            self._linked = DeviceProgram(pr, jb, cp, pe, be, ce, self.device)
+        return self._linked
+
+    @property
+    def link(self) -> torch.Tensor:
+        """The link map of the tiles in batch order, on the device."""
+        if self._link is None:
+            tiles = self.keys
+            off = [self.slot_offset(t) for t in tiles]
+            m = link_map(off, [self.W ** 3] * len(tiles), self.program_entries()[2],
+                         self.pool_doubles)
+            self._link = torch.from_numpy(m).to(self.device)
+        return self._link
 
     def _validate(self, prolongs: np.ndarray, copies: np.ndarray, jobs: np.ndarray):
         """Every entry must address points of blocks inside this pool (the device kernels do
@@ -688,8 +805,12 @@ class AMRHierarchy:
         return self.batch.workspace()
 
     def step_(self, U: torch.Tensor, t: float, dt: float, params: PhysicsParams | None = None,
-              order: int = 4, ko_sigma: float = 0.1, tdev: torch.Tensor | None = None):
-        """One shared-dt RK4 step of every block in place (U synced on entry and on exit)."""
+              order: int = 4, ko_sigma: float = 0.1, tdev: torch.Tensor | None = None,
+              ghosts: str = "program", sync_end: bool = True):
+        """One shared-dt RK4 step of every block in place (U synced on entry and on exit).
+        ghosts="linked": the stages read face-copied ghosts at their sources
+        (bssn_batch_rk4_step_link_f64); with sync_end=False only the ghosts the next linked step
+        reads are left up to date."""
         params = params or PhysicsParams()
         self._check_pool(U)
         b = self.batch
@@ -699,6 +820,16 @@ class AMRHierarchy:
             if tdev.dtype != torch.float64 or tdev.device != self.device or tdev.numel() != 1:
                 raise ValueError("tdev must be one float64 value on the hierarchy's device")
             tptr = tdev.data_ptr()
+        if ghosts == "linked":
+            _lib.check(_lib.load().bssn_batch_rk4_step_link_f64(
+                b._handle, U.data_ptr(), self.workspace().data_ptr(),
+                _lib.dbl(b.S16(params, dt, t)), int(order), float(ko_sigma),
+                self.link.data_ptr(), self.linked_program.ptr, prog.ptr if sync_end else None,
+                float(t), float(dt), params.chi_floor, params.alpha_floor, tptr,
+                stream_handle(self.device)))
+            return U
+        if ghosts != "program":
+            raise ValueError(f"ghosts must be 'program' or 'linked', not {ghosts!r}")
         _lib.check(_lib.load().bssn_batch_rk4_step_prog_f64(
             b._handle, U.data_ptr(), self.workspace().data_ptr(),
             _lib.dbl(b.S16(params, dt, t)), int(order), float(ko_sigma), prog.ptr, float(t),
@@ -711,6 +842,8 @@ class AMRHierarchy:
         each replay). The tiles must not change while the graph is in use."""
         self.workspace()
         prog = self.program
+        if kw.get("ghosts") == "linked":
+            prog = (prog, self.linked_program, self.link)
         scratch = U.clone()               # first launches (code loading) outside the capture
         self.step_(scratch, 0.0, dt, **kw)
         torch.cuda.synchronize(self.device)

@@ -35,7 +35,8 @@ import numpy as np
 import torch
 
 from . import _lib
-from .amr import BLOCK_DTYPE, PROLONG_DTYPE, DeviceProgram, by_parity
+from .amr import (BLOCK_DTYPE, PROLONG_DTYPE, DeviceProgram, by_parity, link_map,
+                  linked_entries)
 from .batch import HALO_DTYPE, Block, BlockBatch, BlockLayout
 from .rhs import stream_handle
 from .state import PhysicsParams
@@ -241,70 +242,14 @@ class PatchHierarchy:
         row-major), the pool offset of field 0 of the point its value is read from and that
         block's field stride -- the copy entry's source for a copied ghost, the point itself
         otherwise (interior points, prolongated ghosts)."""
-        lay = self.layout
-        off = np.asarray(lay.off, dtype=np.int64)
-        P = np.asarray(lay.P, dtype=np.int64)
-        if lay.pool_doubles >= 2 ** 29:
-            raise ValueError("linked steps need a pool below 2^29 doubles (4 GiB)")
-        m = self.link_offsets()
-        n = int(P.sum())
-        blk = np.repeat(np.arange(len(P)), P)
-        q = np.arange(n, dtype=np.int64) - m[blk]
-        out = np.empty((n, 2), dtype=np.int32)
-        out[:, 0] = off[blk] + q
-        out[:, 1] = P[blk]
         if copies is None:
             copies = self.program_entries()[2]
-        if len(copies):
-            dst = copies["dst"].astype(np.int64)
-            b = np.searchsorted(off, dst, side="right") - 1
-            i = m[b] + (dst - off[b])
-            out[i, 0] = copies["src"]
-            out[i, 1] = copies["src_stride"]
-        return out
-
-    @staticmethod
-    def prolong_taps(pr: np.ndarray, ng: int) -> np.ndarray:
-        """Pool offsets (field 0) of the parent ghost points that the prolongation entries read:
-        per axis coarse node fine / 2 (even fine) or (fine - 1) / 2 - 2 .. + 3 (odd), clamped to
-        the parent's padded block, as bssn_prolong_f64 reads them."""
-        found = []
-        fine = pr["fine"].astype(np.int64)
-        pS = pr["pS"].astype(np.int64)
-        par = pr["parent"].astype(np.int64)
-        cls = (fine[:, 0] & 1) << 2 | (fine[:, 1] & 1) << 1 | (fine[:, 2] & 1)
-        for c in range(8):
-            sel = np.flatnonzero(cls == c)
-            for lo in range(0, len(sel), 1 << 15):
-                e = sel[lo:lo + (1 << 15)]
-                taps, ghost = [], []
-                for a in range(3):
-                    f, S = fine[e, a], pS[e, a]
-                    if (c >> (2 - a)) & 1:
-                        t = (f[:, None] - 1) // 2 + np.arange(-2, 4)[None, :]
-                    else:
-                        t = (f // 2)[:, None]
-                    t = np.clip(t, 0, S[:, None] - 1)
-                    taps.append(t)
-                    ghost.append((t < ng) | (t >= S[:, None] - ng))
-                tx, ty, tz = taps
-                S1, S2 = pS[e, 1][:, None, None, None], pS[e, 2][:, None, None, None]
-                flat = par[e][:, None, None, None] + (tx[:, :, None, None] * S1
-                                                      + ty[:, None, :, None]) * S2 \
-                    + tz[:, None, None, :]
-                g = ghost[0][:, :, None, None] | ghost[1][:, None, :, None] | \
-                    ghost[2][:, None, None, :]
-                found.append(flat[g])
-        return np.unique(np.concatenate(found)) if found else np.empty(0, np.int64)
+        return link_map(self.layout.off, self.layout.P, copies, self.layout.pool_doubles)
 
     def linked_program_entries(self):
         """The program of a linked step: every prolongation, and only the copies whose
         destination a prolongation reads, in the program's phases."""
-        pr, jb, cp, pe, be, ce = self.program_entries()
-        need = np.isin(cp["dst"].astype(np.int64), self.prolong_taps(pr, self.ng))
-        kept = np.cumsum(need)
-        ce2 = [int(kept[e - 1]) if e > 0 else 0 for e in ce]
-        return pr, jb, cp[need], pe, be, ce2
+        return linked_entries(self.program_entries(), self.ng)
 
     @property
     def linked_program(self) -> DeviceProgram:

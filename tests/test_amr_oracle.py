@@ -161,6 +161,51 @@ def test_program_covers_every_ghost_once():
         assert np.all((c >= ng) & (c < ng + BS))
 
 
+def test_linked_program_reproduces_prolongations():
+    """The program of a linked step (bssn3d_hip.amr.linked_entries: every prolongation, only
+    the copies their taps read) gives every prolongated ghost -- point entries and whole-block
+    jobs alike -- the value the full program gives it, bit for bit, on a pool whose ghosts
+    start as NaN; the product's expansion of jobs into points (job_prolong_points) equals the
+    emulator's; the link map sends each copied ghost to its source and every other point to
+    itself."""
+    from bssn3d_hip.amr import job_prolong_points, link_map, linked_entries
+    H = three_level()
+    W, ng, BS = H.W, H.ng, H.BS
+    for use_blocks in (True, False):
+        ent = build_ghost_program(H.tree, BS, ng, H.slot_offset, use_blocks)
+        pr, jb, cp, pe, be, ce = ent
+        lpr, ljb, lcp, lpe, lbe, lce = linked_entries(ent, ng)
+        assert len(lpr) == len(pr) and len(ljb) == len(jb) and len(lcp) < len(cp)
+        if use_blocks:
+            a, b = job_prolong_points(jb), expand_jobs(jb)
+            assert all(np.array_equal(a[k], b[k]) for k in ("dst", "parent", "fine", "pS"))
+        pool = np.full(H.pool_doubles, np.nan)
+        rng = np.random.default_rng(6)
+        for k in H.keys:
+            v = H.view(pool, k)
+            v[:, ng:ng + BS, ng:ng + BS, ng:ng + BS] = rng.standard_normal((24, BS, BS, BS))
+        full, linked = pool.copy(), pool.copy()
+        run_program(full, *ent)
+        run_program(linked, lpr, ljb, lcp, lpe, lbe, lce)
+        pts = np.concatenate([pr, expand_jobs(jb)])
+        idx = (pts["dst"][:, None] + np.arange(24)[None, :] * pts["dst_stride"][:, None]).ravel()
+        assert not np.isnan(full[idx]).any()
+        assert np.array_equal(linked[idx], full[idx])
+        tiles = H.keys
+        off = [H.slot_offset(t) for t in tiles]
+        L = link_map(off, [W ** 3] * len(tiles), cp, H.pool_doubles)
+        m = np.arange(len(tiles)) * W ** 3
+        order = np.argsort(off)
+        bi = np.asarray(order)[np.searchsorted(np.asarray(off)[order], cp["dst"], side="right") - 1]
+        i = m[bi] + (cp["dst"] - np.asarray(off)[bi])
+        assert np.array_equal(L[i, 0], cp["src"]) and np.all(L[i, 1] == W ** 3)
+        rest = np.ones(len(L), bool)
+        rest[i] = False
+        blk = np.repeat(np.arange(len(tiles)), W ** 3)
+        q = np.arange(len(L)) - m[blk]
+        assert np.array_equal(L[rest, 0], (np.asarray(off)[blk] + q)[rest])
+
+
 def test_root_sync_equals_single_grid_periodic():
     H = AMRHierarchy(nb_root=(2, 3, 2), BS=8, ng=3, levels=1, device="cpu")
     N = [n * 8 for n in H.nb_root]
