@@ -274,10 +274,21 @@ __device__ __forceinline__ void i8_tile(const I8Args& a, long w, int lane, signe
   __shared__ __attribute__((aligned(16))) signed char th[I8_BOX * I8_BOX];   \
   __shared__ __attribute__((aligned(16))) signed char tl[I8_BOX * I8_BOX]
 
+// Workgroup -> tile, XCD-aware: the dispatcher deals workgroups round-robin over the 8 XCDs;
+// remap so that each XCD takes a contiguous run of tiles (consecutive planes), whose loads and
+// stores share cache lines along the plane axis (a line of 16 doubles along z holds the same
+// point of 16 consecutive (x, y) planes). The grid is a multiple of 8; tiles past the last are
+// empty (i8_tile returns on them).
+__device__ __forceinline__ long xcd_tile() {
+  const long G8 = (long)gridDim.x;
+  const long w = (long)blockIdx.x;
+  return (w & 7) * (G8 >> 3) + (w >> 3);
+}
+
 template <int ORIENT, bool TWO>
 __global__ __launch_bounds__(64) void k_i8_plane(I8Args a) {
   BSSN_I8_LDS;
-  i8_tile<ORIENT, TWO>(a, (long)blockIdx.x, (int)threadIdx.x, tq, th, tl);
+  i8_tile<ORIENT, TWO>(a, xcd_tile(), (int)threadIdx.x, tq, th, tl);
 }
 
 // The same pass over every block of a batch: block b owns wave tiles [pre[b], pre[b + 1]) of
@@ -290,7 +301,7 @@ __global__ __launch_bounds__(64) void k_i8_plane_batch(const double* pool, doubl
                                                        const long long* __restrict__ pre,
                                                        int nblk, int nslice) {
   BSSN_I8_LDS;
-  const long long w = (long long)blockIdx.x;
+  const long long w = (long long)xcd_tile();
   int lo = 0, hi = nblk;                 // pre[lo] <= w < pre[hi]
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
@@ -316,7 +327,7 @@ __global__ __launch_bounds__(64) void k_i8_plane_batch(const double* pool, doubl
 
 template <int ORIENT, bool TWO>
 static hipError_t launch_i8_t(const I8Args& a, hipStream_t st) {
-  const long n = i8_tiles<ORIENT, TWO>(a.S);
+  const long n = (i8_tiles<ORIENT, TWO>(a.S) + 7) & ~7L;
   if (n > 0x7fffffffL) return hipErrorInvalidValue;
   hipLaunchKernelGGL((k_i8_plane<ORIENT, TWO>), dim3((unsigned)n), dim3(64), 0, st, a);
   return hipGetLastError();
@@ -338,8 +349,9 @@ template <int ORIENT, bool TWO>
 static hipError_t launch_i8_batch_t(const double* pool, double* D, const BlockDesc* blk,
                                     const long long* pre, long long n, int nblk, int nslice,
                                     hipStream_t st) {
-  if (n > 0x7fffffffLL) return hipErrorInvalidValue;
   if (n == 0) return hipSuccess;
+  n = (n + 7) & ~7LL;
+  if (n > 0x7fffffffLL) return hipErrorInvalidValue;
   hipLaunchKernelGGL((k_i8_plane_batch<ORIENT, TWO>), dim3((unsigned)n), dim3(64), 0, st, pool,
                      D, blk, pre, nblk, nslice);
   return hipGetLastError();
