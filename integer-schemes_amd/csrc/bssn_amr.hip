@@ -116,13 +116,20 @@ __global__ __launch_bounds__(256) void k_prolong(double* __restrict__ pool,
 // outside the pool or the point array is skipped (the tile test is uniform over the group).
 __global__ __launch_bounds__(64) void k_prolong_tile(double* __restrict__ pool,
                                                      const bssn_prolong_tile* __restrict__ tiles,
+                                                     long ntiles,
                                                      const bssn_prolong_tile_point* __restrict__ tp,
                                                      long long ntp, long long pool_n) {
 #pragma clang fp contract(off)
   __shared__ double W[343];
   __shared__ double X[4 * 49];
   __shared__ double Y[16 * 7];
-  const bssn_prolong_tile T = tiles[blockIdx.x];
+  // XCD-aware: the dispatcher deals workgroups round-robin over the 8 XCDs (gridDim.x is a
+  // multiple of 8, so x % 8 picks the XCD); each XCD takes a contiguous run of tiles, whose
+  // coarse windows (tiles are sorted by parent and position) share cache lines
+  const long G8 = (long)gridDim.x, w = (long)blockIdx.x;
+  const long ti = (w & 7) * (G8 >> 3) + (w >> 3);
+  if (ti >= ntiles) return;
+  const bssn_prolong_tile T = tiles[ti];
   const int f = (int)blockIdx.y;
   const int lane = (int)threadIdx.x;
   if (!prolong_tile_ok(T, pool_n) || T.first + T.count > ntp) return;
@@ -186,8 +193,9 @@ __global__ __launch_bounds__(64) void k_prolong_tile(double* __restrict__ pool,
 hipError_t launch_prolong_tiles(double* pool, long long pool_n, const void* tiles, long n,
                                 const void* tp, long long ntp, hipStream_t st) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_prolong_tile, dim3((unsigned)n, BSSN_NF), dim3(64), 0, st, pool,
-                     (const bssn_prolong_tile*)tiles, (const bssn_prolong_tile_point*)tp, ntp,
+  const long g = (n + 7) & ~7L;
+  hipLaunchKernelGGL(k_prolong_tile, dim3((unsigned)g, BSSN_NF), dim3(64), 0, st, pool,
+                     (const bssn_prolong_tile*)tiles, n, (const bssn_prolong_tile_point*)tp, ntp,
                      pool_n);
   return hipGetLastError();
 }
