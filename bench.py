@@ -777,6 +777,11 @@ def main() -> int:
         res["twopass_rhs_ms"] = round(ms_twopass, 4)
         res["twopass_rhs_mpts_s"] = round(npts / (ms_twopass * 1e-3) / 1e6, 2)
         res["fused_vs_twopass_rhs"] = round(ms_twopass / ms_rhs, 3)
+        # the two-pass RHS at the HBM roofline: the bundle pass reads 24 fields and writes 138
+        # arrays, the algebra pass reads both and writes 24 fields, 348 doubles per padded point
+        floor_ms = 348 * 8 * (N + 6) ** 3 / PEAK_HBM * 1e3
+        res["twopass_hbm_floor_ms"] = round(floor_ms, 4)
+        res["fused_vs_twopass_hbm_floor"] = round(floor_ms / ms_rhs, 3)
     del out, U
     torch.cuda.empty_cache()
     if not args.no_fd6:
