@@ -591,9 +591,9 @@ def main() -> int:
                     help="also run one plain device copy of this many GiB (profiler calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(the halo-exchange sharding is described in DESIGN.md, not built).",
-              file=sys.stderr)
+        print("bench.py runs on one GPU only: the library has no multi-GPU path (sharding "
+              "would be a z-slab decomposition with a ghost-plane exchange per RK stage, "
+              "not built).", file=sys.stderr)
         return 2
 
     import torch

@@ -9,7 +9,7 @@
     exact, BFP48 below 1e-13, errors increasing 48 < 32 < 24, BFP32 constraint perturbations
     under its bars and BFP24 >100x BFP32 -- and the values its docstrings record (BFP32 RHS
     ~7e-10, constraint perturbation ~2.6e-9 / 7.4e-8), which the restatement reproduces;
-  * the reference emulator's model (mixed derivatives quantized twice) and this build's device
+  * the reference emulator's model (mixed derivatives quantized twice) and the device
     form (mixed taken from the state, quantized once) agree on fp64 storage bit for bit and
     differ only in the mixed arrays, at the quantization level;
   * the Gowdy constraints converge at the FD order (an exact solution).
