@@ -329,7 +329,8 @@ int bssn_batch_halo_f64(double* pool, long long pool_doubles, const bssn_halo_co
                         long long n, void* stream);
 
 /* How many of n entries (a host array; kind 0 = bssn_halo_copy, 1 = bssn_prolong_point,
- * 2 = bssn_prolong_block, 3 = bssn_prolong_tile, 4 = bssn_prolong_tile_point) the device
+ * 2 = bssn_prolong_block, 3 = bssn_prolong_tile, 4 = bssn_prolong_tile_point, 5 = link map
+ * entries of bssn_batch_rk4_step_link_f64, 2 ints each) the device
  * kernels would skip on a pool of pool_doubles doubles because they reach outside it (for a
  * block job also: padded extent above 24 or an octant index other than 0 or 1; for a tile: an
  * origin not a multiple of 4 or a count outside 0..4096; for a tile point: a local index outside
@@ -408,13 +409,16 @@ int bssn_batch_rk4_step_prog_f64(const bssn_batch* b, double* U, double* work, c
  * way. link: device array of 2 ints per padded point, block b's points at [m_b, m_b + P_b) with
  * m_b = P_0 + ... + P_(b-1) (blocks in the order given to bssn_batch_create,
  * bssn_batch_points(b, 0) points in all): (pool offset of field 0 of the point to read, field
- * stride of its block); read only at ghost points. Needs ng >= the stencils' reach and a pool
- * below 2^29 doubles (4 GiB). Same batched step as the reference's slot-batched N-level step
+ * stride of its block); read only at ghost points; link_points must equal
+ * bssn_batch_points(b, 0), and an entry that would reach outside the pool is not followed (the
+ * ghost's own value is read; bssn_count_skipped_entries kind 5 finds such entries). Needs
+ * ng >= the stencils' reach and a pool below 2^29 doubles (4 GiB). Same batched step as the reference's slot-batched N-level step
  * (2D/src/mcs2d/amr/evolve.py:659-775); the ghost exchange by reference instead of by copy
  * has no counterpart there. */
 int bssn_batch_rk4_step_link_f64(const bssn_batch* b, double* U, double* work,
                                  const double* S16, int fd_order, double ko_sigma,
-                                 const int* link, const bssn_ghost_program* linked,
+                                 const int* link, long long link_points,
+                                 const bssn_ghost_program* linked,
                                  const bssn_ghost_program* full, double t, double dt,
                                  double chi_floor, double alpha_floor, double* tdev,
                                  void* stream);

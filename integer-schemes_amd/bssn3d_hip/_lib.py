@@ -96,7 +96,7 @@ SIGNATURES = {
                                               _c_double, _c_void_p, _c_double, _c_double,
                                               _c_double, _c_double, _c_void_p, _c_void_p]),
     "bssn_batch_rk4_step_link_f64": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _PD, _c_int,
-                                              _c_double, _c_void_p, _c_void_p, _c_void_p,
+                                              _c_double, _c_void_p, _c_ll, _c_void_p, _c_void_p,
                                               _c_double, _c_double, _c_double, _c_double,
                                               _c_void_p, _c_void_p]),
     "bssn_batch_indicator_f64": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_void_p]),

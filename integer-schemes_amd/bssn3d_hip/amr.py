@@ -496,6 +496,8 @@ def link_map(off, P, copies: np.ndarray, pool_doubles: int) -> np.ndarray:
         i = m[b] + (dst - off[b])
         out[i, 0] = copies["src"]
         out[i, 1] = copies["src_stride"]
+    if _lib.load().bssn_count_skipped_entries(5, out.ctypes.data, len(out), pool_doubles) != 0:
+        raise RuntimeError("link map entry outside the pool")
     return out
 
 
@@ -824,8 +826,9 @@ class AMRHierarchy:
             _lib.check(_lib.load().bssn_batch_rk4_step_link_f64(
                 b._handle, U.data_ptr(), self.workspace().data_ptr(),
                 _lib.dbl(b.S16(params, dt, t)), int(order), float(ko_sigma),
-                self.link.data_ptr(), self.linked_program.ptr, prog.ptr if sync_end else None,
-                float(t), float(dt), params.chi_floor, params.alpha_floor, tptr,
+                self.link.data_ptr(), self.link.shape[0], self.linked_program.ptr,
+                prog.ptr if sync_end else None, float(t), float(dt), params.chi_floor,
+                params.alpha_floor, tptr,
                 stream_handle(self.device)))
             return U
         if ghosts != "program":

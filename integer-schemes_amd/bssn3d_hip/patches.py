@@ -297,7 +297,7 @@ class PatchHierarchy:
             _lib.check(_lib.load().bssn_batch_rk4_step_link_f64(
                 b._handle, U.data_ptr(), b.workspace().data_ptr(),
                 _lib.dbl(b.S16(params, dt, t)), int(order), float(ko_sigma),
-                self.link.data_ptr(), self.linked_program.ptr,
+                self.link.data_ptr(), self.link.shape[0], self.linked_program.ptr,
                 self.program.ptr if sync_end else None, float(t), float(dt), params.chi_floor,
                 params.alpha_floor, tptr, stream_handle(self.device)))
             return U

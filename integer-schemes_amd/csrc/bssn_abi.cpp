@@ -974,7 +974,7 @@ int bssn_batch_halo_f64(double* pool, long long pool_doubles, const bssn_halo_co
 
 long long bssn_count_skipped_entries(int kind, const void* entries, long long n,
                                      long long pool_doubles) {
-  if (n < 0 || (n > 0 && !entries) || pool_doubles <= 0 || kind < 0 || kind > 4) return -1;
+  if (n < 0 || (n > 0 && !entries) || pool_doubles <= 0 || kind < 0 || kind > 5) return -1;
   long long bad = 0;
   for (long long i = 0; i < n; ++i) {
     bool ok;
@@ -982,7 +982,11 @@ long long bssn_count_skipped_entries(int kind, const void* entries, long long n,
     else if (kind == 1) ok = prolong_entry_ok(((const bssn_prolong_point*)entries)[i], pool_doubles);
     else if (kind == 2) ok = block_job_ok(((const bssn_prolong_block*)entries)[i], pool_doubles);
     else if (kind == 3) ok = prolong_tile_ok(((const bssn_prolong_tile*)entries)[i], pool_doubles);
-    else ok = prolong_tile_point_ok(((const bssn_prolong_tile_point*)entries)[i], pool_doubles);
+    else if (kind == 4)
+      ok = prolong_tile_point_ok(((const bssn_prolong_tile_point*)entries)[i], pool_doubles);
+    else
+      ok = link_entry_ok(((const int*)entries)[2 * i], ((const int*)entries)[2 * i + 1],
+                         pool_doubles);
     bad += !ok;
   }
   return bad;
@@ -1025,6 +1029,7 @@ int batch_rk4(const bssn_batch* b, double* U, double* work, const double* S16, i
   auto stage = [&](double toff, int kind, double c0, const double* yin, double* y) {
     BatchArgs a = batch_args(b, interior, S16, dt, t + toff, ko_sigma);
     a.link = link;
+    a.pool_n = b->pool_doubles;
     s.toff = toff;
     s.Yin = yin;
     s.Y = y;
@@ -1149,12 +1154,15 @@ int bssn_batch_rk4_step_prog_f64(const bssn_batch* b, double* U, double* work, c
 
 int bssn_batch_rk4_step_link_f64(const bssn_batch* b, double* U, double* work,
                                  const double* S16, int fd_order, double ko_sigma,
-                                 const int* link, const bssn_ghost_program* linked,
+                                 const int* link, long long link_points,
+                                 const bssn_ghost_program* linked,
                                  const bssn_ghost_program* full, double t, double dt,
                                  double chi_floor, double alpha_floor, double* tdev,
                                  void* stream) {
   TRY(check_batch(b));
   if (!link) return fail(BSSN_E_ARG, "null link map");
+  if (link_points != b->padded_pts)
+    return fail(BSSN_E_ARG, "the link map needs one entry per padded point of the batch");
   TRY(check_program(linked));
   if (full) TRY(check_program(full));
   if (!valid_order(fd_order)) return fail(BSSN_E_ARG, "fd_order must be 4, 6 or 8");

@@ -92,7 +92,7 @@ __device__ __forceinline__ void batch_brick(double* lds, const StageArgs& args,
   a.Uout = shifted(args.Uout, blk.off);
   using Src = WaveSrc<ORD, KO, MODE, RhsOrder, fused_depth<KIND, ORD>()>;
   if constexpr (MODE == LINK) {
-    Src src(lds, args.Yin, blk.off, b.link + 2 * blk.moff, w, d, ox, oy, oz, lane);
+    Src src(lds, args.Yin, blk.off, b.link + 2 * blk.moff, b.pool_n, w, d, ox, oy, oz, lane);
     StageOut<KIND> out(a, d.P, src.gc, lane_valid<G>(d, ox, oy, oz, lane), true);
     bssn_rhs_point<KO>(src, sc, out);
   } else {

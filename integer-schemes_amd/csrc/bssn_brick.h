@@ -28,6 +28,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "bssn_entries.h"
 #include "bssn_params.h"
 
 namespace bssn {
@@ -343,11 +344,12 @@ struct WaveSrc : BoxStencil<Geo<ORD, KO>, Coef<ORD>> {
   }
 
   // LINK: a block of a batch (pool offset boff, link map lmap of its padded points). Cells in
-  // the block's interior are read in place; a ghost cell is read where the map points. Needs
-  // ng >= H (the box stays inside the padded block) and a pool below 2^29 doubles.
+  // the block's interior are read in place; a ghost cell is read where the map points, or in
+  // place if the entry would reach outside the pool (pool_n doubles). Needs ng >= H (the box
+  // stays inside the padded block) and a pool below 2^29 doubles.
   __device__ __forceinline__ WaveSrc(double* lds_, const double* pool, long boff,
-                                     const int* lmap, const Scales& sc_, const Dims& d, int ox,
-                                     int oy, int oz, int lane)
+                                     const int* lmap, long long pool_n, const Scales& sc_,
+                                     const Dims& d, int ox, int oy, int oz, int lane)
       : BoxStencil<G, C>(lds_, sc_), F(pool), Fc(pool + boff), P(d.P) {
     static_assert(MODE == LINK, "the link constructor is for LINK sources");
     const int tx = G::lx(lane), ty = G::ly(lane), tz = G::lz(lane);
@@ -372,8 +374,9 @@ struct WaveSrc : BoxStencil<Geo<ORD, KO>, Coef<ORD>> {
         psrc[k] = (int)d.P;
       } else {
         const int2 m = reinterpret_cast<const int2*>(lmap)[q];
-        off[k] = m.x;
-        psrc[k] = m.y;
+        const bool ok = link_entry_ok(m.x, m.y, pool_n);
+        off[k] = ok ? m.x : (int)boff + q;
+        psrc[k] = ok ? m.y : (int)d.P;
       }
     }
 #pragma unroll

@@ -55,6 +55,11 @@ __host__ __device__ inline bool prolong_tile_point_ok(const bssn_prolong_tile_po
   return q.local >= 0 && q.local < 64 && fields_in_pool(q.dst, q.dst_stride, pool_n);
 }
 
+// a link map entry: every field of the point it names lies in the pool
+__host__ __device__ inline bool link_entry_ok(int off, int stride, long long pool_n) {
+  return fields_in_pool(off, stride, pool_n);
+}
+
 __host__ __device__ inline bool block_job_ok(const bssn_prolong_block& j, long long pool_n) {
   if (j.BS < 1 || j.ng < 0 || j.BS + 2 * j.ng > PB_WMAX) return false;
   for (int a = 0; a < 3; ++a)

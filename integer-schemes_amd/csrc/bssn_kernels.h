@@ -121,6 +121,7 @@ struct BatchArgs {
   Scal base;         // shared scalars; cahd is set per block from its spacing
   double cahd_c, dt, sigma;
   const int* link;   // LINK mode: (source pool offset, source field stride) per padded point
+  long long pool_n;  // LINK mode: pool length in doubles (map entries outside it are not read)
 };
 
 hipError_t launch_fused_batch(const StageArgs& a, const BatchArgs& b, int grid, int ord, bool ko,

@@ -147,5 +147,9 @@ def test_entry_bounds_predicates():
                            block_job(P + 1, 0, (0, 0, 0), BS, ng, 0)])  # child past the end
     assert jobs.dtype == BLOCK_DTYPE and good.dtype == PROLONG_DTYPE
     assert count(2, jobs, 2 * P) == 3 and count(2, jobs[:1], 2 * P) == 0
-    assert lib.bssn_count_skipped_entries(5, None, 0, pool) == -1
+    assert lib.bssn_count_skipped_entries(6, None, 0, pool) == -1
+    # link map entries (pool offset of field 0, field stride): all 24 fields inside the pool
+    links = np.array([[0, 10], [pool - 23 * 10 - 1, 10], [pool - 23 * 10, 10], [-1, 10],
+                      [5, -1]], dtype=np.int32)
+    assert lib.bssn_count_skipped_entries(5, links.ctypes.data, len(links), pool) == 3
     assert lib.bssn_count_skipped_entries(0, None, 4, pool) == -1

@@ -173,4 +173,4 @@ def test_prolongation_tiles_cover_the_points():
     badq["local"][0] = 64
     badq["dst"][1] = pool
     assert count(4, badq) == 2
-    assert lib.bssn_count_skipped_entries(5, None, 0, pool) == -1
+    assert lib.bssn_count_skipped_entries(6, None, 0, pool) == -1
