@@ -108,19 +108,30 @@ __global__ __launch_bounds__(256) void k_prolong(double* __restrict__ pool,
   }
 }
 
-// One 4x4x4 fine tile and one field per 64-lane workgroup (blockIdx.y = field): the 7^3
-// coarse window of the tile (clamped into the parent, as the point kernel's taps) is read once
-// into LDS, then the x pass X[fx][cy][cz], the y pass Y[fx][fy][cz] and, at the tile's points
-// only, the z pass -- each sum over its taps in the point kernel's order, so every point gets
-// the value of its bssn_prolong_point entry bit for bit. A tile or point that would reach
-// outside the pool or the point array is skipped (the tile test is uniform over the group).
+// One 4x4x4 fine tile and PT_FIELDS consecutive fields per 64-lane workgroup (blockIdx.y =
+// field group). Per field, the 7^3 coarse window of the tile (clamped into the parent, as the
+// point kernel's taps) is written to LDS, then the x pass X[fx][cy][cz], the y pass
+// Y[fx][fy][cz] and, at the tile's points only, the z pass -- each sum over its taps in the
+// point kernel's order, so every point gets the value of its bssn_prolong_point entry bit for
+// bit. The window's parent offsets and the lane's tile point are formed once per tile; the next
+// field's window is loaded into registers while the current field's passes run. A tile or point
+// that would reach outside the pool or the point array is skipped (the tile test is uniform
+// over the group). Four fields per group: on the 640-block hierarchy's fill, 4 / 8 / 24 fields
+// per group take 1.122 / 1.133 / 1.179 ms per fill, one field per group 1.352 ms.
+#ifndef BSSN_PT_FIELDS
+#define BSSN_PT_FIELDS 4
+#endif
+constexpr int PT_FIELDS = BSSN_PT_FIELDS;
+static_assert(BSSN_NF % PT_FIELDS == 0, "field groups must tile the 24 fields");
+
 __global__ __launch_bounds__(64) void k_prolong_tile(double* __restrict__ pool,
                                                      const bssn_prolong_tile* __restrict__ tiles,
                                                      long ntiles,
                                                      const bssn_prolong_tile_point* __restrict__ tp,
                                                      long long ntp, long long pool_n) {
 #pragma clang fp contract(off)
-  __shared__ double W[343];
+  constexpr int NW = 343, NS = (NW + 63) / 64;     // window cells; per-lane slots
+  __shared__ double W[NW];
   __shared__ double X[4 * 49];
   __shared__ double Y[16 * 7];
   // XCD-aware: the dispatcher deals workgroups round-robin over the 8 XCDs (gridDim.x is a
@@ -130,63 +141,94 @@ __global__ __launch_bounds__(64) void k_prolong_tile(double* __restrict__ pool,
   const long ti = (w & 7) * (G8 >> 3) + (w >> 3);
   if (ti >= ntiles) return;
   const bssn_prolong_tile T = tiles[ti];
-  const int f = (int)blockIdx.y;
   const int lane = (int)threadIdx.x;
   if (!prolong_tile_ok(T, pool_n) || T.first + T.count > ntp) return;
+  const int f0 = (int)blockIdx.y * PT_FIELDS;
   const long sy = T.pS[2], sx = (long)T.pS[1] * T.pS[2];
   const long P = sx * T.pS[0];
-  const double* c = pool + T.parent + (long)f * P;
   const int c0[3] = {(T.origin[0] >> 1) - 2, (T.origin[1] >> 1) - 2, (T.origin[2] >> 1) - 2};
-  for (int i = lane; i < 343; i += 64) {
+  long woff[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const int i = lane + 64 * s;
     const int x = clampi(c0[0] + i / 49, 0, T.pS[0] - 1);
     const int y = clampi(c0[1] + (i / 7) % 7, 0, T.pS[1] - 1);
     const int z = clampi(c0[2] + i % 7, 0, T.pS[2] - 1);
-    W[i] = c[x * sx + y * sy + z];
+    woff[s] = x * sx + y * sy + z;
   }
-  __syncthreads();
-  for (int i = lane; i < 4 * 49; i += 64) {          // x pass
-    const int fx = i / 49, jk = i % 49;
-    const int fine = T.origin[0] + fx, kx = (fine >> 1) - c0[0];
-    double v;
-    if (fine & 1) {
-      v = 0.0;
-#pragma unroll
-      for (int t = 0; t < 6; ++t) v = v + kWMid[t] * W[(kx - 2 + t) * 49 + jk];
-    } else {
-      v = W[kx * 49 + jk];
-    }
-    X[i] = v;
+  // the lane's point of the tile's first 64 (a tile holds each of its 64 fine points once)
+  bssn_prolong_tile_point q0{};
+  bool ok0 = false;
+  if (lane < T.count) {
+    q0 = tp[T.first + lane];
+    ok0 = prolong_tile_point_ok(q0, pool_n);
   }
-  __syncthreads();
-  for (int i = lane; i < 16 * 7; i += 64) {          // y pass
-    const int fx = i / 28, fy = (i / 7) % 4, k = i % 7;
-    const int fine = T.origin[1] + fy, ky = (fine >> 1) - c0[1];
-    double v;
-    if (fine & 1) {
-      v = 0.0;
+  double wv[NS];
+  {
+    const double* c = pool + T.parent + (long)f0 * P;
 #pragma unroll
-      for (int t = 0; t < 6; ++t) v = v + kWMid[t] * X[fx * 49 + (ky - 2 + t) * 7 + k];
-    } else {
-      v = X[fx * 49 + ky * 7 + k];
-    }
-    Y[i] = v;
+    for (int s = 0; s < NS; ++s)
+      if (lane + 64 * s < NW) wv[s] = c[woff[s]];
   }
-  __syncthreads();
-  for (long long e = T.first + lane; e < T.first + T.count; e += 64) {   // z pass, scatter
-    const bssn_prolong_tile_point q = tp[e];
-    if (!prolong_tile_point_ok(q, pool_n)) continue;
-    const int fz = q.local & 3;
-    const double* yr = Y + (q.local >> 2) * 7;
-    const int fine = T.origin[2] + fz, kz = (fine >> 1) - c0[2];
-    double v;
-    if (fine & 1) {
-      v = 0.0;
+#pragma unroll 1
+  for (int f = f0; f < f0 + PT_FIELDS; ++f) {
 #pragma unroll
-      for (int t = 0; t < 6; ++t) v = v + kWMid[t] * yr[kz - 2 + t];
-    } else {
-      v = yr[kz];
+    for (int s = 0; s < NS; ++s)
+      if (lane + 64 * s < NW) W[lane + 64 * s] = wv[s];
+    __syncthreads();
+    if (f + 1 < f0 + PT_FIELDS) {                     // next field's window, in flight
+      const double* c = pool + T.parent + (long)(f + 1) * P;
+#pragma unroll
+      for (int s = 0; s < NS; ++s)
+        if (lane + 64 * s < NW) wv[s] = c[woff[s]];
     }
-    pool[q.dst + (long)f * q.dst_stride] = v;
+    for (int i = lane; i < 4 * 49; i += 64) {        // x pass
+      const int fx = i / 49, jk = i % 49;
+      const int fine = T.origin[0] + fx, kx = (fine >> 1) - c0[0];
+      double v;
+      if (fine & 1) {
+        v = 0.0;
+#pragma unroll
+        for (int t = 0; t < 6; ++t) v = v + kWMid[t] * W[(kx - 2 + t) * 49 + jk];
+      } else {
+        v = W[kx * 49 + jk];
+      }
+      X[i] = v;
+    }
+    __syncthreads();
+    for (int i = lane; i < 16 * 7; i += 64) {        // y pass
+      const int fx = i / 28, fy = (i / 7) % 4, k = i % 7;
+      const int fine = T.origin[1] + fy, ky = (fine >> 1) - c0[1];
+      double v;
+      if (fine & 1) {
+        v = 0.0;
+#pragma unroll
+        for (int t = 0; t < 6; ++t) v = v + kWMid[t] * X[fx * 49 + (ky - 2 + t) * 7 + k];
+      } else {
+        v = X[fx * 49 + ky * 7 + k];
+      }
+      Y[i] = v;
+    }
+    __syncthreads();
+    auto zpass = [&](const bssn_prolong_tile_point& q) {
+      const int fz = q.local & 3;
+      const double* yr = Y + (q.local >> 2) * 7;
+      const int fine = T.origin[2] + fz, kz = (fine >> 1) - c0[2];
+      double v;
+      if (fine & 1) {
+        v = 0.0;
+#pragma unroll
+        for (int t = 0; t < 6; ++t) v = v + kWMid[t] * yr[kz - 2 + t];
+      } else {
+        v = yr[kz];
+      }
+      pool[q.dst + (long)f * q.dst_stride] = v;
+    };
+    if (ok0) zpass(q0);
+    for (long long e = T.first + 64 + lane; e < T.first + T.count; e += 64) {
+      const bssn_prolong_tile_point q = tp[e];
+      if (prolong_tile_point_ok(q, pool_n)) zpass(q);
+    }
   }
 }
 
@@ -194,9 +236,9 @@ hipError_t launch_prolong_tiles(double* pool, long long pool_n, const void* tile
                                 const void* tp, long long ntp, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   const long g = (n + 7) & ~7L;
-  hipLaunchKernelGGL(k_prolong_tile, dim3((unsigned)g, BSSN_NF), dim3(64), 0, st, pool,
-                     (const bssn_prolong_tile*)tiles, n, (const bssn_prolong_tile_point*)tp, ntp,
-                     pool_n);
+  hipLaunchKernelGGL(k_prolong_tile, dim3((unsigned)g, BSSN_NF / PT_FIELDS), dim3(64), 0, st,
+                     pool, (const bssn_prolong_tile*)tiles, n, (const bssn_prolong_tile_point*)tp,
+                     ntp, pool_n);
   return hipGetLastError();
 }
 
