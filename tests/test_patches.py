@@ -114,6 +114,9 @@ def test_tile_program_equals_point_program(hier):
     V = U.clone()
     pts = DeviceProgram(*H.program_entries(), H.device, tiles=False)
     assert H.program.n_tile > 0 and pts.n_tile == 0
+    # tiles holding more than 64 entries (neighbouring blocks' ghosts at the same fine position
+    # each have one) take the kernel's second pass over the tile's points
+    assert int(H.program.tiles["count"].max()) > 64
     H.sync_(U)
     _lib.check(_lib.load().bssn_ghost_fill_f64(V.data_ptr(), V.numel(), pts.ptr,
                                                stream_handle(V.device)))
