@@ -288,12 +288,36 @@ def bench_batched(reps: int) -> dict:
     kw = dict(params=p, order=4, ko_sigma=0.1)
     tune = H.tune_launch(U0, dt, candidates=[0, 1024, 2048, 4096], reps=2, **kw)
     batch = H.batch
+    prog_shape = batch.persistent_waves
     U = U0.clone()
     OUT = H.empty_pool()
     batch.rhs_into(U, OUT, p, dt, 0.0, 4, 0.0)
     torch.cuda.synchronize()
     ms_rhs = timed(lambda: batch.rhs_into(U, OUT, p, dt, 0.0, 4, 0.0), reps, dev)
-    del OUT
+    # what the batched launch replaces: the single-grid RHS (bssn_rhs_f64) block by block, one
+    # launch per block, eagerly and with the 640 launches captured in one graph; every block's
+    # result must equal the batched one bit for bit
+    from bssn3d_hip.rhs import rhs_into
+    from bssn3d_hip.state import scalars
+    OUT2 = H.empty_pool()
+    per = [(H.view(U, b), H.view(OUT2, b), scalars(p, *H.blocks[b].h, dt, 0.0))
+           for b in range(len(H))]
+
+    def per_block():
+        for F, O, s16 in per:
+            rhs_into(F, O, s16, 4, 0.0)
+    per_block()
+    torch.cuda.synchronize()
+    assert all(torch.equal(H.view(OUT, b), H.view(OUT2, b)) for b in range(len(H))), \
+        "per-block RHS differs from the batched RHS"
+    ms_per_block = timed(per_block, reps, dev)
+    gpb = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(gpb):
+        per_block()
+    gpb.replay()
+    torch.cuda.synchronize()
+    ms_per_block_graph = timed(gpb.replay, reps, dev)
+    del gpb, per, OUT, OUT2
     ms_sync = timed(lambda: H.sync_(U), reps, dev)
     t = [0.0]
 
@@ -318,6 +342,9 @@ def bench_batched(reps: int) -> dict:
     linked = H.linked_program
     H.link
     lk = dict(kw, ghosts="linked")
+    # the linked step's stages read ghosts at their sources: tune its launch shape on its own
+    tune_link = H.tune_launch(U0, dt, candidates=[0, 1024, 2048, 4096], reps=2,
+                              sync_end=False, **lk)
     U.copy_(U0)
     H.step_(U, 0.0, dt, **lk)
     torch.cuda.synchronize()
@@ -341,14 +368,20 @@ def bench_batched(reps: int) -> dict:
             "ghost_program": {"copies": prog.n_copy, "prolongations": prog.n_prolong,
                               "fill_ms": round(ms_sync, 4)},
             "host_build_s": round(build_s, 2),
-            "launch_persistent_waves": batch.persistent_waves,
+            "launch_persistent_waves": prog_shape,
             "autotune_step_ms": {str(k): round(v, 4) for k, v in tune.items()},
             "rhs_ms": round(ms_rhs, 4), "rhs_mpts_s": round(pts / (ms_rhs * 1e-3) / 1e6, 2),
+            "per_block_rhs_ms": round(ms_per_block, 4),
+            "per_block_rhs_graph_ms": round(ms_per_block_graph, 4),
+            "batched_vs_per_block_rhs": round(ms_per_block / ms_rhs, 3),
+            "batched_vs_per_block_rhs_graph": round(ms_per_block_graph / ms_rhs, 3),
             "steps_timed": reps,
             "step_ms": round(ms_step, 4), "step_mpts_s": round(pts / (ms_step * 1e-3) / 1e6, 2),
             "graph_step_ms": round(ms_graph, 4),
             "graph_step_mpts_s": round(pts / (ms_graph * 1e-3) / 1e6, 2),
             "linked": {"program_copies": linked.n_copy, "prolongations": linked.n_prolong,
+                       "launch_persistent_waves": batch.persistent_waves,
+                       "autotune_step_ms": {str(k): round(v, 4) for k, v in tune_link.items()},
                        "step_ms": round(ms_link, 4),
                        "step_mpts_s": round(pts / (ms_link * 1e-3) / 1e6, 2),
                        "graph_step_ms": round(ms_link_graph, 4),

@@ -312,7 +312,8 @@ class BlockBatch(BlockLayout):
                  halo: "HaloMap | None" = None, candidates: Iterable[int] | None = None,
                  reps: int = 3) -> dict:
         """Time one step per launch shape (0 = one workgroup per brick, N = N persistent waves)
-        on a copy of U and keep the fastest. Returns {shape: ms}."""
+        on a copy of U and keep the fastest (the first candidate unless another beats it by more
+        than the timing noise, pick_launch). Returns {shape: ms}."""
         if candidates is None:
             cus = torch.cuda.get_device_properties(self.device).multi_processor_count
             candidates = [0] + [cus * k for k in (4, 8, 16)]
@@ -331,8 +332,7 @@ class BlockBatch(BlockLayout):
             e1.record()
             torch.cuda.synchronize(self.device)
             res[c] = e0.elapsed_time(e1) / reps
-        best = min(res, key=res.get)
-        self.set_launch(best)
+        self.set_launch(pick_launch(res))
         del V
         return res
 
@@ -459,6 +459,19 @@ def fill_punctures(batch: BlockBatch, pool: torch.Tensor, masses=(0.909, 0.0909)
     positions = positions if positions is not None else puncture_positions(seed, masses)
     batch.fill(pool, lambda b, blk, grid: punctures(grid, list(masses), positions, amp=amp,
                                                      seed=seed + 1000 + b, device=pool.device))
+
+
+# a launch shape other than the first candidate must beat it by more than this fraction (the
+# run-to-run noise of a few timed steps), so that near-ties do not flip the shape between runs
+LAUNCH_MARGIN = 0.005
+
+
+def pick_launch(res: dict, margin: float = LAUNCH_MARGIN) -> int:
+    """The launch shape to keep from {shape: ms} (in the order timed): the first one unless
+    another is faster by more than ``margin``, then the fastest."""
+    first = next(iter(res))
+    best = min(res, key=res.get)
+    return best if res[best] < res[first] * (1.0 - margin) else first
 
 
 def timed(fn, reps: int, device) -> float:

@@ -153,3 +153,14 @@ def test_entry_bounds_predicates():
                       [5, -1]], dtype=np.int32)
     assert lib.bssn_count_skipped_entries(5, links.ctypes.data, len(links), pool) == 3
     assert lib.bssn_count_skipped_entries(0, None, 4, pool) == -1
+
+
+def test_pick_launch_keeps_the_first_shape_within_the_noise():
+    """A launch shape replaces the first candidate only when it is faster by more than the
+    margin; among several such, the fastest wins."""
+    from bssn3d_hip.batch import LAUNCH_MARGIN, pick_launch
+    assert LAUNCH_MARGIN == 0.005
+    assert pick_launch({0: 10.0, 1024: 9.96, 2048: 10.3}) == 0          # 0.4 % faster: noise
+    assert pick_launch({0: 10.0, 1024: 9.9, 2048: 9.8}) == 2048          # 2 % faster
+    assert pick_launch({1024: 10.0, 0: 10.01}) == 1024                   # first in timing order
+    assert pick_launch({0: 10.0, 1024: 9.9}, margin=0.02) == 0
