@@ -451,15 +451,19 @@ def bench_amr(reps: int) -> dict:
     def step():
         H.step_(U, t[0], dt, p, 4, 0.1)
         t[0] += dt
+    kw = dict(params=p, order=4, ko_sigma=0.1)
+    tune = H.tune_launch(U, dt, candidates=[0, 1024, 2048], reps=3, **kw)
+    prog_shape = H.batch.persistent_waves
     step()
     torch.cuda.synchronize()
     ms_step = timed(step, reps, dev)
     assert bool(torch.isfinite(U).all()), "non-finite hierarchy state"
     ms_sync = timed(lambda: H.sync_(U), reps, dev)
     # linked steps: face-copied ghosts read at their sources, only the prolongations (and the
-    # copies their taps read) between stages
+    # copies their taps read) between stages; launch shape tuned on its own
     H.linked_program, H.link
-    lk = dict(params=p, order=4, ko_sigma=0.1, ghosts="linked", sync_end=False)
+    lk = dict(kw, ghosts="linked", sync_end=False)
+    tune_link = H.tune_launch(U, dt, candidates=[0, 1024, 2048], reps=3, **lk)
     H.step_(U, 0.0, dt, **lk)
     torch.cuda.synchronize()
     ms_link = timed(lambda: H.step_(U, 0.0, dt, **lk), reps, dev)
@@ -483,10 +487,15 @@ def bench_amr(reps: int) -> dict:
     return {"workload": "bl_binary_amr_3lev_root4x4x4_bs16_box16",
             "blocks_per_level": counts, "interior_points": pts,
             "ghost_program": {"prolongations": prog.n_prolong, "copies": prog.n_copy},
+            "launch_persistent_waves": prog_shape,
+            "autotune_step_ms": {str(k): round(v, 4) for k, v in tune.items()},
             "steps_timed": reps, "step_ms": round(ms_step, 4),
             "step_mpts_s": round(pts / (ms_step * 1e-3) / 1e6, 2),
             "ghost_fill_ms": round(ms_sync, 4),
-            "linked": {"copies": H.linked_program.n_copy, "step_ms": round(ms_link, 4),
+            "linked": {"copies": H.linked_program.n_copy,
+                       "launch_persistent_waves": H.batch.persistent_waves,
+                       "autotune_step_ms": {str(k): round(v, 4) for k, v in tune_link.items()},
+                       "step_ms": round(ms_link, 4),
                        "step_mpts_s": round(pts / (ms_link * 1e-3) / 1e6, 2)},
             "regrid_refine_pass_ms": round(ms_regrid, 1)}
 

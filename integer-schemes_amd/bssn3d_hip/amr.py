@@ -38,7 +38,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .batch import HALO_DTYPE, Block, BlockBatch, _device
+from .batch import HALO_DTYPE, Block, BlockBatch, _device, tune_step_launch
 from .grid import Grid
 from .rhs import stream_handle
 from .state import NUM_VARS, PhysicsParams
@@ -838,6 +838,15 @@ class AMRHierarchy:
             _lib.dbl(b.S16(params, dt, t)), int(order), float(ko_sigma), prog.ptr, float(t),
             float(dt), params.chi_floor, params.alpha_floor, tptr, stream_handle(self.device)))
         return U
+
+    def tune_launch(self, U: torch.Tensor, dt: float, candidates=None, reps: int = 3,
+                    **kw) -> dict:
+        """Time one hierarchy step per batched launch shape on a copy of U and keep the fastest
+        (the first candidate unless another beats it by more than the timing noise,
+        batch.tune_step_launch); ``kw`` goes to step_ (ghosts="linked" tunes the linked step).
+        Returns {shape: ms} (0 = one workgroup per brick, N = N persistent waves)."""
+        return tune_step_launch(self.batch, lambda V: self.step_(V, 0.0, dt, **kw), U,
+                                candidates, reps)
 
     def capture_steps(self, U: torch.Tensor, tdev: torch.Tensor, dt: float, nsteps: int = 1,
                       **kw) -> torch.cuda.CUDAGraph:

@@ -314,27 +314,8 @@ class BlockBatch(BlockLayout):
         """Time one step per launch shape (0 = one workgroup per brick, N = N persistent waves)
         on a copy of U and keep the fastest (the first candidate unless another beats it by more
         than the timing noise, pick_launch). Returns {shape: ms}."""
-        if candidates is None:
-            cus = torch.cuda.get_device_properties(self.device).multi_processor_count
-            candidates = [0] + [cus * k for k in (4, 8, 16)]
-        V = U.clone()
-        res = {}
-        for c in candidates:
-            c = int(c) // 8 * 8
-            self.set_launch(c)
-            self.step_(V, 0.0, dt, params, order, ko_sigma, bc, halo)
-            torch.cuda.synchronize(self.device)
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record()
-            for _ in range(reps):
-                self.step_(V, 0.0, dt, params, order, ko_sigma, bc, halo)
-            e1.record()
-            torch.cuda.synchronize(self.device)
-            res[c] = e0.elapsed_time(e1) / reps
-        self.set_launch(pick_launch(res))
-        del V
-        return res
+        return tune_step_launch(self, lambda V: self.step_(V, 0.0, dt, params, order, ko_sigma,
+                                                           bc, halo), U, candidates, reps)
 
     def capture_steps(self, U: torch.Tensor, tdev: torch.Tensor, dt: float, nsteps: int = 1,
                       params: PhysicsParams | None = None, order: int = 4,
@@ -472,6 +453,34 @@ def pick_launch(res: dict, margin: float = LAUNCH_MARGIN) -> int:
     first = next(iter(res))
     best = min(res, key=res.get)
     return best if res[best] < res[first] * (1.0 - margin) else first
+
+
+def tune_step_launch(batch: "BlockBatch", step, U: torch.Tensor, candidates=None,
+                     reps: int = 3) -> dict:
+    """Time step(V) for each launch shape of ``batch`` (0 = one workgroup per brick, N = N
+    persistent waves; default 0 and 4, 8, 16 waves per CU) on a copy V of U, set the shape
+    pick_launch keeps and return {shape: ms}."""
+    if candidates is None:
+        cus = torch.cuda.get_device_properties(batch.device).multi_processor_count
+        candidates = [0] + [cus * k for k in (4, 8, 16)]
+    V = U.clone()
+    res = {}
+    for c in candidates:
+        c = int(c) // 8 * 8
+        batch.set_launch(c)
+        step(V)
+        torch.cuda.synchronize(batch.device)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            step(V)
+        e1.record()
+        torch.cuda.synchronize(batch.device)
+        res[c] = e0.elapsed_time(e1) / reps
+    batch.set_launch(pick_launch(res))
+    del V
+    return res
 
 
 def timed(fn, reps: int, device) -> float:

@@ -37,7 +37,7 @@ import torch
 from . import _lib
 from .amr import (BLOCK_DTYPE, PROLONG_DTYPE, DeviceProgram, by_parity, link_map,
                   linked_entries)
-from .batch import HALO_DTYPE, Block, BlockBatch, BlockLayout, pick_launch
+from .batch import HALO_DTYPE, Block, BlockBatch, BlockLayout, tune_step_launch
 from .rhs import stream_handle
 from .state import PhysicsParams
 
@@ -335,30 +335,11 @@ class PatchHierarchy:
     def tune_launch(self, U: torch.Tensor, dt: float, candidates=None, reps: int = 3,
                     **kw) -> dict:
         """Time one step per batched launch shape on a copy of U and keep the fastest (the first
-        candidate unless another beats it by more than the timing noise, batch.pick_launch);
+        candidate unless another beats it by more than the timing noise, batch.tune_step_launch);
         ``kw`` goes to step_, so ghosts="linked" tunes the linked step. Returns {shape: ms}
         (0 = one workgroup per brick, N = N persistent waves)."""
-        if candidates is None:
-            cus = torch.cuda.get_device_properties(self.device).multi_processor_count
-            candidates = [0] + [cus * k for k in (4, 8, 16)]
-        V = U.clone()
-        res = {}
-        for c in candidates:
-            c = int(c) // 8 * 8
-            self.batch.set_launch(c)
-            self.step_(V, 0.0, dt, **kw)
-            torch.cuda.synchronize(self.device)
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record()
-            for _ in range(reps):
-                self.step_(V, 0.0, dt, **kw)
-            e1.record()
-            torch.cuda.synchronize(self.device)
-            res[c] = e0.elapsed_time(e1) / reps
-        self.batch.set_launch(pick_launch(res))
-        del V
-        return res
+        return tune_step_launch(self.batch, lambda V: self.step_(V, 0.0, dt, **kw), U,
+                                candidates, reps)
 
 
 # --- the workload -----------------------------------------------------------------------------
