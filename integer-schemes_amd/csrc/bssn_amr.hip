@@ -116,8 +116,9 @@ __global__ __launch_bounds__(256) void k_prolong(double* __restrict__ pool,
 // bit. The window's parent offsets and the lane's tile point are formed once per tile; the next
 // field's window is loaded into registers while the current field's passes run. A tile or point
 // that would reach outside the pool or the point array is skipped (the tile test is uniform
-// over the group). Four fields per group: on the 640-block hierarchy's fill, 4 / 8 / 24 fields
-// per group take 1.122 / 1.133 / 1.179 ms per fill, one field per group 1.352 ms.
+// over the group). Four fields per group: on the 640-block hierarchy's fill, 2 / 3 / 4 / 6 / 8 /
+// 24 fields per group take 1.134 / 1.124 / 1.121 / 1.124 / 1.133 / 1.179 ms per fill, one field
+// per group 1.352 ms.
 #ifndef BSSN_PT_FIELDS
 #define BSSN_PT_FIELDS 4
 #endif
