@@ -269,7 +269,13 @@ struct BoxStencil {
     const int sj = stride(j);
     double acc = 0.0;
 #pragma unroll
-    for (int k = 1; k <= G::R; ++k) acc += C::d1(k) * (n1(c + k * sj, i) - n1(c - k * sj, i));
+    for (int k = 1; k <= G::R; ++k) {
+      acc += C::d1(k) * (n1(c + k * sj, i) - n1(c - k * sj, i));
+      // order 8: 64 box reads per mixed derivative; issued all at once they hold 128 registers
+      // and the stage kernels spill (up to 404 B per lane). One tap pair at a time: RK4 step
+      // at 128^3 8.49 -> 7.39 ms. The sums and their order are unchanged.
+      if constexpr (G::R >= 4) __builtin_amdgcn_sched_barrier(0);
+    }
     return pin(acc * sc.sm[i + j - 1]);
   }
 
