@@ -42,7 +42,9 @@ __global__ __launch_bounds__(256) void k_sommerfeld(StageArgs a, Dims d, Scales 
   double r = sqrt(X[0] * X[0] + X[1] * X[1] + X[2] * X[2]);
   r = r == 0.0 ? 1.0 : r;
   StageOut<KIND> out(a, d.P, p, true);
-#pragma unroll 1
+  // unrolled, so that the write-back operands and the final stage's metric are indexed by
+  // constants and stay in registers (a run-time field index put them in scratch: 352-896 B)
+#pragma unroll
   for (int f = 0; f < BSSN_NF; ++f) {
     const double* u = a.Yin + (long)f * d.P + p;
     double du[3];
