@@ -237,6 +237,7 @@ hipError_t launch_prolong_tiles(double* pool, long long pool_n, const void* tile
                                 const void* tp, long long ntp, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   const long g = (n + 7) & ~7L;
+  if (g > 0x7fffffffL) return hipErrorInvalidValue;       // beyond one grid dimension
   hipLaunchKernelGGL(k_prolong_tile, dim3((unsigned)g, BSSN_NF / PT_FIELDS), dim3(64), 0, st,
                      pool, (const bssn_prolong_tile*)tiles, n, (const bssn_prolong_tile_point*)tp,
                      ntp, pool_n);
@@ -359,6 +360,7 @@ __global__ __launch_bounds__(BSSN_PB_THREADS) void k_prolong_block(
 hipError_t launch_prolong_blocks(double* pool, long long pool_n, const void* jobs, long n,
                                  hipStream_t st) {
   if (n <= 0) return hipSuccess;
+  if (n > 0x7fffffffL / (BSSN_NF * PB_SLABS)) return hipErrorInvalidValue;   // grid too large
   hipLaunchKernelGGL(k_prolong_block, dim3((unsigned)(n * BSSN_NF * PB_SLABS)),
                      dim3(BSSN_PB_THREADS), 0, st, pool, (const bssn_prolong_block*)jobs, pool_n);
   return hipGetLastError();
