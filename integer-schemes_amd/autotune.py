@@ -7,7 +7,8 @@ The stage kernels' schedule is a set of compile-time knobs (bssn_params.h, bssn_
 bssn_stage.h): the brick of output points one wave owns (BSSN_BRICK_X/Y/Z, 64 points: the
 4x4x4 cube stages the fewest halo cells, the flatter bricks longer z runs), the LDS box's
 x-plane stride (BSSN_BOX_XPLANE_MOD32: dense, or padded to a residue modulo 32 doubles that
-spreads the brick's lanes over the banks), how many fields are staged ahead of the one being
+spreads the brick's lanes over the banks), the order in which an XCD's workgroups take their
+bricks (BSSN_BRICK_TILE_XY: z columns in square tiles of columns, or 0 for rows of columns), how many fields are staged ahead of the one being
 evaluated (BSSN_STAGE_DEPTH for the RK/MSRK stage kernels, BSSN_RHS_DEPTH for the bare RHS),
 how many algebra phases ahead of its store a field's write-back operands are loaded
 (BSSN_WB_LEAD), and how many waves per SIMD the register allocator targets (BSSN_STAGE_WAVES,
@@ -62,10 +63,15 @@ SPACES = {
         {"BSSN_BRICK_X": 2, "BSSN_BRICK_Y": 8, "BSSN_BRICK_Z": 4},
         {"BSSN_BRICK_X": 8, "BSSN_BRICK_Y": 4, "BSSN_BRICK_Z": 2},
         {"BSSN_BRICK_X": 2, "BSSN_BRICK_Y": 2, "BSSN_BRICK_Z": 16},
+        {"BSSN_BRICK_TILE_XY": 0},
+        {"BSSN_BRICK_TILE_XY": 2},
+        {"BSSN_BRICK_TILE_XY": 8},
     ],
 }
-# the brick shapes alone (with the defaults as the baseline every space starts with)
+# the brick shapes alone, and the brick orders alone (with the defaults as the baseline every
+# space starts with)
 SPACES["bricks"] = [{}] + [d for d in SPACES["full"] if "BSSN_BRICK_X" in d]
+SPACES["order"] = [{}] + [d for d in SPACES["full"] if "BSSN_BRICK_TILE_XY" in d]
 
 # Runs in the candidate's process: time the step and the RHS, compare the 48^3 step with the
 # installed library's result (written by the parent).
